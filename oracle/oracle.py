@@ -1,0 +1,327 @@
+"""ctypes access to the CPU oracle (liboracle.so) and the reference executable.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product never imports it.
+
+A run configuration is a dict of reference command-line keys ("init.type",
+"grid.Pres", "solver.dt", ...) to a value or a list of values, exactly as the
+reference's paramsParser reads them (paramsParser.hxx:176-193).  ``resolve``
+applies the reference's defaults (vlasovSolver.hxx:61-91, the generators'
+defaultGridParams/defaultSolverParams/setupParameters) and returns the oracle's
+parameter struct.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_EXE = HERE / "_ref" / "vlasolve"
+
+SCALES = {"linear": 0, "logarithmic": 1, "quadratic": 2, "cubic": 3, "quartic": 4}
+INITS = {"UDF": 0, "PLDF": 1, "Hernquist": 2}
+KERNELS = {"delayed": 0, "reflective": 1}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("Pmin", C.c_double), ("Pmax", C.c_double), ("Pres", C.c_int), ("Pscale", C.c_int),
+        ("Umin", C.c_double * 2), ("Umax", C.c_double * 2), ("Ures", C.c_int * 2),
+        ("Uscale", C.c_int * 2),
+        ("G", C.c_double), ("dt", C.c_double), ("noLinking", C.c_int), ("kernelType", C.c_int),
+        ("kernelBufferSize", C.c_int),
+        ("initType", C.c_int),
+        ("M", C.c_double), ("R0", C.c_double), ("alpha", C.c_double),
+        ("smoothingRadius", C.c_double), ("gap0", C.c_double), ("gap1", C.c_double),
+        ("eta", C.c_double), ("n", C.c_double), ("Ra", C.c_double),
+        ("nthreads", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        d = C.c_double
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [P(Params)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_dims.argtypes = [C.c_void_p, P(C.c_long)]
+        L.orc_nval.restype = C.c_long
+        L.orc_nval.argtypes = [C.c_void_p]
+        for name in ("orc_get_f", "orc_set_f"):
+            getattr(L, name).argtypes = [C.c_void_p, P(d)]
+        L.orc_get_axes.argtypes = [C.c_void_p, P(d), P(d), P(d), P(d)]
+        L.orc_kernel_radius.restype = d
+        L.orc_kernel_radius.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_last_moments.argtypes = [C.c_void_p, P(d), P(d), P(d)]
+        L.orc_moments.argtypes = [C.c_void_p, P(d), P(d), P(d)]
+        L.orc_stats.argtypes = [C.c_void_p, P(d)]
+        L.orc_get_feet.argtypes = [C.c_void_p, P(d), P(d), P(C.c_uint8)]
+        L.orc_num_chains.restype = C.c_long
+        L.orc_num_chains.argtypes = [C.c_void_p]
+        L.orc_chain_cells.restype = C.c_long
+        L.orc_chain_cells.argtypes = [C.c_void_p]
+        L.orc_get_chains.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int32), P(d),
+                                     P(d), P(d)]
+        L.orc_spline_build.argtypes = [C.c_void_p, P(d), P(d)]
+        L.orc_spline_evalC.restype = d
+        L.orc_spline_evalC.argtypes = [C.c_void_p, P(d), C.c_int, d, C.c_int, d]
+        L.orc_spline_evalNxPy.restype = d
+        L.orc_spline_evalNxPy.argtypes = [C.c_void_p, P(d), C.c_int, d]
+        L.orc_spline_eval.restype = d
+        L.orc_spline_eval.argtypes = [C.c_void_p, P(d), d, d]
+        L.orc_advance_drift.argtypes = [d, d, d, d, d, P(d), P(d)]
+        L.orc_gen_scale.restype = C.c_long
+        L.orc_gen_scale.argtypes = [d, d, C.c_long, C.c_int, C.c_int, P(d)]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+# ---------------------------------------------------------------------------
+# configuration handling
+
+def _vals(cfg, key):
+    v = cfg.get(key)
+    if v is None:
+        return None
+    return list(v) if isinstance(v, (list, tuple)) else [v]
+
+
+def _get(cfg, key, default, which=0, conv=float):
+    v = _vals(cfg, key)
+    if v is None or len(v) <= which:
+        return default
+    return conv(v[which])
+
+
+def _bool(x):
+    return bool(int(x)) if str(x).lstrip("-").isdigit() else str(x).lower() == "true"
+
+
+def resolve(cfg: dict, nthreads: int = 1) -> Params:
+    """Resolve a reference-style configuration into oracle parameters."""
+    p = Params()
+    itype = _get(cfg, "init.type", "UDF", conv=str)
+    if itype not in INITS:
+        raise ValueError(f"unknown initial conditions type {itype}")
+    p.initType = INITS[itype]
+    # defaultSolverParams (UDF :104-112, PLDF :127-135, Hernquist :77-85)
+    p.G = _get(cfg, "solver.G", 1.0)
+    p.dt = _get(cfg, "solver.dt", 0.025 if itype == "Hernquist" else 0.005)
+    p.noLinking = int(_get(cfg, "solver.noLinking", False, conv=_bool))
+    p.kernelType = KERNELS[_get(cfg, "solver.kernelType", "delayed", conv=str)]
+    p.kernelBufferSize = _get(cfg, "solver.kernelBufferSize", 0, conv=int)
+    p.nthreads = nthreads
+    # generator parameters
+    if itype == "UDF":  # initGen_UniformDensitySphere.hxx:49-62
+        p.R0 = _get(cfg, "init.R0", 2.0)
+        p.M = _get(cfg, "init.M", 1.0)
+        p.alpha = _get(cfg, "init.alpha", 0.5)
+        p.smoothingRadius = _get(cfg, "init.smoothingRadius", 0.5)
+        p.gap0 = _get(cfg, "init.gap", 0.0, 0)
+        p.gap1 = _get(cfg, "init.gap", 0.0, 1)
+        grid = dict(Pmin=0.01, Pmax=25.0, Umin=[-2.0, 0.0], Umax=[2.0, 1.6], Pres=200,
+                    Ures=[200, 200], Pscale=1, Uscale=[0, 2])
+    elif itype == "PLDF":  # initGen_PowerLawDensitySphere.hxx:50-80
+        p.M = _get(cfg, "init.M", 1.0)
+        p.alpha = _get(cfg, "init.alpha", 2.0)
+        p.eta = _get(cfg, "init.eta", 0.1)
+        p.n = _get(cfg, "init.n", 1.0)
+        p.smoothingRadius = _get(cfg, "init.smoothingRadius", 0.5)
+        p.R0 = _get(cfg, "init.R0", 1.0 / p.eta)
+        n = p.n
+        sigO2 = (3.0 - n) / (5.0 - 2 * n) * (p.M * p.eta) / (p.R0 * (p.alpha * 0.5 + 1))
+        sigR2 = sigO2 * 0.5 * p.alpha
+        sigma2 = max(0.0, sigO2 + 2 * sigR2)
+        grid = dict(Pmin=0.01, Pmax=200.0, Umin=[-16 * math.sqrt(sigma2), 0.0],
+                    Umax=[16 * math.sqrt(sigma2), 1.6], Pres=200, Ures=[200, 200], Pscale=1,
+                    Uscale=[0, 2])
+    else:  # initGen_Hernquist.hxx:38-75
+        p.R0 = _get(cfg, "init.R0", 1.0)
+        p.M = _get(cfg, "init.M", 1.0)
+        p.Ra = _get(cfg, "init.Ra", -1.0)
+        grid = dict(Pmin=0.01, Pmax=250.0, Umin=[-1.45, 1.0e-4], Umax=[1.45, 3.24 * 4],
+                    Pres=90, Ures=[80, 80], Pscale=1, Uscale=[0, 1])
+    # gridHandler overrides (grid_handler.hxx:221-234)
+    p.Pmin = _get(cfg, "grid.Pmin", grid["Pmin"])
+    p.Pmax = _get(cfg, "grid.Pmax", grid["Pmax"])
+    p.Pres = _get(cfg, "grid.Pres", grid["Pres"], conv=int)
+    p.Pscale = SCALES[_get(cfg, "grid.Pscale", None, conv=str)] if _vals(cfg, "grid.Pscale") \
+        else grid["Pscale"]
+    for i in range(2):
+        p.Umin[i] = _get(cfg, "grid.Umin", grid["Umin"][i], i)
+        p.Umax[i] = _get(cfg, "grid.Umax", grid["Umax"][i], i)
+        p.Ures[i] = _get(cfg, "grid.Ures", grid["Ures"][i], i, conv=int)
+        v = _vals(cfg, "grid.Uscale")
+        p.Uscale[i] = SCALES[str(v[i])] if v and len(v) > i else grid["Uscale"][i]
+    return p
+
+
+def to_argv(cfg: dict) -> list[str]:
+    """Reference-style command line for a configuration."""
+    out = []
+    for k, v in cfg.items():
+        out.append("-" + k)
+        vals = list(v) if isinstance(v, (list, tuple)) else [v]
+        out += [repr(x) if isinstance(x, float) else str(x) for x in vals]
+    return out
+
+
+# ---------------------------------------------------------------------------
+class Oracle:
+    """One oracle run (initial conditions, feet, delay lines, renormalized f)."""
+
+    def __init__(self, cfg: dict, nthreads: int = 1):
+        self.params = resolve(cfg, nthreads)
+        L = lib()
+        self._h = L.orc_create(C.byref(self.params))
+        if not self._h:
+            raise RuntimeError("oracle: " + L.orc_last_error().decode())
+        dims = (C.c_long * 3)()
+        L.orc_dims(self._h, dims)
+        self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
+        self.shape = (self.nJ, self.nU, self.nR)
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        lib().orc_step(self._h, int(n))
+
+    def f(self):
+        out = np.empty(self.shape, np.float64)
+        lib().orc_get_f(self._h, _ptr(out))
+        return out
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, np.float64)
+        assert a.size == self.nR * self.nU * self.nJ
+        lib().orc_set_f(self._h, _ptr(a))
+
+    def axes(self):
+        R = np.empty(self.nR)
+        U = np.empty(self.nU)
+        J = np.empty(self.nJ)
+        JV = np.empty(self.nJ + 1)
+        lib().orc_get_axes(self._h, _ptr(R), _ptr(U), _ptr(J), _ptr(JV))
+        return R, U, J, JV
+
+    def kernel_radius(self):
+        return lib().orc_kernel_radius(self._h)
+
+    def moments(self):
+        dM = np.empty(self.nR)
+        M = np.empty(self.nR)
+        t = C.c_double()
+        lib().orc_moments(self._h, _ptr(dM), _ptr(M), C.byref(t))
+        return dM, M, t.value
+
+    def last_moments(self):
+        dM = np.empty(self.nR)
+        a = np.empty(self.nR)
+        t = C.c_double()
+        lib().orc_last_moments(self._h, _ptr(dM), _ptr(a), C.byref(t))
+        return dM, a, t.value
+
+    def stats(self):
+        out = np.empty(7)
+        lib().orc_stats(self._h, _ptr(out))
+        return dict(zip(("M", "E", "Ek", "Ep", "S", "L1", "L2"), out))
+
+    def feet(self):
+        n = self.nR * self.nU * self.nJ
+        R = np.empty(n)
+        U = np.empty(n)
+        fl = np.empty(n, np.uint8)
+        lib().orc_get_feet(self._h, _ptr(R), _ptr(U), _ptr(fl, C.c_uint8))
+        return R.reshape(self.shape), U.reshape(self.shape), fl.reshape(self.shape)
+
+    def chains(self):
+        L = lib()
+        nc = L.orc_num_chains(self._h)
+        ncell = L.orc_chain_cells(self._h)
+        point = np.empty(nc, np.int64)
+        off = np.empty(nc, np.int64)
+        ln = np.empty(nc, np.int32)
+        R = np.empty(ncell)
+        U = np.empty(ncell)
+        V = np.empty(ncell)
+        L.orc_get_chains(self._h, _ptr(point, C.c_int64), _ptr(off, C.c_int64),
+                         _ptr(ln, C.c_int32), _ptr(R), _ptr(U), _ptr(V))
+        return dict(point=point, off=off, len=ln, R=R, U=U, val=V)
+
+    def spline_build(self, slice2d):
+        a = np.ascontiguousarray(slice2d, np.float64)
+        assert a.shape == (self.nU, self.nR)
+        coef = np.zeros((self.nU + 2) * (self.nR + 2))
+        lib().orc_spline_build(self._h, _ptr(a), _ptr(coef))
+        return coef.reshape(self.nU + 2, self.nR + 2)
+
+    def spline_evalC(self, coef, Nx, x, Ny, y):
+        c = np.ascontiguousarray(coef, np.float64)
+        return lib().orc_spline_evalC(self._h, _ptr(c), int(Nx), float(x), int(Ny), float(y))
+
+    def spline_evalNxPy(self, coef, Nx, yy):
+        c = np.ascontiguousarray(coef, np.float64)
+        return lib().orc_spline_evalNxPy(self._h, _ptr(c), int(Nx), float(yy))
+
+    def spline_eval(self, coef, xx, yy):
+        c = np.ascontiguousarray(coef, np.float64)
+        return lib().orc_spline_eval(self._h, _ptr(c), float(xx), float(yy))
+
+
+def advance_drift(R, U, J, dt, R0=-1.0):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_advance_drift(R, U, J, dt, R0, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def gen_scale(start, stop, N, sct, cell=False):
+    out = np.empty(N + 2)
+    n = lib().orc_gen_scale(start, stop, N, sct, int(bool(cell)), _ptr(out))
+    return out[:n].copy()
+
+
+# ---------------------------------------------------------------------------
+def run_reference(cfg: dict, workdir, threads: int = 1, timeout: float = 600.0):
+    """Run the reference executable (oracle/_ref/vlasolve) in ``workdir``.
+
+    Returns its stdout.  Raises if the executable was not built."""
+    if not REF_EXE.exists():
+        raise FileNotFoundError(str(REF_EXE))
+    workdir = Path(workdir)
+    workdir.mkdir(parents=True, exist_ok=True)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    argv = [str(REF_EXE)] + to_argv(cfg) + ["-threads_per_node", str(threads)]
+    res = subprocess.run(argv, cwd=workdir, env=env, capture_output=True, text=True,
+                         timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference failed ({res.returncode}): {res.stderr[-2000:]}")
+    return res.stdout

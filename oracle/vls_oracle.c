@@ -1,0 +1,1039 @@
+/*
+ * vls_oracle.c -- CPU restatement of vlasolve's spherically symmetric
+ * semi-Lagrangian spline time step (vlasovSolver21_spline::makeOneStep).
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the parity checker.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product
+ * never links or calls it.
+ *
+ * Build with -O2 -ffp-contract=off and no -march flag: the reference is
+ * compiled for baseline x86-64, which has no FMA, and every expression below
+ * keeps the reference's evaluation order, so the results are bit-identical to
+ * the reference's at the same reduction order.
+ *
+ * Pins (each function is checked by tests/test_oracle_*.py):
+ *  - whole trajectories, feet, delay-line chains, scales and initial
+ *    conditions: bit-identical to the reference executable built from its own
+ *    sources (oracle/_ref, fixtures in tests/golden/) at nthreads=1;
+ *  - spline build/evaluate: interpolation at nodes, exact reproduction of
+ *    linear data, zero second derivative at natural ends, a dense
+ *    brute-force solve of the same system, scipy's natural CubicSpline;
+ *  - feet: conservation of H=(u^2+J^2/r^2)/2 and r*u - 2Ht, RK4 integration;
+ *  - quadrature/statistics: closed forms for constant f, statistics.txt
+ *    of the reference (10 significant digits).
+ */
+#include "vls_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static char g_err[512];
+const char *orc_last_error(void) { return g_err; }
+static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
+
+/* ------------------------------------------------------------------ */
+/* integer power with the reference's semantics.  The reference is C++98:
+ * pow(double,int) resolves to __builtin_powi (libstdc++ <cmath>), i.e. the
+ * repeated-squaring product below (libgcc __powidf2 / LLVM powi expansion).
+ * Used where the reference writes pow(x, <int>), e.g. pow(R0,3)
+ * (initGen_UniformDensitySphere.hxx:65) and pow(scale[i],pp) (scale.hxx:136). */
+static double powi_ref(double x, int m)
+{
+  unsigned n = m < 0 ? -(unsigned)m : (unsigned)m;
+  double y = (n % 2) ? x : 1.0;
+  while (n >>= 1) {
+    x = x * x;
+    if (n % 2) y = y * x;
+  }
+  return m < 0 ? 1.0 / y : y;
+}
+
+/* scale::genScale, scale.hxx:114-151 */
+long orc_gen_scale(double start, double stop, long N, int sct, int cell, double *out)
+{
+  long i;
+  if (sct == ORC_LINEAR) {
+    double dx = (stop - start) / (N);
+    for (i = 0; i < N + 1; i++) out[i] = start + dx * i;
+  } else if (sct == ORC_LOG) {
+    double s = (start < 0) ? -1 : 1;
+    orc_gen_scale(log(fabs(start)), log(fabs(stop)), N, ORC_LINEAR, 0, out);
+    for (i = 0; i < N + 1; i++) out[i] = s * exp(out[i]);
+  } else {
+    int pp = sct;
+    orc_gen_scale(pow(start, 1. / pp), pow(stop, 1. / pp), N, ORC_LINEAR, 0, out);
+    for (i = 0; i < N + 1; i++) out[i] = powi_ref(out[i], pp);
+  }
+  if (cell) {
+    for (i = 0; i < N; i++) out[i] = (out[i] + out[i + 1]) * 0.5;
+    return N;
+  }
+  out[0] = start;
+  out[N] = stop;
+  return N + 1;
+}
+
+/* scale::rescale, scale.hxx:175-189 */
+static double rescale(double x, int sct)
+{
+  if (sct == ORC_LOG) return log(x);
+  if (sct == ORC_LINEAR) return x;
+  return pow(x, 1. / sct);
+}
+
+/* ------------------------------------------------------------------ */
+/* chain of delay cells (cellChain.hxx, dummyCell21_vect.hxx) */
+typedef struct {
+  long point; /* grid index i the chain feeds (map key, sphericalKernel.hxx:174-187) */
+  long j;
+  long n, cap;
+  double *R, *U, *val;
+} chain_t;
+
+struct orc_state {
+  orc_params p;
+  long nR, nU, nJ, nVal;
+  double *R_Val, *U_Val, *J_Val, *R_V, *U_V, *J_V;
+  double *f;
+  /* initial-condition constants */
+  double C0, C1, C2, rho0, R0, Vg;
+  /* spline (localSpline.hxx:60-93) */
+  int sdim[2];
+  double sstart[2], sstep_inv[2];
+  long stride1, cstride; /* padded row stride, padded slice size */
+  double *coef;          /* nJ padded coefficient slices */
+  double *tmpArr;        /* per-thread Thomas work rows */
+  long tmpStride;
+  /* feet (vlasovSolver21.hxx:92-94, vlasovSolver21_spline.hxx:69-72) */
+  double *driftR, *driftU;
+  unsigned char *driftOut;
+  int *driftR_I, *driftU_I;
+  double *driftR_F, *driftU_F;
+  double kernelRadius;
+  /* delay lines */
+  long nChains;
+  chain_t *chains;
+  long *chainOfPoint; /* -1 or chain id */
+  long *chainStart;   /* per slice j: first chain id (chains sorted by j) */
+  /* last step's moments */
+  double *dM, *Mr, mTot;
+  double time;
+  long stepCount;
+};
+
+/* ------------------------------------------------------------------ */
+/* initial conditions */
+
+/* UniformDensitySphere::setupParameters, initGen_UniformDensitySphere.hxx:65-74 */
+static void udf_setup(orc_state *s)
+{
+  const orc_params *p = &s->p;
+  s->rho0 = (3 * p->M) / (4 * M_PI * powi_ref(p->R0, 3));
+  double sig0 = sqrt((p->alpha) / (5 * p->R0));
+  s->C1 = s->rho0 * pow(2 * M_PI * sig0 * sig0, -1.5);
+  s->C2 = 2 * sig0 * sig0;
+}
+
+/* UniformDensitySphere::computeF, initGen_UniformDensitySphere.hxx:158-173 */
+static double udf_F(const orc_state *s, double r, double u, double j)
+{
+  const orc_params *p = &s->p;
+  double sm = 0.5 * (1. + erf((p->R0 - r) / p->smoothingRadius));
+  double res = sm * s->C1 * exp(-(u * u + (j * j) / (r * r)) / s->C2);
+  if (p->gap1 > 0) {
+    double sm0 = 0.5 * (1. + erf((p->gap0 - r) / 0.15));
+    double sm1 = 0.5 * (1. + erf(-(p->gap1 - r) / 0.15));
+    res *= (sm0 + sm1);
+  }
+  if (res < 1.E-200) res = 1.E-200;
+  return res;
+}
+
+/* PowerLawDensitySphere::setupParameters, initGen_PowerLawDensitySphere.hxx:62-80
+ * (R0 resolved by the caller: 1/eta unless given, :63-64) */
+static void pldf_setup(orc_state *s)
+{
+  const orc_params *p = &s->p;
+  double n = p->n;
+  s->rho0 = ((3.0 - n) * p->M) / (4 * M_PI * pow(p->R0, 3.0 - n));
+  double sigO2 = (3.0 - n) / (5.0 - 2 * n) * (p->M * p->eta) / (p->R0 * (p->alpha * 0.5 + 1));
+  double sigR2 = sigO2 * 0.5 * p->alpha;
+  s->C0 = s->rho0 / (sqrt(2 * sigR2 * M_PI * M_PI * M_PI) * sigO2);
+  s->C1 = 1.0 / (2 * sigR2);
+  s->C2 = 1.0 / (sigO2);
+}
+
+/* PowerLawDensitySphere::computeF, initGen_PowerLawDensitySphere.hxx:181-189 */
+static double pldf_F(const orc_state *s, double r, double u, double j)
+{
+  const orc_params *p = &s->p;
+  double sm = 0.5 * (1. + erf((p->R0 - r) / p->smoothingRadius));
+  double res = sm * s->C0 * exp(-u * u * s->C1 - ((j * j) / (r * r)) * s->C2) * pow(r, -p->n);
+  if (res < 1.E-200) res = 1.E-200;
+  return res;
+}
+
+/* Hernquist, initGen_Hernquist.hxx:38-50 (Vg), :92-94 (Phi), :133-154 (F) */
+static double hq_phi(const orc_state *s, double R)
+{
+  return -(s->p.G * s->p.M) / (s->p.R0 + R);
+}
+static double hq_F(const orc_state *s, double phi, double r, double u, double j)
+{
+  const orc_params *p = &s->p;
+  double eps = phi + (u * u + (j * j) / (r * r)) * 0.5;
+  if (p->Ra > 0) eps = eps + (j * j) / (2 * p->Ra * p->Ra);
+  if (eps >= 0) return 0;
+  double q = sqrt(-(p->R0) / (p->G * p->M) * eps);
+  double q2 = q * q;
+  double A1 = (p->M) / (8. * sqrt(2.) * pow(M_PI * p->R0 * s->Vg, 3.)) * pow(1. - q2, -2.5);
+  double A2 = 3 * asin(q) + q * sqrt(1. - q2) * (1. - 2 * q2) * (8 * q2 * q2 - 8 * q2 - 3);
+  double result = A1 * A2;
+  if (p->Ra > 0) {
+    A1 = (p->M) / (sqrt(2.) * powi_ref(M_PI * p->R0 * s->Vg, 3)) * powi_ref(p->R0 / p->Ra, 2);
+    A2 = q * (1. - 2 * q2);
+    result += A1 * A2;
+  }
+  return result;
+}
+
+/* initGen::valueAt for each generator (UDF :175-179, PLDF :191-195, Hernquist :156-161) */
+static double value_at(const orc_state *s, double r, double u, double j)
+{
+  switch (s->p.initType) {
+  case ORC_UDF: return udf_F(s, r, u, j);
+  case ORC_PLDF: return pldf_F(s, r, u, j);
+  default: return hq_F(s, hq_phi(s, r), r, u, j);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* free streaming: vlasovSolver21::advanceParticleDrift, vlasovSolver21.hxx:328-408.
+ * pow(x,2) in the reference is pow(double,int) = exact square x*x. */
+static int advance(double R, double U, double J, double dt, double R0, double *oR, double *oU)
+{
+  double H0, newU, newR, tlim, tR0, sgn, deltaT, t;
+  H0 = 0.5 * (U * U + (J * J) / (R * R));
+  sgn = (U <= 0) ? -1 : 1;
+  deltaT = dt;
+  newR = R;
+  newU = U;
+  if (sgn < 0) {
+    tlim = 10 * dt;
+    tR0 = 10 * dt;
+  } else {
+    tlim = sqrt(2 * R * R * H0 - J * J) / (2 * H0);
+    if (R0 > 0) {
+      double A, B, C, D;
+      A = 2 * H0;
+      B = -2 * sgn * sqrt(2 * R * R * H0 - J * J);
+      C = (R * R - R0 * R0);
+      D = B * B - 4 * A * C;
+      if (D < 0) tR0 = 10 * dt;
+      else tR0 = (-B - sqrt(D)) / (2 * A);
+      if (tR0 < 0) {
+        tR0 = (-B + sqrt(D)) / (2 * A);
+        if (tR0 < 0) tR0 = 10 * dt;
+      }
+    } else tR0 = 10 * dt;
+  }
+  if ((tR0 < deltaT) && (tR0 < tlim)) {
+    t = fabs(newR * newU) - 2 * sgn * H0 * tR0;
+    newR = sqrt(t * t + J * J) * pow(2 * H0, -0.5);
+    newU = (2 * H0 - (J * J) / (newR * newR));
+    newU = -sqrt(newU);
+    sgn = -1;
+    H0 = 0.5 * (newU * newU + (J * J) / (newR * newR));
+    deltaT -= tR0;
+    tlim = 10 * dt;
+    tR0 = 10 * dt;
+  }
+  if (tlim < deltaT) {
+    t = fabs(newR * newU) - 2 * sgn * H0 * tlim;
+    newR = sqrt(t * t + J * J) * pow(2 * H0, -0.5);
+    newU = 0;
+    H0 = 0.5 * (J * J) / (newR * newR);
+    sgn = -1;
+    deltaT -= tlim;
+    tR0 -= tlim;
+    tlim = 10 * dt;
+    if (tR0 < deltaT) { /* the reference prints and exits here (:389-399) */
+      set_err("advanceParticleDrift: pericentre and reflection in one step");
+      return -1;
+    }
+  }
+  t = fabs(newR * newU) - 2 * sgn * H0 * deltaT;
+  newR = sqrt(t * t + J * J) * pow(2 * H0, -0.5);
+  newU = (2 * H0 - (J * J) / (newR * newR));
+  if (newU <= 0) newU = 0;
+  else newU = sgn * sqrt(newU);
+  *oR = newR;
+  *oU = newU;
+  return 0;
+}
+
+void orc_advance_drift(double R, double U, double J, double dt, double R0, double *newR,
+                       double *newU)
+{
+  if (advance(R, U, J, dt, R0, newR, newU)) *newR = *newU = NAN;
+}
+
+/* ------------------------------------------------------------------ */
+/* spline: localSpline<2>, natural boundary conditions on a single rank
+ * (vlasovSolver21_spline.hxx:315-345) */
+
+/* localSpline::create_1D, localSpline.hxx:953-1182, NATURAL rows (:976-981,
+ * :1019-1024), elimination :1045-1059 and :1104-1112, last row :1143-1153,
+ * back substitution :1155-1172.  A is a (M x 4) work array. */
+static void create_1D(const orc_state *s, double *A, double *tab, int dir, long delta)
+{
+  const long M = (s->sdim[dir] + 2);
+  const double hi = s->sstep_inv[dir];
+  double *a = A;
+  double *t = &tab[delta];
+  long row;
+  {
+    double hi2 = hi * hi;
+    a[0] = hi2; a[1] = -2.0 * hi2; a[2] = hi2; a[3] = 0;
+  }
+  a += 4;
+  for (long i = 1; i < M - 1; i++, t += delta, a += 4) {
+    a[0] = 1.0 / 6.0;
+    a[1] = 4.0 / 6.0;
+    a[2] = 1.0 / 6.0;
+    a[3] = *t;
+  }
+  a = &A[(M - 1) * 4];
+  {
+    double hi2 = hi * hi;
+    a[0] = hi2; a[1] = -2.0 * hi2; a[2] = hi2; a[3] = 0;
+  }
+  A[4 * (0) + 1] /= A[4 * (0) + 0];
+  A[4 * (0) + 2] /= A[4 * (0) + 0];
+  A[4 * (0) + 3] /= A[4 * (0) + 0];
+  A[4 * (0) + 0] = 1.0;
+  A[4 * (1) + 1] -= A[4 * (1) + 0] * A[4 * (0) + 1];
+  A[4 * (1) + 2] -= A[4 * (1) + 0] * A[4 * (0) + 2];
+  A[4 * (1) + 3] -= A[4 * (1) + 0] * A[4 * (0) + 3];
+  A[4 * (1) + 0] = 0.0;
+  A[4 * (1) + 2] /= A[4 * (1) + 1];
+  A[4 * (1) + 3] /= A[4 * (1) + 1];
+  A[4 * (1) + 1] = 1.0;
+  row = 2;
+  a = &A[4 * (row - 1) + 0];
+  for (; row < M - 1; row++, a += 4) {
+    a[4 + 1] -= a[4 + 0] * a[2];
+    a[4 + 3] -= a[4 + 0] * a[3];
+    a[4 + 2] /= a[4 + 1];
+    a[4 + 3] /= a[4 + 1];
+    a[4 + 0] = 0.0;
+    a[4 + 1] = 1.0;
+  }
+  A[4 * (M - 1) + 1] -= A[4 * (M - 1) + 0] * A[4 * (M - 3) + 2];
+  A[4 * (M - 1) + 3] -= A[4 * (M - 1) + 0] * A[4 * (M - 3) + 3];
+  A[4 * (M - 1) + 2] -= A[4 * (M - 1) + 1] * A[4 * (M - 2) + 2];
+  A[4 * (M - 1) + 3] -= A[4 * (M - 1) + 1] * A[4 * (M - 2) + 3];
+  A[4 * (M - 1) + 3] /= A[4 * (M - 1) + 2];
+  A[4 * (M - 1) + 2] = 1.0;
+
+  t = &tab[delta * (M - 1)];
+  *t = A[4 * (M - 1) + 3];
+  t -= delta;
+  for (row = M - 2; row > 0; row--, t -= delta) (*t) = A[4 * (row) + 3] - A[4 * (row) + 2] * t[delta];
+  (*t) = A[4 * (0) + 3] - A[4 * (0) + 1] * t[delta] - A[4 * (0) + 2] * t[2 * delta];
+}
+
+/* localSpline::assign (:211-246) + create_2D (:1238-1258): the r-direction
+ * solve of rows 1..dim1 (stride 1), then the v-direction solve of every padded
+ * column (stride dim0+2).  data must hold (dim0+2)*(dim1+2) doubles. */
+static void build_slice(const orc_state *s, double *A, const double *slice, double *data)
+{
+  const long d0 = s->sdim[0], d1 = s->sdim[1], st = s->stride1;
+  long i, r, u;
+  for (u = 0; u < d1; u++)
+    for (r = 0; r < d0; r++) data[(u + 1) * st + (r + 1)] = slice[u * d0 + r];
+  double *tab = &data[st];
+  for (i = 0; i <= d1 - 1; i++, tab += st) create_1D(s, A, tab, 0, 1);
+  tab = &data[0];
+  for (i = -1; i <= d0; i++, tab += 1) create_1D(s, A, tab, 1, st);
+}
+
+static inline double BV2(double x) { return x * x * x; }
+static inline double BV1(double x) { return 1.0 + 3.0 * x + 3.0 * x * x - 3.0 * x * x * x; }
+
+/* localSpline::evaluateC(Nx,x,Ny,y), localSpline.hxx:381-398 */
+static double evalC(long st, const double *data, int Nx, double x, int Ny, double y)
+{
+  Nx++;
+  Ny++;
+  const long ref[4] = {Nx + (Ny - 1) * st, Nx + Ny * st, Nx + (Ny + 1) * st, Nx + (Ny + 2) * st};
+  const double b2xI = BV2(1. - x);
+  const double b1xI = BV1(1. - x);
+  const double b2x = BV2(x);
+  const double b1x = BV1(x);
+  double result = 1.0 / 36.0;
+  result *=
+      BV2(1. - y) * (data[ref[0] - 1] * b2xI + data[ref[0]] * b1xI + data[ref[0] + 1] * b1x + data[ref[0] + 2] * b2x) +
+      BV1(1. - y) * (data[ref[1] - 1] * b2xI + data[ref[1]] * b1xI + data[ref[1] + 1] * b1x + data[ref[1] + 2] * b2x) +
+      BV1(y) * (data[ref[2] - 1] * b2xI + data[ref[2]] * b1xI + data[ref[2] + 1] * b1x + data[ref[2] + 2] * b2x) +
+      BV2(y) * (data[ref[3] - 1] * b2xI + data[ref[3]] * b1xI + data[ref[3] + 1] * b1x + data[ref[3] + 2] * b2x);
+  return result;
+}
+
+/* localSpline::evaluateNxPy, localSpline.hxx:423-441 */
+static double evalNxPy(const orc_state *s, const double *data, int Nx, double yy)
+{
+  const long st = s->stride1;
+  Nx++;
+  yy = rescale(yy, s->p.Uscale[0]);
+  double y = (yy - s->sstart[1]) * s->sstep_inv[1];
+  y = modf(y, &yy);
+  int Ny = 1 + (int)yy;
+  const long ref[4] = {Nx + (Ny - 1) * st, Nx + Ny * st, Nx + (Ny + 1) * st, Nx + (Ny + 2) * st};
+  double result = 1.0 / 36.0;
+  result *= BV2(1. - y) * (data[ref[0] - 1] + data[ref[0]] * 4 + data[ref[0] + 1]) +
+            BV1(1. - y) * (data[ref[1] - 1] + data[ref[1]] * 4 + data[ref[1] + 1]) +
+            BV1(y) * (data[ref[2] - 1] + data[ref[2]] * 4 + data[ref[2] + 1]) +
+            BV2(y) * (data[ref[3] - 1] + data[ref[3]] * 4 + data[ref[3] + 1]);
+  return result;
+}
+
+/* localSpline::pos2Coord, localSpline.hxx:352-358 */
+static void pos2coord(const orc_state *s, double p, int dir, int *I, double *F)
+{
+  p = rescale(p, dir == 0 ? s->p.Pscale : s->p.Uscale[0]);
+  double x = (p - s->sstart[dir]) * s->sstep_inv[dir];
+  x = modf(x, &p);
+  *I = (int)p;
+  *F = x;
+}
+
+/* localSpline::evaluate(x,y), localSpline.hxx:477-488 */
+static double eval_xy(const orc_state *s, const double *data, double xx, double yy)
+{
+  int I, J;
+  double x, y;
+  pos2coord(s, xx, 0, &I, &x);
+  pos2coord(s, yy, 1, &J, &y);
+  return evalC(s->stride1, data, I, x, J, y);
+}
+
+/* ------------------------------------------------------------------ */
+/* quadrature (quadrature21.hxx, integrand21.hxx, slice21Iterator.hxx:84-94) */
+
+static inline double get_dR(const orc_state *s, long r)
+{
+  return (r >= s->nR - 1) ? 0 : (s->R_V[r + 1] - s->R_V[r]);
+}
+static inline double get_dU(const orc_state *s, long u)
+{
+  return (u >= s->nU - 1) ? 0 : (s->U_V[u + 1] - s->U_V[u]);
+}
+/* slice21Iterator.hxx:86 compares j against U_V.size(), not J_V.size() */
+static inline double get_dJ(const orc_state *s, long j)
+{
+  return (j >= s->nU - 1) ? 0 : (s->J_V[j + 1] - s->J_V[j]);
+}
+
+/* slice21Iterator::get_avg, slice21Iterator.hxx:90-94 */
+static inline double avg4(const orc_state *s, const double *f, long i)
+{
+  return ((f[i]) + f[i + 1] + f[i + s->nR] + f[i + s->nR + 1]) * 0.25;
+}
+
+/* static schedule chunk of iteration range [0,n) for thread t of T
+ * (contiguous, balanced; identical to every OpenMP runtime when T | n) */
+static void chunk(long n, int T, int t, long *lo, long *hi)
+{
+  long q = n / T, rem = n % T;
+  *lo = t * q + (t < rem ? t : rem);
+  *hi = *lo + q + (t < rem ? 1 : 0);
+}
+
+/* per-point moment term of integrate1D (quadrature21.hxx:111-119) */
+static inline double moment_term(const orc_state *s, const double *f, long j, long u, long r,
+                                 double Ct1)
+{
+  double dU = get_dU(s, u);
+  if (dU <= 0) return NAN;
+  double dR = get_dR(s, r);
+  if (dR <= 0) return NAN;
+  return Ct1 * dU * dR * avg4(s, f, (j * s->nU + u) * s->nR + r);
+}
+
+/* quadrature21::integrate1D with int21::f (quadrature21.hxx:88-142) */
+static void integrate1D(orc_state *s, const double *f, double *Mrv)
+{
+  const long nR = s->nR, nU = s->nU, nJ = s->nJ;
+  long r;
+  int T = s->p.nthreads;
+  if (T >= 1) {
+    double *tmp = calloc((size_t)T * nR, sizeof(double));
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < T; t++) {
+      long lo, hi;
+      chunk(nJ, T, t, &lo, &hi);
+      double *Mr = tmp + (size_t)t * nR;
+      for (long j = lo; j < hi; j++) {
+        double Ct1 = 8 * M_PI * M_PI * s->J_Val[j] * get_dJ(s, j);
+        for (long u = 0; u < nU; u++)
+          for (long rr = 0; rr < nR; rr++) {
+            double dU = get_dU(s, u);
+            if (dU <= 0) continue;
+            double dR = get_dR(s, rr);
+            if (dR <= 0) continue;
+            Mr[rr] += Ct1 * dU * dR * avg4(s, f, (j * nU + u) * nR + rr);
+          }
+      }
+    }
+    for (r = 0; r < nR; r++) {
+      Mrv[r] = 0;
+      for (int t = 0; t < T; t++) Mrv[r] += tmp[(size_t)t * nR + r];
+    }
+    free(tmp);
+    return;
+  }
+  /* device order: P[j][r] = serial sum over u, then dM[r] = serial sum over j */
+  double *P = calloc((size_t)nJ * nR, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < nJ; j++) {
+    double Ct1 = 8 * M_PI * M_PI * s->J_Val[j] * get_dJ(s, j);
+    for (long rr = 0; rr < nR - 1; rr++) {
+      double acc = 0;
+      for (long u = 0; u < nU - 1; u++) {
+        double tval = moment_term(s, f, j, u, rr, Ct1);
+        if (tval == tval) acc += tval;
+      }
+      P[(size_t)j * nR + rr] = acc;
+    }
+  }
+  for (r = 0; r < nR; r++) {
+    double acc = 0;
+    for (long j = 0; j < nJ; j++) acc += P[(size_t)j * nR + r];
+    Mrv[r] = acc;
+  }
+  free(P);
+}
+
+/* integrand kinds of integrand21.hxx: f (:66-79), kineticEnergy (:82-95),
+ * entropy (:98-112), L1Norm (:115-128), L2Norm (:131-145) */
+enum { IG_F = 0, IG_EK, IG_S, IG_L1, IG_L2 };
+
+static inline double integrand(int kind, double r, double u, double j, double v)
+{
+  switch (kind) {
+  case IG_F: return v;
+  case IG_EK: return 0.5 * v * (u * u + j * j / (r * r));
+  case IG_S: return (v <= 0) ? 0 : -v * log(v);
+  case IG_L1: return fabs(v);
+  default: return v * v;
+  }
+}
+
+/* quadrature21::integrate (quadrature21.hxx:57-85) */
+static double integrate(orc_state *s, const double *f, int kind)
+{
+  const long nR = s->nR, nU = s->nU, nJ = s->nJ;
+  int T = s->p.nthreads;
+  if (T >= 1) {
+    double *res = calloc((size_t)T, sizeof(double));
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < T; t++) {
+      long lo, hi;
+      chunk(nJ, T, t, &lo, &hi);
+      double acc = 0;
+      for (long j = lo; j < hi; j++) {
+        double J = s->J_Val[j];
+        double dJ = get_dJ(s, j);
+        double Ct1 = 8 * M_PI * M_PI * J * dJ;
+        for (long u = 0; u < nU; u++)
+          for (long r = 0; r < nR; r++) {
+            double dU = get_dU(s, u);
+            if (dU <= 0) continue;
+            double dR = get_dR(s, r);
+            if (dR <= 0) continue;
+            double v = avg4(s, f, (j * nU + u) * nR + r);
+            acc += Ct1 * dU * dR * integrand(kind, s->R_Val[r] + 0.5 * dR, s->U_Val[u] + 0.5 * dU, J, v);
+          }
+      }
+      res[t] = acc;
+    }
+    for (int t = 1; t < T; t++) res[0] += res[t];
+    double out = res[0];
+    free(res);
+    return out;
+  }
+  /* device order: per (j,r) serial over u, per j serial over r, then over j */
+  double *Q = calloc((size_t)nJ, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < nJ; j++) {
+    double J = s->J_Val[j];
+    double dJ = get_dJ(s, j);
+    double Ct1 = 8 * M_PI * M_PI * J * dJ;
+    double qj = 0;
+    for (long r = 0; r < nR - 1; r++) {
+      double dR = get_dR(s, r);
+      double acc = 0;
+      for (long u = 0; u < nU - 1; u++) {
+        double dU = get_dU(s, u);
+        double v = avg4(s, f, (j * nU + u) * nR + r);
+        acc += Ct1 * dU * dR * integrand(kind, s->R_Val[r] + 0.5 * dR, s->U_Val[u] + 0.5 * dU, J, v);
+      }
+      qj += acc;
+    }
+    Q[j] = qj;
+  }
+  double tot = 0;
+  for (long j = 0; j < nJ; j++) tot += Q[j];
+  free(Q);
+  return tot;
+}
+
+/* gridHandler::accumulate with addSpare=1 (grid_handler.hxx:350-407):
+ * exclusive prefix Mr[i] = sum_{k<i} dM[k]; returns the total */
+static double accumulate(const double *dM, double *Mr, long n)
+{
+  double prev = 0; /* result[0] = 0 */
+  for (long i = 0; i < n; i++) {
+    Mr[i] = prev;
+    prev += dM[i]; /* result[i+1] = result[i] + dM[i] */
+  }
+  return prev;
+}
+
+/* ------------------------------------------------------------------ */
+static void chain_push(chain_t *c, double R, double U, double v)
+{
+  if (c->n == c->cap) {
+    c->cap = c->cap ? 2 * c->cap : 4;
+    c->R = realloc(c->R, c->cap * sizeof(double));
+    c->U = realloc(c->U, c->cap * sizeof(double));
+    c->val = realloc(c->val, c->cap * sizeof(double));
+  }
+  c->R[c->n] = R;
+  c->U[c->n] = U;
+  c->val[c->n] = v;
+  c->n++;
+}
+
+/* cellChain::readRoll + roll (cellChain.hxx:49-63): read the front value, then
+ * successive swaps move every value one place towards the front and the old
+ * front value to the back */
+static double chain_read_roll(chain_t *c)
+{
+  double result = c->val[0];
+  for (long k = 0; k + 1 < c->n; k++) {
+    double t = c->val[k];
+    c->val[k] = c->val[k + 1];
+    c->val[k + 1] = t;
+  }
+  return result;
+}
+
+/* vlasovSolver21::generateDisplacement (vlasovSolver21.hxx:102-143) */
+static int gen_displacement(orc_state *s, double dt, int reflective)
+{
+  const double dth = 0.5 * dt;
+  const double R0 = reflective ? s->kernelRadius : -1;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (long j = 0; j < s->nJ; j++) {
+    double J = s->J_Val[j];
+    for (long u = 0; u < s->nU; u++)
+      for (long r = 0; r < s->nR; r++) {
+        long i = (j * s->nU + u) * s->nR + r;
+        double nr, nu;
+        if (advance(s->R_Val[r], s->U_Val[u], J, dth, R0, &nr, &nu)) bad = 1;
+        s->driftR[i] = nr;
+        s->driftU[i] = nu;
+        s->driftOut[i] = (nr >= s->R_Val[s->nR - 1]) || (nr <= s->R_Val[0]) ||
+                         (nu >= s->U_Val[s->nU - 1]) || (nu <= s->U_Val[0]);
+      }
+  }
+  return bad ? -1 : 0;
+}
+
+/* vlasovSolver21::setupBoundaryConditions (vlasovSolver21.hxx:265-326) */
+static int setup_chains(orc_state *s, double dt)
+{
+  const double dth = 0.5 * dt;
+  long cap = 0;
+  s->nChains = 0;
+  s->chains = NULL;
+  for (long j = 0; j < s->nJ; j++) {
+    double J = s->J_Val[j];
+    s->chainStart[j] = s->nChains;
+    for (long u = 0; u < s->nU; u++)
+      for (long r = 0; r < s->nR; r++) {
+        long i = (j * s->nU + u) * s->nR + r;
+        if (!(s->driftR[i] < s->kernelRadius)) continue;
+        if (s->nChains == cap) {
+          cap = cap ? 2 * cap : 64;
+          s->chains = realloc(s->chains, cap * sizeof(chain_t));
+        }
+        chain_t *c = &s->chains[s->nChains];
+        memset(c, 0, sizeof *c);
+        c->point = i;
+        c->j = j;
+        s->chainOfPoint[i] = s->nChains++;
+        double deltaT = dth, nr, nu;
+        if (advance(s->R_Val[r], s->U_Val[u], J, deltaT, -1, &nr, &nu)) return -1;
+        while (nr < s->kernelRadius) {
+          chain_push(c, nr, nu, value_at(s, nr, nu, J));
+          deltaT += dth;
+          if (advance(s->R_Val[r], s->U_Val[u], J, deltaT, -1, &nr, &nu)) return -1;
+        }
+        chain_push(c, nr, nu, 0);
+      }
+  }
+  s->chainStart[s->nJ] = s->nChains;
+  return 0;
+}
+
+/* vlasovSolver21::renormalize (vlasovSolver21.hxx:468-524), kernelMass = 0 */
+static void renormalize(orc_state *s, double norm)
+{
+  if (norm <= 0) return;
+  double mass = integrate(s, s->f, IG_F) + 0.0;
+  double fac = norm / mass;
+  for (long i = 0; i < s->nVal; i++) s->f[i] *= fac;
+  for (long c = 0; c < s->nChains; c++)
+    for (long k = 0; k < s->chains[c].n; k++) s->chains[c].val[k] *= fac;
+}
+
+/* ------------------------------------------------------------------ */
+static int thread_count(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+static int thread_id(void)
+{
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
+
+static void build_all(orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++)
+    build_slice(s, s->tmpArr + (size_t)thread_id() * s->tmpStride, s->f + j * s->nU * s->nR,
+                s->coef + (size_t)j * s->cstride);
+}
+
+/* kick loop, vlasovSolver21_spline.hxx:750-765 */
+static void kick(orc_state *s)
+{
+  const double Ulo = s->U_Val[0], Uhi = s->U_Val[s->nU - 1];
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++) {
+    const double *c = s->coef + (size_t)j * s->cstride;
+    for (long u = 0; u < s->nU; u++)
+      for (long r = 0; r < s->nR; r++) {
+        long i = (j * s->nU + u) * s->nR + r;
+        double newU = s->U_Val[u] + s->Mr[r];
+        if ((newU <= Ulo) || (newU >= Uhi)) s->f[i] = 0;
+        else s->f[i] = evalNxPy(s, c, (int)r, newU);
+      }
+  }
+}
+
+/* drift loop, vlasovSolver21_spline.hxx:769-798 (delayed-kernel back values
+ * :776-786, then per point :788-796) */
+static void drift(orc_state *s)
+{
+  const double Ulo = s->U_Val[0], Uhi = s->U_Val[s->nU - 1];
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++) {
+    const double *c = s->coef + (size_t)j * s->cstride;
+    if (s->p.kernelType == ORC_DELAYED)
+      for (long k = s->chainStart[j]; k < s->chainStart[j + 1]; k++) {
+        chain_t *ch = &s->chains[k];
+        double R = ch->R[ch->n - 1], U = ch->U[ch->n - 1];
+        if ((U <= Ulo) || (U >= Uhi)) ch->val[ch->n - 1] = 0;
+        else ch->val[ch->n - 1] = eval_xy(s, c, R, U);
+      }
+    for (long u = 0; u < s->nU; u++)
+      for (long r = 0; r < s->nR; r++) {
+        long i = (j * s->nU + u) * s->nR + r;
+        if (s->driftR[i] < s->kernelRadius) s->f[i] = chain_read_roll(&s->chains[s->chainOfPoint[i]]);
+        else if (s->driftOut[i]) s->f[i] = 0;
+        else s->f[i] = evalC(s->stride1, c, s->driftR_I[i], s->driftR_F[i], s->driftU_I[i], s->driftU_F[i]);
+      }
+  }
+}
+
+static void moment_and_accel(orc_state *s)
+{
+  integrate1D(s, s->f, s->dM);
+  s->mTot = accumulate(s->dM, s->Mr, s->nR);
+  for (long i = 0; i < s->nR; i++) s->Mr[i] *= s->p.G / (s->R_Val[i] * s->R_Val[i]) * s->p.dt;
+}
+
+/* vlasovSolver21_spline::makeOneStep, vlasovSolver21_spline.hxx:680-801 */
+void orc_step(orc_state *s, int nsteps)
+{
+  for (int n = 0; n < nsteps; n++) {
+    if (s->p.noLinking) {
+      build_all(s);
+      drift(s);
+    }
+    moment_and_accel(s);
+    build_all(s);
+    kick(s);
+    build_all(s);
+    drift(s);
+    s->time += s->p.dt;
+    s->stepCount++;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+orc_state *orc_create(const orc_params *p)
+{
+  orc_state *s = calloc(1, sizeof *s);
+  s->p = *p;
+  if (p->Uscale[0] != ORC_LINEAR) {
+    set_err("only a linear v_r scale is supported");
+    free(s);
+    return NULL;
+  }
+  s->nR = p->Pres + 1;
+  s->nU = p->Ures[0] + 1;
+  s->nJ = p->Ures[1];
+  s->nVal = s->nR * s->nU * s->nJ;
+  s->R_Val = malloc((s->nR + 1) * sizeof(double));
+  s->R_V = malloc((s->nR + 1) * sizeof(double));
+  s->U_Val = malloc((s->nU + 1) * sizeof(double));
+  s->U_V = malloc((s->nU + 1) * sizeof(double));
+  s->J_Val = malloc((s->nJ + 2) * sizeof(double));
+  s->J_V = malloc((s->nJ + 2) * sizeof(double));
+  /* gridBase::init, grid_base.hxx:271-311 (VERTEX r and v, CELL j) */
+  orc_gen_scale(p->Pmin, p->Pmax, p->Pres, p->Pscale, 0, s->R_Val);
+  orc_gen_scale(p->Pmin, p->Pmax, p->Pres, p->Pscale, 0, s->R_V);
+  orc_gen_scale(p->Umin[0], p->Umax[0], p->Ures[0], p->Uscale[0], 0, s->U_Val);
+  orc_gen_scale(p->Umin[0], p->Umax[0], p->Ures[0], p->Uscale[0], 0, s->U_V);
+  orc_gen_scale(p->Umin[1], p->Umax[1], p->Ures[1], p->Uscale[1], 1, s->J_Val);
+  orc_gen_scale(p->Umin[1], p->Umax[1], p->Ures[1], p->Uscale[1], 0, s->J_V);
+
+  s->f = calloc((size_t)s->nVal, sizeof(double));
+  if (p->initType == ORC_UDF) udf_setup(s);
+  else if (p->initType == ORC_PLDF) pldf_setup(s);
+  else s->Vg = sqrt((p->G * p->M) / (p->R0));
+  /* generate (UDF :182-208, PLDF :197-224, Hernquist :163-204) */
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++) {
+    double J = s->J_Val[j];
+    for (long u = 0; u < s->nU; u++)
+      for (long r = 0; r < s->nR; r++) {
+        long i = (j * s->nU + u) * s->nR + r;
+        if (p->initType == ORC_HERNQUIST) s->f[i] = hq_F(s, hq_phi(s, s->R_Val[r]), s->R_Val[r], s->U_Val[u], J);
+        else s->f[i] = value_at(s, s->R_Val[r], s->U_Val[u], J);
+      }
+  }
+
+  /* spline set-up, vlasovSolver21_spline.hxx:296-314 + localSpline::init :60-93 */
+  {
+    double tmp[2][8];
+    int sc[2] = {p->Pscale, p->Uscale[0]};
+    double st[2] = {s->R_V[0], s->U_V[0]}, sp_[2] = {s->R_V[s->nR - 1], s->U_V[s->nU - 1]};
+    long N[2] = {p->Pres, p->Ures[0]};
+    (void)tmp;
+    for (int d = 0; d < 2; d++) {
+      double *scl = malloc((N[d] + 2) * sizeof(double));
+      long n = orc_gen_scale(st[d], sp_[d], N[d], sc[d], 0, scl);
+      s->sdim[d] = (int)n;
+      s->sstart[d] = rescale(scl[0], sc[d]);
+      double stop = rescale(scl[n - 1], sc[d]);
+      double step = (stop - s->sstart[d]) / (n - 1);
+      s->sstep_inv[d] = 1. / step;
+      free(scl);
+    }
+  }
+  s->stride1 = s->sdim[0] + 2;
+  s->cstride = (long)(s->sdim[0] + 2) * (s->sdim[1] + 2);
+  s->coef = calloc((size_t)s->nJ * s->cstride, sizeof(double));
+  s->tmpStride = 4 * (2 + (s->sdim[0] > s->sdim[1] ? s->sdim[0] : s->sdim[1]));
+  s->tmpArr = malloc((size_t)thread_count() * s->tmpStride * sizeof(double));
+
+  /* vlasovSolver21::setup, vlasovSolver21.hxx:411-446 */
+  s->kernelRadius = s->R_Val[p->kernelBufferSize];
+  s->driftR = malloc(s->nVal * sizeof(double));
+  s->driftU = malloc(s->nVal * sizeof(double));
+  s->driftOut = malloc(s->nVal);
+  s->chainOfPoint = malloc(s->nVal * sizeof(long));
+  s->chainStart = calloc(s->nJ + 1, sizeof(long));
+  for (long i = 0; i < s->nVal; i++) s->chainOfPoint[i] = -1;
+  double dtFeet = p->noLinking ? p->dt : p->dt * 2;
+  if (gen_displacement(s, dtFeet, p->kernelType == ORC_REFLECTIVE)) goto fail;
+  /* vlasovSolver21::setup(init,params), :533-548 */
+  if (p->kernelType == ORC_DELAYED) {
+    if (setup_chains(s, dtFeet)) goto fail;
+  } else {
+    for (long i = 0; i < s->nVal; i++)
+      if (s->driftR[i] < s->kernelRadius) {
+        set_err("reflective kernel: a foot falls inside the kernel radius");
+        goto fail;
+      }
+  }
+  /* decomposeDisplacement, vlasovSolver21_spline.hxx:76-100 */
+  s->driftR_I = malloc(s->nVal * sizeof(int));
+  s->driftU_I = malloc(s->nVal * sizeof(int));
+  s->driftR_F = malloc(s->nVal * sizeof(double));
+  s->driftU_F = malloc(s->nVal * sizeof(double));
+  for (long i = 0; i < s->nVal; i++) {
+    pos2coord(s, s->driftR[i], 0, &s->driftR_I[i], &s->driftR_F[i]);
+    pos2coord(s, s->driftU[i], 1, &s->driftU_I[i], &s->driftU_F[i]);
+  }
+  /* vlasovSolver::init renormalizes to the generator's mass (:94-95) */
+  renormalize(s, p->M);
+  s->dM = calloc(s->nR, sizeof(double));
+  s->Mr = calloc(s->nR, sizeof(double));
+  return s;
+fail:
+  orc_destroy(s);
+  return NULL;
+}
+
+void orc_destroy(orc_state *s)
+{
+  if (!s) return;
+  for (long c = 0; c < s->nChains; c++) {
+    free(s->chains[c].R);
+    free(s->chains[c].U);
+    free(s->chains[c].val);
+  }
+  free(s->chains);
+  free(s->R_Val); free(s->R_V); free(s->U_Val); free(s->U_V); free(s->J_Val); free(s->J_V);
+  free(s->f); free(s->coef); free(s->tmpArr);
+  free(s->driftR); free(s->driftU); free(s->driftOut);
+  free(s->driftR_I); free(s->driftU_I); free(s->driftR_F); free(s->driftU_F);
+  free(s->chainOfPoint); free(s->chainStart); free(s->dM); free(s->Mr);
+  free(s);
+}
+
+void orc_dims(const orc_state *s, long dims[3])
+{
+  dims[0] = s->nR;
+  dims[1] = s->nU;
+  dims[2] = s->nJ;
+}
+long orc_nval(const orc_state *s) { return s->nVal; }
+void orc_get_f(const orc_state *s, double *out) { memcpy(out, s->f, s->nVal * sizeof(double)); }
+void orc_set_f(orc_state *s, const double *in) { memcpy(s->f, in, s->nVal * sizeof(double)); }
+double orc_kernel_radius(const orc_state *s) { return s->kernelRadius; }
+
+void orc_get_axes(const orc_state *s, double *R_Val, double *U_Val, double *J_Val, double *J_V)
+{
+  if (R_Val) memcpy(R_Val, s->R_Val, s->nR * sizeof(double));
+  if (U_Val) memcpy(U_Val, s->U_Val, s->nU * sizeof(double));
+  if (J_Val) memcpy(J_Val, s->J_Val, s->nJ * sizeof(double));
+  if (J_V) memcpy(J_V, s->J_V, (s->nJ + 1) * sizeof(double));
+}
+
+void orc_last_moments(const orc_state *s, double *dM, double *accel, double *mTot)
+{
+  if (dM) memcpy(dM, s->dM, s->nR * sizeof(double));
+  if (accel) memcpy(accel, s->Mr, s->nR * sizeof(double));
+  if (mTot) *mTot = s->mTot;
+}
+
+void orc_moments(orc_state *s, double *dM, double *Mexcl, double *mTot)
+{
+  double *d = malloc(s->nR * sizeof(double));
+  double *m = malloc(s->nR * sizeof(double));
+  integrate1D(s, s->f, d);
+  double t = accumulate(d, m, s->nR);
+  if (dM) memcpy(dM, d, s->nR * sizeof(double));
+  if (Mexcl) memcpy(Mexcl, m, s->nR * sizeof(double));
+  if (mTot) *mTot = t;
+  free(d);
+  free(m);
+}
+
+/* vlasovSolver21::statistics + potentialEnergy (vlasovSolver21.hxx:627-689) */
+void orc_stats(orc_state *s, double out[7])
+{
+  double *Mr = malloc(s->nR * sizeof(double));
+  integrate1D(s, s->f, Mr);
+  double M = Mr[0] + 0.0, Ep = 0;
+  for (long i = 1; i < s->nR; i++) {
+    double r = 0.5 * (s->R_Val[i - 1] + s->R_Val[i]);
+    Ep -= (Mr[i] * s->p.G * M) / r;
+    M += Mr[i];
+  }
+  free(Mr);
+  double Ek = integrate(s, s->f, IG_EK);
+  double E = Ek + Ep;
+  double S = integrate(s, s->f, IG_S);
+  double L1 = integrate(s, s->f, IG_L1);
+  double L2 = integrate(s, s->f, IG_L2);
+  out[0] = M; out[1] = E; out[2] = Ek; out[3] = Ep; out[4] = S; out[5] = L1; out[6] = L2;
+}
+
+void orc_get_feet(const orc_state *s, double *driftR, double *driftU, uint8_t *flag)
+{
+  for (long i = 0; i < s->nVal; i++) {
+    if (driftR) driftR[i] = s->driftR[i];
+    if (driftU) driftU[i] = s->driftU[i];
+    if (flag) flag[i] = (s->driftR[i] < s->kernelRadius) ? 2 : (s->driftOut[i] ? 1 : 0);
+  }
+}
+
+long orc_num_chains(const orc_state *s) { return s->nChains; }
+long orc_chain_cells(const orc_state *s)
+{
+  long n = 0;
+  for (long c = 0; c < s->nChains; c++) n += s->chains[c].n;
+  return n;
+}
+
+void orc_get_chains(const orc_state *s, int64_t *point, int64_t *off, int32_t *len, double *cellR,
+                    double *cellU, double *cellVal)
+{
+  long o = 0;
+  for (long c = 0; c < s->nChains; c++) {
+    const chain_t *ch = &s->chains[c];
+    if (point) point[c] = ch->point;
+    if (off) off[c] = o;
+    if (len) len[c] = (int32_t)ch->n;
+    for (long k = 0; k < ch->n; k++) {
+      if (cellR) cellR[o + k] = ch->R[k];
+      if (cellU) cellU[o + k] = ch->U[k];
+      if (cellVal) cellVal[o + k] = ch->val[k];
+    }
+    o += ch->n;
+  }
+}
+
+void orc_spline_build(const orc_state *s, const double *slice, double *coef)
+{
+  double *A = malloc(s->tmpStride * sizeof(double));
+  memset(coef, 0, s->cstride * sizeof(double));
+  build_slice(s, A, slice, coef);
+  free(A);
+}
+double orc_spline_evalC(const orc_state *s, const double *coef, int Nx, double x, int Ny, double y)
+{
+  return evalC(s->stride1, coef, Nx, x, Ny, y);
+}
+double orc_spline_evalNxPy(const orc_state *s, const double *coef, int Nx, double yy)
+{
+  return evalNxPy(s, coef, Nx, yy);
+}
+double orc_spline_eval(const orc_state *s, const double *coef, double xx, double yy)
+{
+  return eval_xy(s, coef, xx, yy);
+}
