@@ -1,0 +1,164 @@
+/*
+ * vlasolve_hip.h -- C ABI of the MI355X (gfx950) spherical Vlasov-Poisson
+ * spline time step.
+ *
+ * Two layers, both plain C (pointers, sizes, status codes; no C++ or torch
+ * types):
+ *
+ *  1. vls_*  : the hot path.  A device context owns the whole (r, v_r, j)
+ *              distribution, its spline coefficients, the static foot tables
+ *              and the delay-line rings in HBM, and advances them with HIP
+ *              kernels.  This replaces the work of
+ *              vlasovSolver21_spline::makeOneStep
+ *              (src/vlasovSolver21_spline.hxx:680-801) and its helpers.
+ *  2. vlsd_* : the solver surface.  A C++ host solver that keeps the
+ *              reference's parameter keys, initial conditions, foot and
+ *              delay-line construction, statistics.txt, NDfield snapshots and
+ *              .vls restarts, and calls layer 1 for every step.  It replaces
+ *              vlasovSolver<...>::init/run (src/vlasovSolver.hxx:61-190) as
+ *              driven by main (src/vlasolve.cpp:20-55).
+ *
+ * Layout: f is double[nJ][nU][nR] with r fastest, as the reference's grid21
+ * stores it (src/grid_base.hxx:285-310, src/slice21Iterator.hxx:63-64).
+ * Every function returns 0 on success and a non-zero status otherwise; the
+ * message is in vls_last_error() (thread-local).  The reference prints and
+ * calls exit(-1) instead; the vlasolve_hip executable keeps that behaviour.
+ */
+#ifndef VLASOLVE_HIP_H
+#define VLASOLVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VLS_OK 0
+#define VLS_ERR_ARG 1     /* invalid argument or size */
+#define VLS_ERR_DEVICE 2  /* HIP runtime error or no device */
+#define VLS_ERR_STATE 3   /* call out of order */
+#define VLS_ERR_IO 4      /* file error */
+
+/* Point flags of the drift step (vlasovSolver21_spline.hxx:792-795). */
+#define VLS_FOOT_EVAL 0   /* f = spline at the foot */
+#define VLS_FOOT_OUT 1    /* foot outside the grid: f = 0 (driftOut) */
+#define VLS_FOOT_CHAIN 2  /* foot inside the kernel radius: delay line */
+
+typedef struct vls_ctx vls_ctx;
+
+/* Grid geometry of one run: node values and quadrature widths, as the
+ * reference's grid21 / slice21Iterator provide them
+ * (src/slice21Iterator.hxx:81-86), and the uniform knot spacing of the
+ * spline in rescaled coordinates (src/localSpline.hxx:60-93). */
+typedef struct vls_geometry {
+  int32_t nR;             /* r nodes  = Pres+1   (VERTEX)            */
+  int32_t nU;             /* v nodes  = Ures[0]+1 (VERTEX)            */
+  int32_t nJ;             /* j slices = Ures[1]   (CELL)              */
+  const double *R_Val;    /* [nR] */
+  const double *U_Val;    /* [nU] */
+  const double *J_Val;    /* [nJ] */
+  const double *dR;       /* [nR] R_V[r+1]-R_V[r], 0 at the last node */
+  const double *dU;       /* [nU] */
+  const double *dJ;       /* [nJ] get_dJ, including its v-size test  */
+  double spline_start[2]; /* rescaled first knot (r, v)               */
+  double spline_hinv[2];  /* 1/knot spacing (r, v)                    */
+} vls_geometry;
+
+typedef struct vls_solver_params {
+  double G;               /* solver.G  */
+  double dt;              /* solver.dt */
+  int32_t no_linking;     /* solver.noLinking: D(dt/2) K(dt) D(dt/2) */
+  int32_t kernel_type;    /* 0 delayed, 1 reflective (solver.kernelType) */
+} vls_solver_params;
+
+/* ---------------------------------------------------------------- layer 1 */
+
+/* Create a device context holding nRuns independent runs (an ensemble; the
+ * runs may have different grids).  g and sp are arrays of nRuns entries.
+ * Replaces the solver's setup (vlasovSolver21_spline::setup,
+ * src/vlasovSolver21_spline.hxx:296-374) as far as device state goes. */
+int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vls_ctx **out);
+void vls_destroy(vls_ctx *ctx);
+const char *vls_last_error(void);
+
+/* Distribution function of one run, host layout [nJ][nU][nR].
+ * Replaces the grid21 data the reference's step reads and writes
+ * (gridBase::getDataPtr, src/grid_base.hxx:383). */
+int vls_upload_f(vls_ctx *ctx, int run, const double *f);
+int vls_download_f(vls_ctx *ctx, int run, double *f);
+
+/* Static drift feet of every point.  xr, xu are the foot coordinates in
+ * spline units, (rescale(p) - start) * hinv, whose integer and fractional
+ * parts are the reference's driftR_I/driftR_F and driftU_I/driftU_F
+ * (decomposeDisplacement, src/vlasovSolver21_spline.hxx:76-100); flag is a
+ * VLS_FOOT_* value (driftR < kernelRadius, then driftOut;
+ * src/vlasovSolver21.hxx:140-143).  Arrays of nJ*nU*nR entries. */
+int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu,
+                  const uint8_t *flag);
+
+/* Delay lines of the delayed central kernel (sphericalKernel,
+ * src/sphericalKernel.hxx:48-54, cellChain::readRoll src/cellChain.hxx:49-63).
+ * Chain c feeds grid point point[c]; it holds len[c] >= 2 values in a ring
+ * stored at vals[off[c] .. off[c]+len[c]), off = exclusive prefix sum of len,
+ * whose front is at ring index head[c].  back_xr/back_xu are the spline-unit
+ * coordinates of the chain's last cell and back_out[c] is 1 when that cell's
+ * v lies outside (U_min, U_max), where the reference stores 0
+ * (src/vlasovSolver21_spline.hxx:776-786). */
+int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
+                   const int32_t *len, const int32_t *head, const double *back_xr,
+                   const double *back_xu, const uint8_t *back_out, const double *vals);
+/* Current ring state (for restarts and checkpoints). */
+int vls_get_chains(vls_ctx *ctx, int run, int32_t *head, double *vals);
+
+/* Advance every run by nSteps steps (vlasovSolver21_spline::makeOneStep,
+ * src/vlasovSolver21_spline.hxx:680-801), stream-ordered on the context's
+ * stream.  Does not wait for completion. */
+int vls_step(vls_ctx *ctx, int nSteps);
+int vls_synchronize(vls_ctx *ctx);
+
+/* Moments used by the last step's kick: dM[nR] (quadrature21::integrate1D,
+ * src/quadrature21.hxx:88-142), accel[nR] = M(<r) G/r^2 dt
+ * (src/vlasovSolver21_spline.hxx:736-746) and the total mass mTot. */
+int vls_moments(vls_ctx *ctx, int run, double *dM, double *accel, double *mTot);
+
+/* Statistics of the current f (vlasovSolver21::statistics,
+ * src/vlasovSolver21.hxx:627-689): out = {Mass, E, Ek, Ep, S, L1, L2}. */
+int vls_stats(vls_ctx *ctx, int run, double out[7]);
+
+/* Coefficients of the last drift spline of slice j, (nU+2)*(nR+2) doubles,
+ * the padded array of localSpline<2> (src/localSpline.hxx:60-150). */
+int vls_download_coef(vls_ctx *ctx, int run, int j, double *coef);
+
+/* Per-kernel timing of the last vls_step call on the context's stream
+ * (HIP events).  names/ms are arrays of at least *n entries; on return *n is
+ * the number of kernels recorded.  Enabled with vls_set_timing(ctx, 1). */
+int vls_set_timing(vls_ctx *ctx, int on);
+int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n);
+
+/* ---------------------------------------------------------------- layer 2 */
+
+typedef struct vlsd_solver vlsd_solver;
+
+/* Build a solver from reference-style arguments ("-init.type UDF -grid.Pres
+ * 64 ..." and an optional parameter file as first argument; paramsParser,
+ * src/paramsParser.hxx:195-218), run init or reload, and upload to the GPU.
+ * argv[0] is the program name, as for main (src/vlasolve.cpp:20-48). */
+int vlsd_create(int argc, const char *const *argv, vlsd_solver **out);
+void vlsd_destroy(vlsd_solver *s);
+/* geometry: {nR, nU, nJ}; time, step count */
+int vlsd_info(vlsd_solver *s, int64_t dims[3], double *time, int64_t *stepCount);
+int vlsd_step(vlsd_solver *s, int nSteps);          /* makeOneStep x n, time += dt */
+int vlsd_get_f(vlsd_solver *s, double *f);
+int vlsd_stats(vlsd_solver *s, double out[7]);
+int vlsd_moments(vlsd_solver *s, double *dM, double *accel, double *mTot);
+int vlsd_run(vlsd_solver *s);                       /* vlasovSolver::run, src/vlasovSolver.hxx:134-190 */
+int vlsd_write_restart(vlsd_solver *s, const char *fname_no_ext);   /* .vls */
+int vlsd_snapshot(vlsd_solver *s);                  /* snapshot_%4.4f.ND */
+vls_ctx *vlsd_context(vlsd_solver *s);
+const char *vlsd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -700,7 +700,12 @@ static int setup_chains(orc_state *s, double dt)
 static void renormalize(orc_state *s, double norm)
 {
   if (norm <= 0) return;
+  /* set-up runs on the host in the reference's order; in the device order
+   * mode the serial (one-thread) order is used */
+  int T = s->p.nthreads;
+  if (T < 1) s->p.nthreads = 1;
   double mass = integrate(s, s->f, IG_F) + 0.0;
+  s->p.nthreads = T;
   double fac = norm / mass;
   for (long i = 0; i < s->nVal; i++) s->f[i] *= fac;
   for (long c = 0; c < s->nChains; c++)
@@ -864,7 +869,10 @@ orc_state *orc_create(const orc_params *p)
   }
   s->stride1 = s->sdim[0] + 2;
   s->cstride = (long)(s->sdim[0] + 2) * (s->sdim[1] + 2);
-  s->coef = calloc((size_t)s->nJ * s->cstride, sizeof(double));
+  /* a foot on the last knot makes the stencil read one column past its row
+   * (and one row past its slice) with weight zero; pad the tail so those
+   * reads stay inside the allocation */
+  s->coef = calloc((size_t)s->nJ * s->cstride + 2 * (size_t)(s->sdim[0] + 2) + 8, sizeof(double));
   s->tmpStride = 4 * (2 + (s->sdim[0] > s->sdim[1] ? s->sdim[0] : s->sdim[1]));
   s->tmpArr = malloc((size_t)thread_count() * s->tmpStride * sizeof(double));
 
@@ -1021,7 +1029,7 @@ void orc_get_chains(const orc_state *s, int64_t *point, int64_t *off, int32_t *l
 void orc_spline_build(const orc_state *s, const double *slice, double *coef)
 {
   double *A = malloc(s->tmpStride * sizeof(double));
-  memset(coef, 0, s->cstride * sizeof(double));
+  memset(coef, 0, s->cstride * sizeof(double)); /* caller provides cstride doubles */
   build_slice(s, A, slice, coef);
   free(A);
 }

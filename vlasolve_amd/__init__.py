@@ -1,0 +1,319 @@
+"""vlasolve_amd: MI355X (gfx950) spherical Vlasov-Poisson spline step.
+
+The time step runs in hand-written HIP kernels behind the C ABI declared in
+include/vlasolve_hip.h; this module binds that ABI with ctypes.  There is no
+CPU fallback: if the library is not built, or no GPU is present, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libvlasolve_hip.so"
+EXE_PATH = PKG / "bin" / "vlasolve_hip"
+
+VLS_FOOT_EVAL, VLS_FOOT_OUT, VLS_FOOT_CHAIN = 0, 1, 2
+STAT_NAMES = ("M", "E", "Ek", "Ep", "S", "L1", "L2")
+
+# every exported symbol of include/vlasolve_hip.h
+EXPORTS = (
+    "vls_create", "vls_destroy", "vls_last_error", "vls_upload_f", "vls_download_f",
+    "vls_set_drift", "vls_set_chains", "vls_get_chains", "vls_step", "vls_synchronize",
+    "vls_moments", "vls_stats", "vls_download_coef", "vls_set_timing", "vls_kernel_times",
+    "vlsd_create", "vlsd_destroy", "vlsd_info", "vlsd_step", "vlsd_get_f", "vlsd_stats",
+    "vlsd_moments", "vlsd_run", "vlsd_write_restart", "vlsd_snapshot", "vlsd_context",
+    "vlsd_last_error",
+)
+
+
+class VlsError(RuntimeError):
+    pass
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("nR", C.c_int32), ("nU", C.c_int32), ("nJ", C.c_int32),
+        ("R_Val", C.POINTER(C.c_double)), ("U_Val", C.POINTER(C.c_double)),
+        ("J_Val", C.POINTER(C.c_double)), ("dR", C.POINTER(C.c_double)),
+        ("dU", C.POINTER(C.c_double)), ("dJ", C.POINTER(C.c_double)),
+        ("spline_start", C.c_double * 2), ("spline_hinv", C.c_double * 2),
+    ]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("G", C.c_double), ("dt", C.c_double), ("no_linking", C.c_int32),
+                ("kernel_type", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise VlsError(f"HIP library not built: {LIB_PATH} (run `python -m vlasolve_amd.build`)")
+        L = C.CDLL(str(LIB_PATH))
+        P, d, i32, i64, vp = C.POINTER, C.c_double, C.c_int32, C.c_int64, C.c_void_p
+        L.vls_last_error.restype = C.c_char_p
+        L.vlsd_last_error.restype = C.c_char_p
+        L.vls_create.argtypes = [P(Geometry), P(SolverParams), C.c_int, P(vp)]
+        L.vls_destroy.argtypes = [vp]
+        L.vls_destroy.restype = None
+        L.vls_upload_f.argtypes = [vp, C.c_int, P(d)]
+        L.vls_download_f.argtypes = [vp, C.c_int, P(d)]
+        L.vls_set_drift.argtypes = [vp, C.c_int, P(d), P(d), P(C.c_uint8)]
+        L.vls_set_chains.argtypes = [vp, C.c_int, i64, P(i64), P(i32), P(i32), P(d), P(d),
+                                     P(C.c_uint8), P(d)]
+        L.vls_get_chains.argtypes = [vp, C.c_int, P(i32), P(d)]
+        L.vls_step.argtypes = [vp, C.c_int]
+        L.vls_synchronize.argtypes = [vp]
+        L.vls_moments.argtypes = [vp, C.c_int, P(d), P(d), P(d)]
+        L.vls_stats.argtypes = [vp, C.c_int, P(d)]
+        L.vls_download_coef.argtypes = [vp, C.c_int, C.c_int, P(d)]
+        L.vls_set_timing.argtypes = [vp, C.c_int]
+        L.vls_kernel_times.argtypes = [vp, P(C.c_char_p), P(d), P(C.c_int)]
+        L.vlsd_create.argtypes = [C.c_int, P(C.c_char_p), P(vp)]
+        L.vlsd_destroy.argtypes = [vp]
+        L.vlsd_destroy.restype = None
+        L.vlsd_info.argtypes = [vp, P(i64), P(d), P(i64)]
+        L.vlsd_step.argtypes = [vp, C.c_int]
+        L.vlsd_get_f.argtypes = [vp, P(d)]
+        L.vlsd_stats.argtypes = [vp, P(d)]
+        L.vlsd_moments.argtypes = [vp, P(d), P(d), P(d)]
+        L.vlsd_run.argtypes = [vp]
+        L.vlsd_write_restart.argtypes = [vp, C.c_char_p]
+        L.vlsd_snapshot.argtypes = [vp]
+        L.vlsd_context.argtypes = [vp]
+        L.vlsd_context.restype = vp
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(rc, driver=False):
+    if rc:
+        L = lib()
+        msg = (L.vlsd_last_error() if driver else L.vls_last_error()) or b""
+        raise VlsError(f"status {rc}: {msg.decode(errors='replace')}")
+
+
+def to_argv(cfg) -> list[str]:
+    """Reference-style command line ("-cat.key v ...") from a dict or a list."""
+    if isinstance(cfg, (list, tuple)):
+        return [str(x) for x in cfg]
+    out = []
+    for k, v in cfg.items():
+        out.append("-" + k)
+        vals = list(v) if isinstance(v, (list, tuple)) else [v]
+        out += [repr(x) if isinstance(x, float) else str(x) for x in vals]
+    return out
+
+
+class Solver:
+    """The drop-in solver: reference parameters in, every step on the GPU."""
+
+    def __init__(self, cfg, verbose: int = 0):
+        argv = ["vlasolve_hip"] + to_argv(cfg)
+        if not any(a == "-verbose" for a in argv):
+            argv += ["-verbose", str(verbose)]
+        arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+        h = C.c_void_p()
+        _check(lib().vlsd_create(len(argv), arr, C.byref(h)), driver=True)
+        self._h = h
+        dims = (C.c_int64 * 3)()
+        _check(lib().vlsd_info(self._h, dims, None, None), driver=True)
+        self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
+        self.shape = (self.nJ, self.nU, self.nR)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vlsd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def time(self):
+        t = C.c_double()
+        _check(lib().vlsd_info(self._h, None, C.byref(t), None), driver=True)
+        return t.value
+
+    @property
+    def step_count(self):
+        n = C.c_int64()
+        _check(lib().vlsd_info(self._h, None, None, C.byref(n)), driver=True)
+        return n.value
+
+    def step(self, n=1):
+        _check(lib().vlsd_step(self._h, int(n)), driver=True)
+
+    def f(self):
+        out = np.empty(self.shape, np.float64)
+        _check(lib().vlsd_get_f(self._h, _ptr(out)), driver=True)
+        return out
+
+    def stats(self):
+        out = np.empty(7)
+        _check(lib().vlsd_stats(self._h, _ptr(out)), driver=True)
+        return dict(zip(STAT_NAMES, out))
+
+    def moments(self):
+        dM = np.empty(self.nR)
+        a = np.empty(self.nR)
+        t = C.c_double()
+        _check(lib().vlsd_moments(self._h, _ptr(dM), _ptr(a), C.byref(t)), driver=True)
+        return dM, a, t.value
+
+    def run(self):
+        _check(lib().vlsd_run(self._h), driver=True)
+
+    def write_restart(self, fname_no_ext):
+        _check(lib().vlsd_write_restart(self._h, str(fname_no_ext).encode()), driver=True)
+
+    def snapshot(self):
+        _check(lib().vlsd_snapshot(self._h), driver=True)
+
+    @property
+    def context(self):
+        return Context(lib().vlsd_context(self._h), owner=self)
+
+
+class Context:
+    """Layer-1 device context (vls_*); created directly or borrowed from a Solver."""
+
+    def __init__(self, handle=None, owner=None, geometries=None, params=None):
+        self._owner = owner
+        self._own = False
+        if handle is not None:
+            self._h = C.c_void_p(handle)
+            return
+        n = len(geometries)
+        G = (Geometry * n)(*geometries)
+        S = (SolverParams * n)(*params)
+        h = C.c_void_p()
+        _check(lib().vls_create(G, S, n, C.byref(h)))
+        self._h = h
+        self._own = True
+
+    def close(self):
+        if self._own and self._h:
+            lib().vls_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        _check(lib().vls_step(self._h, int(n)))
+
+    def synchronize(self):
+        _check(lib().vls_synchronize(self._h))
+
+    def upload_f(self, run, f):
+        a = np.ascontiguousarray(f, np.float64)
+        _check(lib().vls_upload_f(self._h, run, _ptr(a)))
+
+    def download_f(self, run, shape):
+        out = np.empty(shape, np.float64)
+        _check(lib().vls_download_f(self._h, run, _ptr(out)))
+        return out
+
+    def coef(self, run, j, nU, nR):
+        out = np.empty((nU + 2, nR + 2))
+        _check(lib().vls_download_coef(self._h, run, j, _ptr(out)))
+        return out
+
+    def stats(self, run=0):
+        out = np.empty(7)
+        _check(lib().vls_stats(self._h, run, _ptr(out)))
+        return dict(zip(STAT_NAMES, out))
+
+    def set_timing(self, on=True):
+        _check(lib().vls_set_timing(self._h, int(bool(on))))
+
+    def kernel_times(self, cap=4096):
+        names = (C.c_char_p * cap)()
+        ms = np.zeros(cap)
+        n = C.c_int(cap)
+        _check(lib().vls_kernel_times(self._h, names, _ptr(ms), C.byref(n)))
+        return [(names[k].decode(), float(ms[k])) for k in range(min(n.value, cap))]
+
+
+class HostSetup:
+    """Host-only set-up of a run (initial f, feet, delay lines), no device.
+
+    Runs the same C++ set-up as Solver up to the upload; used to check the
+    host logic on machines without a GPU."""
+
+    def __init__(self, cfg, verbose: int = 0):
+        L = lib()
+        P, d, i64 = C.POINTER, C.c_double, C.c_int64
+        L.vlsd_host_create.argtypes = [C.c_int, P(C.c_char_p), P(C.c_void_p)]
+        L.vlsd_host_destroy.argtypes = [C.c_void_p]
+        L.vlsd_host_destroy.restype = None
+        L.vlsd_host_info.argtypes = [C.c_void_p, P(i64), P(i64), P(i64), P(d)]
+        L.vlsd_host_f.argtypes = [C.c_void_p, P(d)]
+        L.vlsd_host_feet.argtypes = [C.c_void_p, P(d), P(d), P(C.c_uint8)]
+        L.vlsd_host_chains.argtypes = [C.c_void_p, P(i64), P(C.c_int32), P(d), P(d), P(d)]
+        argv = ["vlasolve_hip"] + to_argv(cfg)
+        if not any(a == "-verbose" for a in argv):
+            argv += ["-verbose", str(verbose)]
+        arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+        h = C.c_void_p()
+        _check(L.vlsd_host_create(len(argv), arr, C.byref(h)), driver=True)
+        self._h = h
+        dims = (i64 * 3)()
+        nc, ncell, kr = i64(), i64(), d()
+        _check(L.vlsd_host_info(h, dims, C.byref(nc), C.byref(ncell), C.byref(kr)), driver=True)
+        self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
+        self.shape = (self.nJ, self.nU, self.nR)
+        self.n_chains, self.n_cells, self.kernel_radius = nc.value, ncell.value, kr.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vlsd_host_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def f(self):
+        out = np.empty(self.shape)
+        _check(lib().vlsd_host_f(self._h, _ptr(out)), driver=True)
+        return out
+
+    def feet(self):
+        R = np.empty(self.shape)
+        U = np.empty(self.shape)
+        fl = np.empty(self.shape, np.uint8)
+        _check(lib().vlsd_host_feet(self._h, _ptr(R), _ptr(U), _ptr(fl, C.c_uint8)), driver=True)
+        return R, U, fl
+
+    def chains(self):
+        point = np.empty(self.n_chains, np.int64)
+        ln = np.empty(self.n_chains, np.int32)
+        R = np.empty(self.n_cells)
+        U = np.empty(self.n_cells)
+        V = np.empty(self.n_cells)
+        _check(lib().vlsd_host_chains(self._h, _ptr(point, C.c_int64), _ptr(ln, C.c_int32),
+                                      _ptr(R), _ptr(U), _ptr(V)), driver=True)
+        return dict(point=point, len=ln, R=R, U=U, val=V)

@@ -1,0 +1,308 @@
+// Grid geometry and initial conditions.  The arithmetic of every expression
+// follows the reference's (cited per function) so that node values and the
+// initial distribution are bit-identical to it on x86-64.
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <cstdio>
+
+#include "vls_host.hpp"
+
+namespace vlh {
+
+double powiRef(double x, int m) {
+  // the reference is C++98, where pow(double,int) is __builtin_powi: this
+  // repeated-squaring product (libgcc __powidf2)
+  unsigned n = m < 0 ? -(unsigned)m : (unsigned)m;
+  double y = (n % 2) ? x : 1.0;
+  while (n >>= 1) {
+    x = x * x;
+    if (n % 2) y = y * x;
+  }
+  return m < 0 ? 1.0 / y : y;
+}
+
+ScaleType scaleFromString(const std::string &s) {
+  // scale::scaleTypeSelect (src/scale.hxx:22-32); names are lower case
+  if (s == "linear") return LINEAR;
+  if (s == "logarithmic") return LOGARITHMIC;
+  if (s == "quadratic") return QUADRATIC;
+  if (s == "cubic") return CUBIC;
+  if (s == "quartic") return QUARTIC;
+  throw Failure("ERROR: unknown scale type value: '" + s + "'");
+}
+
+std::string scaleToString(int s) {
+  static const char *n[] = {"linear", "logarithmic", "quadratic", "cubic", "quartic"};
+  if (s < 0 || s > 4) throw Failure("ERROR in typeselect::getString : no string associated to type 'scale type'");
+  return n[s];
+}
+
+std::vector<double> genScale(double start, double stop, long N, int sct, bool cell) {
+  std::vector<double> v;
+  if (sct == LINEAR) {
+    double dx = (stop - start) / (N);
+    v.resize(N + 1);
+    for (long i = 0; i < N + 1; i++) v[i] = start + dx * i;
+  } else if (sct == LOGARITHMIC) {
+    if (start * stop <= 0) throw Failure("error: cannot generate logscale from a 0-crossing range.");
+    double s = (start < 0) ? -1 : 1;
+    v = genScale(std::log(std::fabs(start)), std::log(std::fabs(stop)), N, LINEAR, false);
+    for (long i = 0; i < N + 1; i++) v[i] = s * std::exp(v[i]);
+  } else {
+    int pp = sct;
+    v = genScale(std::pow(start, 1. / pp), std::pow(stop, 1. / pp), N, LINEAR, false);
+    for (long i = 0; i < N + 1; i++) v[i] = powiRef(v[i], pp);
+  }
+  if (cell) {
+    for (size_t j = 0; j + 1 < v.size(); j++) v[j] = (v[j] + v[j + 1]) * 0.5;
+    v.resize(v.size() - 1);
+  } else {
+    v.front() = start;
+    v.back() = stop;
+  }
+  return v;
+}
+
+double rescale(double x, int sct) {
+  if (sct == LOGARITHMIC) return std::log(x);
+  if (sct == LINEAR) return x;
+  return std::pow(x, 1. / sct);
+}
+
+void Grid21::init(const GridParams &gp) {
+  p = gp;
+  if (p.Uscale[0] != LINEAR)
+    throw Failure("ERROR: the v_r axis must use a linear scale (grid.Uscale)");
+  if (p.Pres < 3 || p.Ures[0] < 3 || p.Ures[1] < 1) throw Failure("ERROR: grid resolution too small");
+  R_V = genScale(p.Pmin, p.Pmax, p.Pres, p.Pscale, false);
+  R_Val = R_V;
+  U_V = genScale(p.Umin[0], p.Umax[0], p.Ures[0], p.Uscale[0], false);
+  U_Val = U_V;
+  J_V = genScale(p.Umin[1], p.Umax[1], p.Ures[1], p.Uscale[1], false);
+  J_Val = genScale(p.Umin[1], p.Umax[1], p.Ures[1], p.Uscale[1], true);
+  nR = (int)R_Val.size();
+  nU = (int)U_Val.size();
+  nJ = (int)J_Val.size();
+  nVal = (int64_t)nR * nU * nJ;
+  dR.assign(nR, 0.0);
+  dU.assign(nU, 0.0);
+  dJ.assign(nJ, 0.0);
+  for (int r = 0; r + 1 < nR; r++) dR[r] = R_V[r + 1] - R_V[r];
+  for (int u = 0; u + 1 < nU; u++) dU[u] = U_V[u + 1] - U_V[u];
+  // slice21Iterator::get_dJ tests j against the v vertex count
+  // (src/slice21Iterator.hxx:86): slices with j >= nU-1 weigh nothing
+  for (int j = 0; j < nJ; j++) dJ[j] = (j >= (int)U_V.size() - 1) ? 0 : (J_V[j + 1] - J_V[j]);
+  // spline knots (vlasovSolver21_spline::setup :304-313; localSpline::init :60-93)
+  int sc[2] = {p.Pscale, p.Uscale[0]};
+  const std::vector<double> *V[2] = {&R_V, &U_V};
+  long N[2] = {p.Pres, p.Ures[0]};
+  for (int d = 0; d < 2; d++) {
+    std::vector<double> scl = genScale(V[d]->front(), V[d]->back(), N[d], sc[d], false);
+    double start = rescale(scl.front(), sc[d]);
+    double stop = rescale(scl.back(), sc[d]);
+    double step = (stop - start) / (double)(scl.size() - 1);
+    splineStart[d] = start;
+    splineHinv[d] = 1. / step;
+  }
+}
+
+double Grid21::splineCoord(double pos, int dir) const {
+  // localSpline::pos2Coord before its modf (src/localSpline.hxx:352-356)
+  double q = rescale(pos, dir == 0 ? p.Pscale : p.Uscale[0]);
+  return (q - splineStart[dir]) * splineHinv[dir];
+}
+
+vls_geometry Grid21::geometry() const {
+  vls_geometry g{};
+  g.nR = nR;
+  g.nU = nU;
+  g.nJ = nJ;
+  g.R_Val = R_Val.data();
+  g.U_Val = U_Val.data();
+  g.J_Val = J_Val.data();
+  g.dR = dR.data();
+  g.dU = dU.data();
+  g.dJ = dJ.data();
+  for (int d = 0; d < 2; d++) {
+    g.spline_start[d] = splineStart[d];
+    g.spline_hinv[d] = splineHinv[d];
+  }
+  return g;
+}
+
+// ------------------------------------------------------------ generators
+
+void InitGen::generate(const Grid21 &g, double *f) const {
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < g.nJ; j++)
+    for (int u = 0; u < g.nU; u++)
+      for (int r = 0; r < g.nR; r++) f[g.index(j, u, r)] = valueAt(g.R_Val[r], g.U_Val[u], g.J_Val[j]);
+}
+
+namespace {
+
+// Uniform-density sphere with erf edge (initGen_UniformDensitySphere.hxx)
+class UDF : public InitGen {
+  double M = 1, R0 = 2, alpha = 0.5, sr = 0.5, gap[2] = {0, 0};
+  double rho0 = 0, C1 = 0, C2 = 0;
+
+ public:
+  std::string type() const override { return "UDF"; }
+  void setup(const Params &prm, const SolverParams &) override {
+    // :51-74
+    R0 = prm.get<double>("R0", "init", 2.0, 0);
+    M = prm.get<double>("M", "init", 1.0, 0);
+    alpha = prm.get<double>("alpha", "init", 0.5, 0);
+    sr = prm.get<double>("smoothingRadius", "init", 0.5, 0);
+    gap[0] = prm.get<double>("gap", "init", 0.0, 0);
+    gap[1] = prm.get<double>("gap", "init", 0.0, 1);
+    rho0 = (3 * M) / (4 * M_PI * powiRef(R0, 3));
+    double sig0 = std::sqrt((alpha) / (5 * R0));
+    C1 = rho0 * std::pow(2 * M_PI * sig0 * sig0, -1.5);
+    C2 = 2 * sig0 * sig0;
+  }
+  GridParams defaultGrid() const override {  // :82-102
+    GridParams g;
+    g.Pmin = 0.01; g.Pmax = 25;
+    g.Umin[0] = -2.0; g.Umax[0] = 2.0; g.Umin[1] = 0; g.Umax[1] = 1.6;
+    g.Pres = 200; g.Ures[0] = 200; g.Ures[1] = 200;
+    g.Pscale = LOGARITHMIC; g.Uscale[0] = LINEAR; g.Uscale[1] = QUADRATIC;
+    return g;
+  }
+  SolverParams defaultSolver() const override {  // :104-112
+    SolverParams s;
+    s.G = 1; s.T0 = 0; s.dt = 0.005; s.Tmax = 60.; s.nSpecies = 1;
+    return s;
+  }
+  double valueAt(double r, double u, double j) const override {  // computeF :158-173
+    double sm = 0.5 * (1. + std::erf((R0 - r) / sr));
+    double res = sm * C1 * std::exp(-(u * u + (j * j) / (r * r)) / C2);
+    if (gap[1] > 0) {
+      double sm0 = 0.5 * (1. + std::erf((gap[0] - r) / 0.15));
+      double sm1 = 0.5 * (1. + std::erf(-(gap[1] - r) / 0.15));
+      res *= (sm0 + sm1);
+    }
+    if (res < 1.E-200) res = 1.E-200;
+    return res;
+  }
+  double mass() const override { return M; }
+};
+
+// Power-law density sphere (initGen_PowerLawDensitySphere.hxx)
+class PLDF : public InitGen {
+  double M = 1, alpha = 2, eta = 0.1, n = 1, sr = 0.5, R0 = 10;
+  double rho0 = 0, sigO2 = 0, sigR2 = 0, C0 = 0, C1 = 0, C2 = 0;
+
+ public:
+  std::string type() const override { return "PLDF"; }
+  void setup(const Params &prm, const SolverParams &) override {
+    // :50-80
+    M = prm.get<double>("M", "init", 1.0, 0);
+    alpha = prm.get<double>("alpha", "init", 2.0, 0);
+    eta = prm.get<double>("eta", "init", 0.1, 0);
+    n = prm.get<double>("n", "init", 1.0, 0);
+    sr = prm.get<double>("smoothingRadius", "init", 0.5, 0);
+    R0 = 1. / eta;
+    R0 = prm.get<double>("R0", "init", R0, 0);
+    rho0 = ((3.0 - n) * M) / (4 * M_PI * std::pow(R0, 3.0 - n));
+    sigO2 = (3.0 - n) / (5.0 - 2 * n) * (M * eta) / (R0 * (alpha * 0.5 + 1));
+    sigR2 = sigO2 * 0.5 * alpha;
+    C0 = rho0 / (std::sqrt(2 * sigR2 * M_PI * M_PI * M_PI) * sigO2);
+    C1 = 1.0 / (2 * sigR2);
+    C2 = 1.0 / (sigO2);
+  }
+  GridParams defaultGrid() const override {  // :88-121
+    GridParams g;
+    double sigma2 = 0;
+    double sig2 = sigO2 + 2 * sigR2;
+    if (sigma2 < sig2) sigma2 = sig2;
+    g.Pmin = 0.01; g.Pmax = 200;
+    g.Umin[0] = -16 * std::sqrt(sigma2); g.Umax[0] = 16 * std::sqrt(sigma2);
+    g.Umin[1] = 0; g.Umax[1] = 1.6;
+    g.Pres = 200; g.Ures[0] = 200; g.Ures[1] = 200;
+    g.Pscale = LOGARITHMIC; g.Uscale[0] = LINEAR; g.Uscale[1] = QUADRATIC;
+    return g;
+  }
+  SolverParams defaultSolver() const override {
+    SolverParams s;
+    s.G = 1; s.T0 = 0; s.dt = 0.005; s.Tmax = 60.; s.nSpecies = 1;
+    return s;
+  }
+  double valueAt(double r, double u, double j) const override {  // computeF :181-189
+    double sm = 0.5 * (1. + std::erf((R0 - r) / sr));
+    double res = sm * C0 * std::exp(-u * u * C1 - ((j * j) / (r * r)) * C2) * std::pow(r, -n);
+    if (res < 1.E-200) res = 1.E-200;
+    return res;
+  }
+  double mass() const override { return M; }
+};
+
+// Hernquist sphere, isotropic or Osipkov-Merritt (initGen_Hernquist.hxx)
+class Hernquist : public InitGen {
+  double M = 1, R0 = 1, Ra = -1, G = 1, Vg = 1;
+
+  double phi(double R) const { return -(G * M) / (R0 + R); }  // :92-94
+  double F(double ph, double r, double u, double j) const {    // :133-154
+    double eps = ph + (u * u + (j * j) / (r * r)) * 0.5;
+    if (Ra > 0) eps = eps + (j * j) / (2 * Ra * Ra);
+    if (eps >= 0) return 0;
+    double q = std::sqrt(-(R0) / (G * M) * eps);
+    double q2 = q * q;
+    double A1 = (M) / (8. * std::sqrt(2.) * std::pow(M_PI * R0 * Vg, 3.)) * std::pow(1. - q2, -2.5);
+    double A2 = 3 * std::asin(q) + q * std::sqrt(1. - q2) * (1. - 2 * q2) * (8 * q2 * q2 - 8 * q2 - 3);
+    double result = A1 * A2;
+    if (Ra > 0) {
+      A1 = (M) / (std::sqrt(2.) * powiRef(M_PI * R0 * Vg, 3)) * powiRef(R0 / Ra, 2);
+      A2 = q * (1. - 2 * q2);
+      result += A1 * A2;
+    }
+    return result;
+  }
+
+ public:
+  std::string type() const override { return "Hernquist"; }
+  void setup(const Params &prm, const SolverParams &sp) override {  // :38-50
+    G = sp.G;
+    R0 = prm.get<double>("R0", "init", 1.0);
+    M = prm.get<double>("M", "init", 1.0);
+    Ra = prm.get<double>("Ra", "init", -1.0);
+    Vg = std::sqrt((G * M) / (R0));
+  }
+  GridParams defaultGrid() const override {  // :58-75
+    GridParams g;
+    g.Pmin = 0.01; g.Pmax = 25 * 10;
+    g.Umin[0] = -1.45; g.Umax[0] = 1.45; g.Umin[1] = 1.E-4; g.Umax[1] = 3.24 * 4;
+    g.Pres = 45 * 2; g.Ures[0] = 40 * 2; g.Ures[1] = 20 * 4;
+    g.Pscale = LOGARITHMIC; g.Uscale[0] = LINEAR; g.Uscale[1] = LOGARITHMIC;
+    return g;
+  }
+  SolverParams defaultSolver() const override {  // :77-85
+    SolverParams s;
+    s.G = 1; s.T0 = 0; s.Tmax = 62; s.dt = 0.025; s.nSpecies = 1;
+    return s;
+  }
+  double valueAt(double r, double u, double j) const override { return F(phi(r), r, u, j); }
+  void generate(const Grid21 &g, double *f) const override {  // :163-204
+    std::vector<double> Phi(g.nR);
+    for (int r = 0; r < g.nR; r++) Phi[r] = phi(g.R_Val[r]);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < g.nJ; j++)
+      for (int u = 0; u < g.nU; u++)
+        for (int r = 0; r < g.nR; r++)
+          f[g.index(j, u, r)] = F(Phi[r], g.R_Val[r], g.U_Val[u], g.J_Val[j]);
+  }
+  double mass() const override { return M; }
+};
+
+}  // namespace
+
+std::unique_ptr<InitGen> makeInitGen(const std::string &type) {
+  // initGenAll::get (src/initGen_all.hxx:39-55)
+  if (type == "UDF") return std::unique_ptr<InitGen>(new UDF());
+  if (type == "PLDF") return std::unique_ptr<InitGen>(new PLDF());
+  if (type == "Hernquist") return std::unique_ptr<InitGen>(new Hernquist());
+  throw Failure("ERROR: unknown initial conditions type " + type + "\n.");
+}
+
+}  // namespace vlh

@@ -1,0 +1,436 @@
+// C ABI of the device context (include/vlasolve_hip.h, layer 1).
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vls_internal.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+template <class F>
+int guarded(F &&fn) {
+  try {
+    fn();
+    return VLS_OK;
+  } catch (const vls::Error &e) {
+    return fail(e.code, e.msg);
+  } catch (const std::exception &e) {
+    return fail(VLS_ERR_ARG, e.what());
+  }
+}
+
+template <class T>
+T *dalloc(vls::Run &r, size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  VLS_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+  r.allocs.push_back(p);
+  return static_cast<T *>(p);
+}
+
+size_t kCoefPad(const vls::Run &r) { return 2 * (size_t)(r.nR + 2) + 8; }
+
+// A foot can be evaluated when its integer parts lie in [0, dim-1] on both
+// axes (dim-1 only on the last knot).
+bool foot_in_domain(double xr, double xu, const vls::Run &r) {
+  double a = std::trunc(xr), b = std::trunc(xu);
+  return a >= 0 && a <= r.nR - 1 && b >= 0 && b <= r.nU - 1;
+}
+
+template <class T>
+void h2d(T *dst, const T *src, size_t n, hipStream_t s) {
+  if (n) VLS_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+vls::Run &run_of(vls_ctx *ctx, int run);
+}  // namespace
+
+struct vls_ctx {
+  vls::Context c;
+};
+
+namespace {
+vls::Run &run_of(vls_ctx *ctx, int run) {
+  if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+  if (run < 0 || run >= (int)ctx->c.runs.size())
+    throw vls::Error(VLS_ERR_ARG, "run index out of range");
+  return ctx->c.runs[run];
+}
+}  // namespace
+
+namespace vls {
+
+// Runs the elimination of create_1D (src/localSpline.hxx:968-1153) on the
+// data-independent coefficients and records what the data path needs.
+void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
+                  std::vector<double> &d) {
+  const int M = dim + 2;
+  std::vector<double> A(4 * (size_t)M);
+  const double hi2 = hinv * hinv;
+  A[0] = hi2; A[1] = -2.0 * hi2; A[2] = hi2; A[3] = 0;
+  for (int i = 1; i < M - 1; i++) {
+    A[4 * i + 0] = 1.0 / 6.0;
+    A[4 * i + 1] = 4.0 / 6.0;
+    A[4 * i + 2] = 1.0 / 6.0;
+    A[4 * i + 3] = 0;
+  }
+  double *L = &A[4 * (M - 1)];
+  L[0] = hi2; L[1] = -2.0 * hi2; L[2] = hi2; L[3] = 0;
+  u.assign(M, 0.0);
+  d.assign(M, 0.0);
+  A[1] /= A[0];
+  A[2] /= A[0];
+  A[3] /= A[0];
+  A[0] = 1.0;
+  A[4 + 1] -= A[4 + 0] * A[1];
+  A[4 + 2] -= A[4 + 0] * A[2];
+  d[1] = A[4 + 1];
+  A[4 + 2] /= A[4 + 1];
+  u[1] = A[4 + 2];
+  for (int row = 2; row < M - 1; row++) {
+    double *a = &A[4 * (row - 1)];
+    a[4 + 1] -= a[4 + 0] * a[2];
+    d[row] = a[4 + 1];
+    a[4 + 2] /= a[4 + 1];
+    u[row] = a[4 + 2];
+  }
+  ax.M = M;
+  ax.c16 = 1.0 / 6.0;
+  ax.A01 = A[1];
+  ax.A02 = A[2];
+  ax.A03 = A[3];
+  ax.hi2 = L[0];
+  double p = L[1] - L[0] * A[4 * (M - 3) + 2];
+  double q = L[2] - p * A[4 * (M - 2) + 2];
+  ax.p = p;
+  ax.q = q;
+}
+
+}  // namespace vls
+
+extern "C" {
+
+const char *vls_last_error(void) { return g_err.c_str(); }
+
+int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vls_ctx **out) {
+  if (!out) return fail(VLS_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!g || !sp || nRuns < 1) return fail(VLS_ERR_ARG, "vls_create: need nRuns >= 1 geometries");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(VLS_ERR_DEVICE, "vls_create: no HIP device");
+  vls_ctx *ctx = new vls_ctx();
+  int rc = guarded([&] {
+    VLS_HIP_CHECK(hipGetDevice(&ctx->c.device));
+    VLS_HIP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    hipStream_t s = ctx->c.stream;
+    ctx->c.runs.resize(nRuns);
+    for (int k = 0; k < nRuns; k++) {
+      const vls_geometry &G = g[k];
+      vls::Run &r = ctx->c.runs[k];
+      if (G.nR < 4 || G.nU < 4 || G.nJ < 1)
+        throw vls::Error(VLS_ERR_ARG, "vls_create: grid too small (need nR, nU >= 4, nJ >= 1)");
+      if (!G.R_Val || !G.U_Val || !G.J_Val || !G.dR || !G.dU || !G.dJ)
+        throw vls::Error(VLS_ERR_ARG, "vls_create: missing geometry table");
+      r.nR = G.nR;
+      r.nU = G.nU;
+      r.nJ = G.nJ;
+      r.sp = sp[k];
+      r.nVal = (int64_t)r.nR * r.nU * r.nJ;
+      r.stride = r.nR + 2;
+      r.cs = (int64_t)(r.nR + 2) * (r.nU + 2);
+      r.start_v = G.spline_start[1];
+      r.hinv_v = G.spline_hinv[1];
+      r.Ulo = G.U_Val[0];
+      r.Uhi = G.U_Val[r.nU - 1];
+      r.f = dalloc<double>(r, r.nVal);
+      // A foot on the last knot (integer part dim-1, fraction 0) makes the
+      // 4x4 stencil touch one column past its padded row and one row past its
+      // slice, with weight zero, as in the reference; the tail pad keeps
+      // those reads inside the allocation.
+      r.c = dalloc<double>(r, (size_t)r.nJ * r.cs + kCoefPad(r));
+      VLS_HIP_CHECK(hipMemsetAsync(r.c, 0, ((size_t)r.nJ * r.cs + kCoefPad(r)) * sizeof(double), s));
+      r.xr = dalloc<double>(r, r.nVal);
+      r.xu = dalloc<double>(r, r.nVal);
+      r.flag = dalloc<uint8_t>(r, r.nVal);
+      r.P = dalloc<double>(r, (size_t)r.nJ * r.nR);
+      r.dM = dalloc<double>(r, r.nR);
+      r.Mx = dalloc<double>(r, r.nR);
+      r.acc = dalloc<double>(r, r.nR);
+      r.mTot = dalloc<double>(r, 1);
+      r.st_part = dalloc<double>(r, 5 * (size_t)r.nJ * r.nR);
+      r.st_q = dalloc<double>(r, 4 * (size_t)r.nJ + r.nR);
+      r.st_out = dalloc<double>(r, 8);
+      // tables
+      std::vector<double> Ct1(r.nJ), gfac(r.nR);
+      for (int j = 0; j < r.nJ; j++) Ct1[j] = 8 * M_PI * M_PI * G.J_Val[j] * G.dJ[j];
+      for (int i = 0; i < r.nR; i++) gfac[i] = r.sp.G / (G.R_Val[i] * G.R_Val[i]) * r.sp.dt;
+      r.R_Val = dalloc<double>(r, r.nR);
+      r.U_Val = dalloc<double>(r, r.nU);
+      r.J_Val = dalloc<double>(r, r.nJ);
+      r.dR = dalloc<double>(r, r.nR);
+      r.dU = dalloc<double>(r, r.nU);
+      r.Ct1 = dalloc<double>(r, r.nJ);
+      r.gfac = dalloc<double>(r, r.nR);
+      h2d(r.R_Val, G.R_Val, r.nR, s);
+      h2d(r.U_Val, G.U_Val, r.nU, s);
+      h2d(r.J_Val, G.J_Val, r.nJ, s);
+      h2d(r.dR, G.dR, r.nR, s);
+      h2d(r.dU, G.dU, r.nU, s);
+      h2d(r.Ct1, Ct1.data(), r.nJ, s);
+      h2d(r.gfac, gfac.data(), r.nR, s);
+      std::vector<double> u, d;
+      double *dev;
+      vls::thomas_table(r.nR, G.spline_hinv[0], r.tr, u, d);
+      dev = dalloc<double>(r, 2 * (size_t)r.tr.M);
+      h2d(dev, u.data(), r.tr.M, s);
+      h2d(dev + r.tr.M, d.data(), r.tr.M, s);
+      r.tr.u = dev;
+      r.tr.d = dev + r.tr.M;
+      VLS_HIP_CHECK(hipStreamSynchronize(s));  // host vectors are reused below
+      vls::thomas_table(r.nU, G.spline_hinv[1], r.tv, u, d);
+      dev = dalloc<double>(r, 2 * (size_t)r.tv.M);
+      h2d(dev, u.data(), r.tv.M, s);
+      h2d(dev + r.tv.M, d.data(), r.tv.M, s);
+      r.tv.u = dev;
+      r.tv.d = dev + r.tv.M;
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  });
+  if (rc) {
+    vls_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return VLS_OK;
+}
+
+void vls_destroy(vls_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  for (auto &r : ctx->c.runs)
+    for (void *p : r.allocs) (void)hipFree(p);
+  for (auto e : ctx->c.timing.ev) (void)hipEventDestroy(e);
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+int vls_upload_f(vls_ctx *ctx, int run, const double *f) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!f) throw vls::Error(VLS_ERR_ARG, "null f");
+    VLS_HIP_CHECK(hipMemcpyAsync(r.f, f, r.nVal * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->c.stream));
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    r.have_f = true;
+  });
+}
+
+int vls_download_f(vls_ctx *ctx, int run, double *f) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!f) throw vls::Error(VLS_ERR_ARG, "null f");
+    VLS_HIP_CHECK(hipMemcpyAsync(f, r.f, r.nVal * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->c.stream));
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, const uint8_t *flag) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!xr || !xu || !flag) throw vls::Error(VLS_ERR_ARG, "null foot table");
+    // every evaluated foot must index inside the padded coefficient array
+    for (int64_t i = 0; i < r.nVal; i++) {
+      if (flag[i] > VLS_FOOT_CHAIN) throw vls::Error(VLS_ERR_ARG, "invalid foot flag");
+      if (flag[i] != VLS_FOOT_EVAL) continue;
+      if (!foot_in_domain(xr[i], xu[i], r))
+        throw vls::Error(VLS_ERR_ARG, "vls_set_drift: foot outside the spline domain at point " +
+                                          std::to_string(i));
+    }
+    hipStream_t s = ctx->c.stream;
+    h2d(r.xr, xr, r.nVal, s);
+    h2d(r.xu, xu, r.nVal, s);
+    h2d(r.flag, flag, r.nVal, s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    r.have_drift = true;
+  });
+}
+
+int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
+                   const int32_t *len, const int32_t *head, const double *back_xr,
+                   const double *back_xu, const uint8_t *back_out, const double *vals) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (nChains < 0) throw vls::Error(VLS_ERR_ARG, "negative chain count");
+    if (nChains > 0 && (!point || !len || !head || !back_xr || !back_xu || !back_out || !vals))
+      throw vls::Error(VLS_ERR_ARG, "null chain table");
+    std::vector<int64_t> off(nChains);
+    int64_t cells = 0;
+    for (int64_t k = 0; k < nChains; k++) {
+      if (len[k] < 2) throw vls::Error(VLS_ERR_ARG, "chain shorter than 2 cells");
+      if (head[k] < 0 || head[k] >= len[k]) throw vls::Error(VLS_ERR_ARG, "chain head out of range");
+      if (point[k] < 0 || point[k] >= r.nVal) throw vls::Error(VLS_ERR_ARG, "chain point out of range");
+      if (!back_out[k] && !foot_in_domain(back_xr[k], back_xu[k], r))
+        throw vls::Error(VLS_ERR_ARG, "chain back cell outside the spline domain");
+      off[k] = cells;
+      cells += len[k];
+    }
+    // replace any previous chain arrays
+    hipStream_t s = ctx->c.stream;
+    r.nChains = nChains;
+    r.nCells = cells;
+    r.ch_point = dalloc<int64_t>(r, nChains);
+    r.ch_off = dalloc<int64_t>(r, nChains);
+    r.ch_len = dalloc<int32_t>(r, nChains);
+    r.ch_head = dalloc<int32_t>(r, nChains);
+    r.ch_xr = dalloc<double>(r, nChains);
+    r.ch_xu = dalloc<double>(r, nChains);
+    r.ch_out = dalloc<uint8_t>(r, nChains);
+    r.ch_vals = dalloc<double>(r, cells);
+    h2d(r.ch_point, point, nChains, s);
+    h2d(r.ch_off, off.data(), nChains, s);
+    h2d(r.ch_len, len, nChains, s);
+    h2d(r.ch_head, head, nChains, s);
+    h2d(r.ch_xr, back_xr, nChains, s);
+    h2d(r.ch_xu, back_xu, nChains, s);
+    h2d(r.ch_out, back_out, nChains, s);
+    h2d(r.ch_vals, vals, cells, s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    r.have_chains = true;
+  });
+}
+
+int vls_get_chains(vls_ctx *ctx, int run, int32_t *head, double *vals) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    hipStream_t s = ctx->c.stream;
+    if (head && r.nChains)
+      VLS_HIP_CHECK(hipMemcpyAsync(head, r.ch_head, r.nChains * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
+    if (vals && r.nCells)
+      VLS_HIP_CHECK(hipMemcpyAsync(vals, r.ch_vals, r.nCells * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_step(vls_ctx *ctx, int nSteps) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    hipStream_t s = ctx->c.stream;
+    vls::Timing *t = &ctx->c.timing;
+    if (t->on) {
+      for (auto e : t->ev) (void)hipEventDestroy(e);
+      t->ev.clear();
+      t->names.clear();
+      vls::time_mark(t, s, "start");
+    }
+    for (auto &r : ctx->c.runs)
+      if (!r.have_f || !r.have_drift)
+        throw vls::Error(VLS_ERR_STATE, "vls_step: f and drift feet must be set first");
+    for (int n = 0; n < nSteps; n++) {
+      for (auto &r : ctx->c.runs) {
+        if (r.sp.no_linking) {
+          vls::launch_spline(r, r.f, s, t);
+          vls::launch_chains(r, s);
+          vls::launch_drift(r, s);
+          vls::time_mark(t, s, "drift");
+        }
+        vls::launch_moment(r, s);
+        vls::time_mark(t, s, "moment");
+        vls::launch_spline(r, r.f, s, t);
+        vls::launch_kick(r, s);
+        vls::time_mark(t, s, "kick");
+        vls::launch_spline(r, r.f, s, t);
+        vls::launch_chains(r, s);
+        vls::time_mark(t, s, "chains");
+        vls::launch_drift(r, s);
+        vls::time_mark(t, s, "drift");
+        r.stepped = true;
+      }
+    }
+    VLS_HIP_CHECK(hipGetLastError());
+  });
+}
+
+int vls_synchronize(vls_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int vls_moments(vls_ctx *ctx, int run, double *dM, double *accel, double *mTot) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!r.stepped) throw vls::Error(VLS_ERR_STATE, "vls_moments: no step taken yet");
+    hipStream_t s = ctx->c.stream;
+    if (dM) VLS_HIP_CHECK(hipMemcpyAsync(dM, r.dM, r.nR * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (accel)
+      VLS_HIP_CHECK(hipMemcpyAsync(accel, r.acc, r.nR * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (mTot) VLS_HIP_CHECK(hipMemcpyAsync(mTot, r.mTot, sizeof(double), hipMemcpyDeviceToHost, s));
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_stats(vls_ctx *ctx, int run, double out[7]) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!out) throw vls::Error(VLS_ERR_ARG, "null output");
+    if (!r.have_f) throw vls::Error(VLS_ERR_STATE, "vls_stats: f not set");
+    hipStream_t s = ctx->c.stream;
+    vls::launch_stats(r, s);
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipMemcpyAsync(out, r.st_out, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_download_coef(vls_ctx *ctx, int run, int j, double *coef) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (j < 0 || j >= r.nJ || !coef) throw vls::Error(VLS_ERR_ARG, "bad slice or null output");
+    VLS_HIP_CHECK(hipMemcpyAsync(coef, r.c + (int64_t)j * r.cs, r.cs * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->c.stream));
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int vls_set_timing(vls_ctx *ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    ctx->c.timing.on = on != 0;
+  });
+}
+
+int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n) {
+  return guarded([&] {
+    if (!ctx || !n) throw vls::Error(VLS_ERR_ARG, "null argument");
+    vls::Timing &t = ctx->c.timing;
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    int cnt = t.ev.empty() ? 0 : (int)t.ev.size() - 1;
+    int cap = *n;
+    for (int k = 0; k < cnt && k < cap; k++) {
+      float x = 0;
+      VLS_HIP_CHECK(hipEventElapsedTime(&x, t.ev[k], t.ev[k + 1]));
+      if (names) names[k] = t.names[k + 1];
+      if (ms) ms[k] = x;
+    }
+    *n = cnt;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Internal types shared by the device code and the C ABI of the spherical
+// spline step.  Not part of the public interface (include/vlasolve_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vlasolve_hip.h"
+
+namespace vls {
+
+// Data-independent part of the natural cubic B-spline Thomas solve along one
+// axis (localSpline::create_1D with BT_NATURAL at both ends,
+// src/localSpline.hxx:953-1182).  The reference rebuilds these numbers in
+// every call; they depend only on the knot spacing, so they are computed once
+// on the host by the very same operations and read from a table.
+struct ThomasAxis {
+  int M;        // dim + 2 rows (padded line length)
+  double c16;   // 1.0/6.0: sub-diagonal of interior rows
+  double A01;   // first row after normalisation: coefficient of c1
+  double A02;   //                                coefficient of c2
+  double A03;   //                                right-hand side
+  double hi2;   // last row, coefficient of c_{M-3}
+  double p;     // last row, coefficient of c_{M-2} after eliminating c_{M-3}
+  double q;     // last row, diagonal after eliminating c_{M-2}
+  const double *u;  // device [M]: super-diagonal after division, rows 1..M-2
+  const double *d;  // device [M]: divisor of row k before normalisation
+};
+
+// Build the table on the host: u[k], d[k] for k=1..M-2 and the constants.
+void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
+                  std::vector<double> &d);
+
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<const char *> names;
+  std::vector<double> ms;
+};
+
+// One run of an ensemble: every array of its state in HBM.
+struct Run {
+  int nR = 0, nU = 0, nJ = 0;
+  int64_t nVal = 0;
+  int64_t cs = 0;       // padded coefficient slice size (nU+2)*(nR+2)
+  int stride = 0;       // padded row stride nR+2
+  vls_solver_params sp{};
+  double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
+
+  // state
+  double *f = nullptr;      // [nJ][nU][nR]
+  double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
+  // static feet
+  double *xr = nullptr, *xu = nullptr;
+  uint8_t *flag = nullptr;
+  // moment
+  double *P = nullptr;      // [nJ][nR] per-slice partials
+  double *dM = nullptr, *Mx = nullptr, *acc = nullptr, *mTot = nullptr;
+  // tables
+  double *R_Val = nullptr, *U_Val = nullptr, *J_Val = nullptr;
+  double *dR = nullptr, *dU = nullptr, *Ct1 = nullptr, *gfac = nullptr;
+  ThomasAxis tr{}, tv{};
+  // delay lines
+  int64_t nChains = 0, nCells = 0;
+  int64_t *ch_point = nullptr, *ch_off = nullptr;
+  int32_t *ch_len = nullptr, *ch_head = nullptr;
+  double *ch_xr = nullptr, *ch_xu = nullptr, *ch_vals = nullptr;
+  uint8_t *ch_out = nullptr;
+  // statistics
+  double *st_part = nullptr;  // [5][nJ][nR]
+  double *st_q = nullptr;     // [4][nJ]
+  double *st_out = nullptr;   // [8]
+
+  bool have_f = false, have_drift = false, have_chains = false, stepped = false;
+  std::vector<void *> allocs;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Run> runs;
+  Timing timing;
+};
+
+// kernel launchers (vls_kernels.hip)
+void launch_moment(Run &r, hipStream_t s);
+void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
+void launch_kick(Run &r, hipStream_t s);
+void launch_drift(Run &r, hipStream_t s);
+void launch_chains(Run &r, hipStream_t s);
+void launch_stats(Run &r, hipStream_t s);
+
+void time_mark(Timing *t, hipStream_t s, const char *name);
+
+}  // namespace vls
+
+#define VLS_HIP_CHECK(expr)                                                         \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      throw vls::Error(VLS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace vls {
+struct Error {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+}  // namespace vls
