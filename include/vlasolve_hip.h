@@ -136,6 +136,17 @@ int vls_download_coef(vls_ctx *ctx, int run, int j, double *coef);
 int vls_set_timing(vls_ctx *ctx, int on);
 int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n);
 
+/* Host-side helpers for tests (no device needed).  vls_thomas_table returns
+ * the data-independent factors of the natural spline solve of one axis
+ * (u, d, rd of dim+2 entries; consts = {c16, A01, A02, A03, hi2, p, q, rq});
+ * vls_check_division counts the numerators for which the known-divisor
+ * division of the line solves differs from IEEE division. */
+int vls_thomas_table(int dim, double hinv, double *u, double *d, double *rd, double consts[8]);
+/* One-line description of the current device (CUs, clock, LDS, L2). */
+int vls_device_info(char *buf, int len);
+int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
+                       int64_t *mismatches);
+
 /* ---------------------------------------------------------------- layer 2 */
 
 typedef struct vlsd_solver vlsd_solver;

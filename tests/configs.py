@@ -43,3 +43,8 @@ SMALL_CASES = {
     "udf_buffer_gap": (UDF_BUFFER, 40),
     "udf_dj_quirk": (UDF_DJ_QUIRK, 40),
 }
+
+# a shape without a register line-solver instantiation: exercises the
+# LDS-tiled spline kernels
+UDF_ODD = dict(UDF_C1, **{"grid.Pres": 40, "grid.Ures": [36, 10]})
+SMALL_CASES["udf_odd_shape"] = (UDF_ODD, 30)

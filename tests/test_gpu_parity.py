@@ -64,3 +64,20 @@ def test_against_reference_order(name):
     sg, sr = gpu.stats(), ref.stats()
     for key in sg:
         assert abs(sg[key] - sr[key]) <= 1e-11 * max(abs(sr[key]), 1e-300), key
+
+
+def test_c2_shape_bit_exact():
+    """The 201 x 201 x 200 grid of the throughput benchmark, a few steps."""
+    from tests.configs import UDF_C2
+
+    gpu = V.Solver(UDF_C2)
+    orc = O.Oracle(UDF_C2, nthreads=0)
+    for k in (1, 3):
+        gpu.step(k)
+        orc.step(k)
+        fg, fo = gpu.f(), orc.f()
+        bad = np.count_nonzero(fg != fo)
+        assert bad == 0, f"{bad} points differ"
+    dMg, ag, mg = gpu.moments()
+    dMo, ao, mo = orc.last_moments()
+    assert same(dMg, dMo) and same(ag, ao) and mg == mo

@@ -5,10 +5,12 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "vls_div.hpp"
 #include "vls_internal.hpp"
 
 namespace {
@@ -75,7 +77,7 @@ namespace vls {
 // Runs the elimination of create_1D (src/localSpline.hxx:968-1153) on the
 // data-independent coefficients and records what the data path needs.
 void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
-                  std::vector<double> &d) {
+                  std::vector<double> &d, std::vector<double> &rd) {
   const int M = dim + 2;
   std::vector<double> A(4 * (size_t)M);
   const double hi2 = hinv * hinv;
@@ -116,6 +118,18 @@ void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
   double q = L[2] - p * A[4 * (M - 2) + 2];
   ax.p = p;
   ax.q = q;
+  ax.rq = 1.0 / q;
+  rd.assign(M, 0.0);
+  for (int k = 1; k <= M - 2; k++) rd[k] = 1.0 / d[k];
+  // first row of the constant regime (the table is used before it)
+  int K0 = M - 2;
+  while (K0 > 1 && d[K0 - 1] == d[M - 2] && u[K0 - 1] == u[M - 2] && rd[K0 - 1] == rd[M - 2]) K0--;
+  if (K0 - 1 > 16)  // kHead of the kernels
+    throw Error(VLS_ERR_ARG, "spline pivots do not settle within 16 rows");
+  ax.K0 = K0;
+  ax.dS = d[M - 2];
+  ax.rdS = rd[M - 2];
+  ax.uS = u[M - 2];
 }
 
 }  // namespace vls
@@ -123,6 +137,94 @@ void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
 extern "C" {
 
 const char *vls_last_error(void) { return g_err.c_str(); }
+
+// Short description of device 0 (name, CUs, clock, LDS, L2) for reports.
+int vls_device_info(char *buf, int len) {
+  return guarded([&] {
+    if (!buf || len < 1) throw vls::Error(VLS_ERR_ARG, "bad buffer");
+    int dev = 0, cus = 0, clk = 0, ldsBlk = 0, ldsOpt = 0, ldsCU = 0, l2 = 0, mclk = 0, bus = 0;
+    VLS_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    VLS_HIP_CHECK(hipGetDeviceProperties(&p, dev));
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev);
+    hipDeviceGetAttribute(&ldsBlk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    hipDeviceGetAttribute(&ldsOpt, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+    hipDeviceGetAttribute(&ldsCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+    hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, dev);
+    hipDeviceGetAttribute(&mclk, hipDeviceAttributeMemoryClockRate, dev);
+    hipDeviceGetAttribute(&bus, hipDeviceAttributeMemoryBusWidth, dev);
+    snprintf(buf, len,
+             "%s %s: %d CUs, %d MHz, LDS %d B/block (opt-in %d, %d per CU), L2 %d B, "
+             "memory %d MHz x %d bit",
+             p.name, p.gcnArchName, cus, clk / 1000, ldsBlk, ldsOpt, ldsCU, l2, mclk / 1000, bus);
+  });
+}
+
+// The data-independent Thomas table of one axis (dim nodes, knot spacing
+// 1/hinv): u, d, rd have dim+2 entries; consts = {c16, A01, A02, A03, hi2, p,
+// q, rq}.
+int vls_thomas_table(int dim, double hinv, double *u, double *d, double *rd, double consts[8]) {
+  return guarded([&] {
+    if (dim < 2 || !u || !d || !rd || !consts) throw vls::Error(VLS_ERR_ARG, "bad arguments");
+    vls::ThomasAxis ax{};
+    std::vector<double> U, D, RD;
+    vls::thomas_table(dim, hinv, ax, U, D, RD);
+    std::copy(U.begin(), U.end(), u);
+    std::copy(D.begin(), D.end(), d);
+    std::copy(RD.begin(), RD.end(), rd);
+    double c[8] = {ax.c16, ax.A01, ax.A02, ax.A03, ax.hi2, ax.p, ax.q, ax.rq};
+    std::copy(c, c + 8, consts);
+  });
+}
+
+// Host check of the known-divisor division against IEEE division: for each of
+// the nb divisors, `samples` numerators spread over many binades (random
+// significands and signs, exact multiples, values next to multiples).
+int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
+                       int64_t *mismatches) {
+  if (!b || nb < 0 || samples < 0 || !mismatches) return fail(VLS_ERR_ARG, "bad arguments");
+  uint64_t x = seed ? seed : 0x9E3779B97F4A7C15ull;
+  auto next = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  int64_t bad = 0;
+  for (int i = 0; i < nb; i++) {
+    const double bi = b[i], yb = 1.0 / bi;
+    for (int64_t k = 0; k < samples; k++) {
+      uint64_t r = next();
+      double a;
+      switch (k % 4) {
+        case 0: {  // random significand, exponent in [-600, 600]
+          uint64_t mant = r & ((1ull << 52) - 1);
+          int e = (int)((r >> 52) % 1201) - 600;
+          a = std::ldexp(1.0 + (double)mant / 4503599627370496.0, e);
+          break;
+        }
+        case 1:  // exact multiple
+          a = bi * (double)(int64_t)((r % 2000001) - 1000000);
+          break;
+        case 2: {  // next to a multiple
+          double m = bi * (double)(int64_t)((r % 2000001) - 1000000);
+          a = std::nextafter(m, (r >> 40) & 1 ? 1e300 : -1e300);
+          break;
+        }
+        default: {  // random in [-10, 10]
+          a = ((double)(r >> 11) / 9007199254740992.0 - 0.5) * 20.0;
+          break;
+        }
+      }
+      if ((r >> 63) & 1) a = -a;
+      double q = a / bi, p = vls::div_known(a, bi, yb);
+      if (!(q == p)) bad++;
+    }
+  }
+  *mismatches = bad;
+  return VLS_OK;
+}
 
 int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vls_ctx **out) {
   if (!out) return fail(VLS_ERR_ARG, "null output pointer");
@@ -134,6 +236,12 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
   vls_ctx *ctx = new vls_ctx();
   int rc = guarded([&] {
     VLS_HIP_CHECK(hipGetDevice(&ctx->c.device));
+    int maxLds = 0;
+    VLS_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeSharedMemPerBlockOptin,
+                                        ctx->c.device));
+    if (maxLds <= 0)
+      VLS_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                          ctx->c.device));
     VLS_HIP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     hipStream_t s = ctx->c.stream;
     ctx->c.runs.resize(nRuns);
@@ -155,6 +263,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.hinv_v = G.spline_hinv[1];
       r.Ulo = G.U_Val[0];
       r.Uhi = G.U_Val[r.nU - 1];
+      vls::configure_tiles(r, maxLds);
       r.f = dalloc<double>(r, r.nVal);
       // A foot on the last knot (integer part dim-1, fraction 0) makes the
       // 4x4 stencil touch one column past its padded row and one row past its
@@ -191,22 +300,19 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       h2d(r.dU, G.dU, r.nU, s);
       h2d(r.Ct1, Ct1.data(), r.nJ, s);
       h2d(r.gfac, gfac.data(), r.nR, s);
-      std::vector<double> u, d;
-      double *dev;
-      vls::thomas_table(r.nR, G.spline_hinv[0], r.tr, u, d);
-      dev = dalloc<double>(r, 2 * (size_t)r.tr.M);
-      h2d(dev, u.data(), r.tr.M, s);
-      h2d(dev + r.tr.M, d.data(), r.tr.M, s);
-      r.tr.u = dev;
-      r.tr.d = dev + r.tr.M;
-      VLS_HIP_CHECK(hipStreamSynchronize(s));  // host vectors are reused below
-      vls::thomas_table(r.nU, G.spline_hinv[1], r.tv, u, d);
-      dev = dalloc<double>(r, 2 * (size_t)r.tv.M);
-      h2d(dev, u.data(), r.tv.M, s);
-      h2d(dev + r.tv.M, d.data(), r.tv.M, s);
-      r.tv.u = dev;
-      r.tv.d = dev + r.tv.M;
-      VLS_HIP_CHECK(hipStreamSynchronize(s));
+      for (int axis = 0; axis < 2; axis++) {
+        std::vector<double> u, d, rd;
+        vls::ThomasAxis &ax = axis == 0 ? r.tr : r.tv;
+        vls::thomas_table(axis == 0 ? r.nR : r.nU, G.spline_hinv[axis], ax, u, d, rd);
+        double *dev = dalloc<double>(r, 3 * (size_t)ax.M);
+        h2d(dev, u.data(), ax.M, s);
+        h2d(dev + ax.M, d.data(), ax.M, s);
+        h2d(dev + 2 * ax.M, rd.data(), ax.M, s);
+        ax.u = dev;
+        ax.d = dev + ax.M;
+        ax.rd = dev + 2 * ax.M;
+        VLS_HIP_CHECK(hipStreamSynchronize(s));  // u, d, rd go out of scope
+      }
     }
   });
   if (rc) {

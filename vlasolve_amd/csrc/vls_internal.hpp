@@ -26,13 +26,20 @@ struct ThomasAxis {
   double hi2;   // last row, coefficient of c_{M-3}
   double p;     // last row, coefficient of c_{M-2} after eliminating c_{M-3}
   double q;     // last row, diagonal after eliminating c_{M-2}
-  const double *u;  // device [M]: super-diagonal after division, rows 1..M-2
-  const double *d;  // device [M]: divisor of row k before normalisation
+  double rq;    // RN(1/q)
+  // The pivot recurrence d_k = 4/6 - u_{k-1}/6, u_k = (1/6)/d_k does not
+  // depend on the data or the knot spacing and reaches an exact fixed point:
+  // from row K0 on, d_k = dS, RN(1/d_k) = rdS and u_k = uS.
+  int K0;
+  double dS, rdS, uS;
+  const double *u;   // device [M]: super-diagonal after division, rows 1..M-2
+  const double *d;   // device [M]: divisor of row k before normalisation
+  const double *rd;  // device [M]: RN(1/d[k])
 };
 
-// Build the table on the host: u[k], d[k] for k=1..M-2 and the constants.
+// Build the table on the host: u[k], d[k], rd[k] for k=1..M-2 and the constants.
 void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
-                  std::vector<double> &d);
+                  std::vector<double> &d, std::vector<double> &rd);
 
 struct Timing {
   bool on = false;
@@ -47,6 +54,8 @@ struct Run {
   int64_t nVal = 0;
   int64_t cs = 0;       // padded coefficient slice size (nU+2)*(nR+2)
   int stride = 0;       // padded row stride nR+2
+  int TR = 32, LS = 0;  // r-sweep tile: rows per block, odd LDS row stride (>= TR)
+  int TC = 32;          // v-sweep tile: columns per block
   vls_solver_params sp{};
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
 
@@ -86,6 +95,7 @@ struct Context {
 };
 
 // kernel launchers (vls_kernels.hip)
+void configure_tiles(Run &r, int maxLdsPerBlock);
 void launch_moment(Run &r, hipStream_t s);
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 void launch_kick(Run &r, hipStream_t s);

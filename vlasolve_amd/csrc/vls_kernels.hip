@@ -9,6 +9,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
+#include "vls_div.hpp"
 #include "vls_internal.hpp"
 
 namespace vls {
@@ -33,11 +36,14 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
 
 // ---------------------------------------------------------------- moment
 // Per (j, r) partial of quadrature21::integrate1D (src/quadrature21.hxx:104-133):
-// sum over u < nU-1 of ((Ct1_j*dU_u)*dR_r) * avg4, serial in u.
+// sum over u < nU-1 of ((Ct1_j*dU_u)*dR_r) * avg4, serial in u.  The row
+// loads of the next kU iterations are issued before their additions so that
+// the memory latency overlaps the serial sum.
 __global__ void __launch_bounds__(kBlock)
 k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
                  const double *__restrict__ dU, const double *__restrict__ dR,
                  double *__restrict__ P, int nR, int nU, int nJ) {
+  constexpr int kU = 8;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (r >= nR) return;
@@ -45,10 +51,27 @@ k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
   if (r < nR - 1) {
     const double ct = Ct1[j];
     const double dr = dR[r];
-    const double *p = f + (int64_t)j * nU * nR + r;
+    const double *p = f + (size_t)j * nU * nR + r;
     double a0 = p[0], a1 = p[1];
-    for (int u = 0; u < nU - 1; u++) {
-      const double *q = p + (int64_t)(u + 1) * nR;
+    int u = 0;
+    for (; u + kU <= nU - 1; u += kU) {
+      double b0[kU], b1[kU];
+#pragma unroll
+      for (int k = 0; k < kU; k++) {
+        const double *q = p + (size_t)(u + k + 1) * nR;
+        b0[k] = q[0];
+        b1[k] = q[1];
+      }
+#pragma unroll
+      for (int k = 0; k < kU; k++) {
+        double avg = (((a0 + a1) + b0[k]) + b1[k]) * 0.25;
+        acc += ((ct * dU[u + k]) * dr) * avg;
+        a0 = b0[k];
+        a1 = b1[k];
+      }
+    }
+    for (; u < nU - 1; u++) {
+      const double *q = p + (size_t)(u + 1) * nR;
       double b0 = q[0], b1 = q[1];
       double avg = (((a0 + a1) + b0) + b1) * 0.25;
       acc += ((ct * dU[u]) * dr) * avg;
@@ -56,90 +79,334 @@ k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
       a1 = b1;
     }
   }
-  P[(int64_t)j * nR + r] = acc;
+  P[(size_t)j * nR + r] = acc;
 }
 
 // dM[r] = serial sum over j; exclusive radial prefix (gridHandler::accumulate
 // with addSpare=1, src/grid_handler.hxx:350-407); acceleration
-// M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).
-__global__ void __launch_bounds__(kBlock)
+// M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).  One block.
+__global__ void __launch_bounds__(1024)
 k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, double *__restrict__ dM,
               double *__restrict__ Mx, double *__restrict__ acc, double *__restrict__ mTot, int nR,
               int nJ) {
+  extern __shared__ double sdm[];  // [nR]
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
+    constexpr int kJ = 8;
     double s = 0.0;
-    for (int j = 0; j < nJ; j++) s += P[(int64_t)j * nR + r];
+    int j = 0;
+    for (; j + kJ <= nJ; j += kJ) {
+      double v[kJ];
+#pragma unroll
+      for (int k = 0; k < kJ; k++) v[k] = P[(size_t)(j + k) * nR + r];
+#pragma unroll
+      for (int k = 0; k < kJ; k++) s += v[k];
+    }
+    for (; j < nJ; j++) s += P[(size_t)j * nR + r];
     dM[r] = s;
+    sdm[r] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double prev = 0.0;
     for (int r = 0; r < nR; r++) {
-      Mx[r] = prev;
-      prev += dM[r];
+      double d = sdm[r];
+      sdm[r] = prev;
+      prev += d;
     }
     *mTot = prev;
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < nR; r += blockDim.x) acc[r] = Mx[r] * gfac[r];
+  for (int r = threadIdx.x; r < nR; r += blockDim.x) {
+    double m = sdm[r];
+    Mx[r] = m;
+    acc[r] = m * gfac[r];
+  }
 }
 
 // ---------------------------------------------------------------- spline
-// Thomas solve of one padded line in place (create_1D with natural rows,
-// src/localSpline.hxx:1045-1172).  rhs(k) for k = 1..M-2 is read through
-// `in` with stride `is`; the M coefficients are written through `out` with
-// stride `os` (out may alias in: each rhs is read before its slot is written).
-__device__ __forceinline__ void thomas_line(const double *in, int64_t is, double *out, int64_t os,
-                                            const ThomasAxis &ax) {
-  const int M = ax.M;
-  double y = (in[0] - ax.c16 * ax.A03) / ax.d[1];
-  out[os] = y;
-  double ym2 = 0.0, ym1 = y;  // y_{k-2}, y_{k-1}
-  for (int k = 2; k <= M - 2; k++) {
-    double yk = (in[(int64_t)(k - 1) * is] - ax.c16 * ym1) / ax.d[k];
-    out[(int64_t)k * os] = yk;
-    ym2 = ym1;
-    ym1 = yk;
+// The natural cubic B-spline solve of one padded line (create_1D with
+// BT_NATURAL rows, src/localSpline.hxx:1045-1172), split in its two sweeps.
+//
+// Forward elimination of row k (k = 1..M-2): y_k = (rhs_k - c16*y_{k-1})/d_k,
+// with y_0's contribution c16*A03; the last row then folds in y_{M-3} and
+// y_{M-2}.  Back substitution: c_k = y_k - u_k*c_{k+1}, and the first row
+// c_0 = (A03 - A01*c_1) - A02*c_2.
+
+// The rows before the constant regime (1..K0-1, at most kHead of them) form a
+// head whose pivots and right-hand sides are all loaded up front (the table
+// through per-lane vector loads, index + lz with lz a zero the compiler
+// cannot see, so that no scalar-memory wait enters the loops).  From row K0
+// on the pivots are the scalars dS, rdS, uS: the steady loops load nothing
+// but the data stream, kPF rows ahead.
+constexpr int kHead = 16;
+
+// Forward elimination over rows 1..M-2.  RHS(k) returns rhs_k (any k <= M-2),
+// STORE(k, y) keeps y_k.  Row 1 takes y_0's contribution A03 in place of
+// y_{k-1}.  Returns y_{M-3} and y_{M-2} through ym2/ym1.
+template <int kPF, class RHS, class STORE>
+__device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS rhs, STORE store,
+                                               double &ym2, double &ym1) {
+  const int M = ax.M, P = ax.K0 - 1 < M - 2 ? ax.K0 - 1 : M - 2;  // head rows 1..P
+  auto kc = [&](int k) { return k <= M - 2 ? k : M - 2; };
+  double hr[kHead], hd[kHead], he[kHead], rb[kPF];
+#pragma unroll
+  for (int q = 0; q < kHead; q++) {
+    hr[q] = rhs(kc(1 + q));
+    hd[q] = ax.d[kc(1 + q) + lz];
+    he[q] = ax.rd[kc(1 + q) + lz];
   }
-  // last row: ym2 = y_{M-3}, ym1 = y_{M-2}
-  double t = 0.0 - ax.hi2 * ym2;
-  t = t - ax.p * ym1;
-  double cn = t / ax.q;
-  out[(int64_t)(M - 1) * os] = cn;
-  double c1 = 0.0, c2 = cn;
-  for (int k = M - 2; k >= 1; k--) {
-    double ck = out[(int64_t)k * os] - ax.u[k] * cn;
-    out[(int64_t)k * os] = ck;
-    c2 = cn;
-    cn = ck;
-    c1 = ck;
+#pragma unroll
+  for (int q = 0; q < kPF; q++) rb[q] = rhs(kc(P + 1 + q));
+  // loop constants as per-lane values (x + lz == x): their scalar loads are
+  // waited for here, once, instead of inside every guarded loop body
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  ym2 = 0.0;
+  ym1 = ax.A03;
+#pragma unroll
+  for (int q = 0; q < kHead; q++) {
+    if (q < P) {
+      double yk = div_known(hr[q] - c16 * ym1, hd[q], he[q]);
+      store(1 + q, yk);
+      ym2 = ym1;
+      ym1 = yk;
+    }
   }
-  out[0] = (ax.A03 - ax.A01 * c1) - ax.A02 * c2;
+  for (int k = P + 1; k <= M - 2; k += kPF) {
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {  // rhs_kk sits in slot (kk-P-1) % kPF == q
+      const int kk = k + q;
+      if (kk <= M - 2) {
+        double yk = div_known(rb[q] - c16 * ym1, dS, rdS);
+        store(kk, yk);
+        ym2 = ym1;
+        ym1 = yk;
+      }
+      rb[q] = rhs(kc(kk + kPF));
+    }
+  }
 }
 
-// r-direction solves of the nU interior rows of every slice
-// (create_2D first loop, src/localSpline.hxx:1248-1250).
-__global__ void __launch_bounds__(kBlock)
+// Back substitution c_k = y_k - u_k c_{k+1} for k = M-2..1 and the first row,
+// kPF rows ahead.  YLOAD(k) returns y_k, OUT(k, c) stores c_k (k = 0..M-1).
+template <int kPF, class YLOAD, class OUT>
+__device__ __forceinline__ void thomas_backward(const ThomasAxis &ax, int lz, double ym2, double ym1,
+                                                YLOAD yload, OUT out) {
+  const int M = ax.M, P = ax.K0 - 1 < M - 2 ? ax.K0 - 1 : M - 2;
+  double hu[kHead];
+#pragma unroll
+  for (int q = 0; q < kHead; q++) hu[q] = ax.u[(1 + q <= M - 2 ? 1 + q : M - 2) + lz];
+  double tt = 0.0 - ax.hi2 * ym2;
+  tt = tt - ax.p * ym1;
+  double cn = div_known(tt, ax.q, ax.rq);
+  out(M - 1, cn);
+  auto kc = [&](int k) { return k >= 1 ? k : 1; };
+  double yb[kPF];
+#pragma unroll
+  for (int q = 0; q < kPF; q++) yb[q] = yload(kc(M - 2 - q));
+  const double uS = ax.uS + (double)lz;  // see thomas_forward
+  double c1 = 0.0, c2 = cn;
+  // steady rows M-2 .. P+1
+  for (int k = M - 2; k >= P + 1; k -= kPF) {
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+      const int kk = k - q;  // y_kk sits in slot (M-2-kk) % kPF == q
+      if (kk >= P + 1) {
+        double ck = yb[q] - uS * cn;
+        out(kk, ck);
+        c2 = cn;
+        cn = ck;
+        c1 = ck;
+      }
+      yb[q] = yload(kc(kk - kPF));
+    }
+  }
+  // head rows P .. 1
+#pragma unroll
+  for (int q = kHead - 1; q >= 0; q--) {
+    if (q < P) {
+      const int kk = 1 + q;
+      double ck = yload(kk) - hu[q] * cn;
+      out(kk, ck);
+      c2 = cn;
+      cn = ck;
+      c1 = ck;
+    }
+  }
+  out(0, (ax.A03 - ax.A01 * c1) - ax.A02 * c2);
+}
+
+// Copy n consecutive doubles between global memory and an LDS tile in
+// batches of kB per lane: all loads of a batch are issued before the first
+// use, so their latencies overlap.  Element e of the stream maps to
+// (row, k) = (e / len, e % len), tracked incrementally (no division);
+// src(e, row, k) and dst(row, k, v) address the two sides.
+template <int kB, class SRC, class DST>
+__device__ __forceinline__ void stream_copy(int n, int len, SRC src, DST dst) {
+  const int t = threadIdx.x;
+  int row = 0, k = t;
+  while (k >= len) {
+    k -= len;
+    row++;
+  }
+  for (int base = t; base < n; base += 64 * kB) {
+    double v[kB];
+    int rr[kB], kk[kB];
+#pragma unroll
+    for (int q = 0; q < kB; q++) {
+      const int e = base + 64 * q;
+      rr[q] = row;
+      kk[q] = k;
+      v[q] = e < n ? src(e, row, k) : 0.0;
+      k += 64;
+      while (k >= len) {
+        k -= len;
+        row++;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kB; q++)
+      if (base + 64 * q < n) dst(rr[q], kk[q], v[q]);
+  }
+}
+
+// r-direction solves (create_2D first loop, src/localSpline.hxx:1248-1250).
+// A block (one wave) stages TR consecutive rows of one slice in LDS with
+// coalesced loads, lane t solves row t in place (position k holds rhs_k =
+// f[k-1], then y_k, then c_k), and the padded rows are stored back coalesced.
+// LDS layout [k][row] with an odd row stride TP >= TR: the lanes of the solve
+// touch consecutive words and the staging copies stride by an odd count, so
+// neither conflicts on the banks.
+__global__ void __launch_bounds__(64)
 k_spline_r(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
-           int64_t cs, ThomasAxis ax) {
-  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= (int64_t)nJ * nU) return;
-  int64_t j = row / nU, u = row % nU;
-  const double *src = f + row * nR;  // rhs(k) = f[k-1]
-  double *dst = c + j * cs + (u + 1) * (nR + 2);
-  thomas_line(src, 1, dst, 1, ax);
+           int64_t cs, int TR, int TP, int lzmask, ThomasAxis ax) {
+  extern __shared__ double tile[];  // [M][TP]
+  const int j = blockIdx.y;
+  const int u0 = blockIdx.x * TR;
+  const int rows = min(TR, nU - u0);
+  const int t = threadIdx.x;
+  const int lz = t & lzmask;  // a zero the compiler treats as per-lane
+  const int M = ax.M;  // nR + 2
+  const double *src = f + ((size_t)j * nU + u0) * nR;
+  stream_copy<16>(
+      rows * nR, nR, [&](int e, int, int) { return src[e]; },
+      [&](int row, int k, double v) { tile[(k + 1) * TP + row] = v; });
+  __syncthreads();
+  if (t < rows) {
+    double *L = tile + t;
+    double ym2, ym1;
+    thomas_forward<12>(
+        ax, lz, [&](int k) { return L[k * TP]; }, [&](int k, double y) { L[k * TP] = y; }, ym2,
+        ym1);
+    thomas_backward<12>(
+        ax, lz, ym2, ym1, [&](int k) { return L[k * TP]; },
+        [&](int k, double v) { L[k * TP] = v; });
+  }
+  __syncthreads();
+  double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  stream_copy<16>(
+      rows * M, M, [&](int, int row, int k) { return tile[k * TP + row]; },
+      [&](int row, int k, double v) { dst[row * M + k] = v; });
 }
 
 // v-direction solves of all nR+2 padded columns (create_2D second loop,
-// src/localSpline.hxx:1252-1254); consecutive lanes take consecutive columns.
-__global__ void __launch_bounds__(kBlock)
-k_spline_v(double *__restrict__ c, int nR, int nU, int nJ, int64_t cs, ThomasAxis ax) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (col >= nR + 2) return;
+// src/localSpline.hxx:1252-1254).  Lane = column, so every global access is
+// coalesced; the forward values y_k wait in LDS ([k][lane]) for the back
+// substitution instead of taking a second trip through memory.  A block
+// covers TC columns (TC <= 64, chosen from the LDS budget).
+__global__ void __launch_bounds__(64)
+k_spline_v(double *__restrict__ c, int nR, int nU, int nJ, int64_t cs, int TC, int lzmask,
+           ThomasAxis ax) {
+  extern __shared__ double ybuf[];  // [M][TC]
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;  // a zero the compiler treats as per-lane
+  if (lane >= TC) return;  // no barriers below
+  const int col = blockIdx.x * TC + lane;
+  const int j = blockIdx.y;
   const int64_t st = nR + 2;
-  double *tab = c + (int64_t)j * cs + col;
-  thomas_line(tab + st, st, tab, st, ax);
+  const bool live = col < nR + 2;
+  // dead lanes read a valid column and store nothing
+  double *tab = c + (size_t)j * cs + (live ? col : nR + 1);
+  double ym2, ym1;
+  thomas_forward<24>(
+      ax, lz, [&](int k) { return tab[(int64_t)k * st]; },
+      [&](int k, double y) { ybuf[k * TC + lane] = y; }, ym2, ym1);
+  thomas_backward<12>(
+      ax, lz, ym2, ym1, [&](int k) { return ybuf[k * TC + lane]; },
+      [&](int k, double v) {
+        if (live) tab[(int64_t)k * st] = v;
+      });
+}
+
+// Whole-line solve in registers, for a line length M known at compile time.
+// Lane = line: all M-2 right-hand sides are loaded up front, the forward
+// elimination and the back substitution run on the register array y[] with
+// no memory traffic and no branches, and the M coefficients are stored as
+// they are produced.  Every line of the grid is in flight at once (one wave
+// per SIMD at ~2M VGPRs), which is what a latency-bound recurrence needs.
+// Rows 1..kHead use the pivot table (whose entries from K0 <= kHead+1 on
+// equal the fixed point), later rows the fixed point dS, rdS, uS.
+//   ROWS = true : the nU rows (j, u) of every slice; rhs_k = f[j][u][k-1],
+//                 c_k -> c[j][u+1][k]      (create_2D first loop)
+//   ROWS = false: the nR+2 = S padded columns (j, col); rhs_k = c[j][k][col],
+//                 c_k -> c[j][k][col]      (create_2D second loop)
+template <int M, bool ROWS, int S>
+__global__ void __launch_bounds__(64)
+k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, int nJ, int64_t cs,
+       int lzmask, ThomasAxis ax) {
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const double *in;
+  double *out;
+  int64_t is, os;
+  bool live;
+  if (ROWS) {
+    const int nLines = nJ * nU;
+    int L = blockIdx.x * 64 + lane;
+    live = L < nLines;
+    if (!live) L = nLines - 1;
+    const int j = L / nU, u = L - j * nU;
+    in = src + (size_t)L * nR - 1;  // in[k] = f[j][u][k-1]
+    out = c + (size_t)j * cs + (size_t)(u + 1) * M;
+    is = 1;
+    os = 1;
+  } else {
+    int col = blockIdx.x * 64 + lane;
+    live = col < nR + 2;
+    if (!live) col = nR + 1;
+    in = c + (size_t)blockIdx.y * cs + col;
+    out = const_cast<double *>(in);
+    is = S;  // = nR + 2, a compile-time stride
+    os = S;
+  }
+  double y[M];
+#pragma unroll
+  for (int k = 1; k <= M - 2; k++) y[k] = in[k * is];
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double prev = ax.A03;
+#pragma unroll
+  for (int k = 1; k <= M - 2; k++) {
+    double dv = dS, ev = rdS;
+    if (k <= kHead) {
+      dv = ax.d[k + lz];
+      ev = ax.rd[k + lz];
+    }
+    y[k] = div_known(y[k] - c16 * prev, dv, ev);
+    prev = y[k];
+  }
+  double tt = 0.0 - ax.hi2 * y[M - 3];
+  tt = tt - ax.p * y[M - 2];
+  double cn = div_known(tt, ax.q, ax.rq);
+  if (live) out[(M - 1) * os] = cn;
+  const double uS = ax.uS + (double)lz;
+#pragma unroll
+  for (int k = M - 2; k >= 1; k--) {
+    double uv = uS;
+    if (k <= kHead) uv = ax.u[k + lz];
+    y[k] = y[k] - uv * cn;
+    cn = y[k];
+    if (live) out[k * os] = cn;
+  }
+  if (live) out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
 }
 
 // ---------------------------------------------------------------- kick
@@ -149,13 +416,12 @@ __global__ void __launch_bounds__(kBlock)
 k_kick(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ U_Val,
        const double *__restrict__ acc, int nR, int nU, int nJ, int64_t cs, double Ulo, double Uhi,
        double start_v, double hinv_v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t nVal = (int64_t)nJ * nU * nR;
-  if (i >= nVal) return;
-  int r = (int)(i % nR);
-  int64_t ju = i / nR;
-  int u = (int)(ju % nU);
-  int64_t j = ju / nU;
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;  // index inside the slice
+  if (il >= nU * nR) return;
+  const int j = blockIdx.y;
+  const int u = il / nR;
+  const int r = il - u * nR;
+  const size_t i = (size_t)j * nU * nR + il;
   double newU = U_Val[u] + acc[r];
   if ((newU <= Ulo) || (newU >= Uhi)) {
     f[i] = 0;
@@ -165,9 +431,9 @@ k_kick(double *__restrict__ f, const double *__restrict__ c, const double *__res
   int n;
   split(y, n, y);
   const int Ny = 1 + n, Nx = r + 1;
-  const int64_t st = nR + 2;
-  const double *d = c + j * cs;
-  const double *r0 = d + Nx + (int64_t)(Ny - 1) * st;
+  const int st = nR + 2;
+  const double *d = c + (size_t)j * cs;
+  const double *r0 = d + Nx + (Ny - 1) * st;
   const double *r1 = r0 + st;
   const double *r2 = r1 + st;
   const double *r3 = r2 + st;
@@ -207,17 +473,18 @@ __device__ __forceinline__ double eval_c(const double *d, int64_t st, double xr,
 __global__ void __launch_bounds__(kBlock)
 k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ xr,
         const double *__restrict__ xu, const uint8_t *__restrict__ flag, int nR, int nU,
-        int64_t nVal, int64_t cs) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nVal) return;
+        int64_t cs) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= nU * nR) return;
+  const int j = blockIdx.y;
+  const size_t i = (size_t)j * nU * nR + il;
   uint8_t fl = flag[i];
   if (fl == VLS_FOOT_CHAIN) return;
   if (fl == VLS_FOOT_OUT) {
     f[i] = 0;
     return;
   }
-  int64_t j = i / ((int64_t)nU * nR);
-  f[i] = eval_c(c + j * cs, nR + 2, xr[i], xu[i]);
+  f[i] = eval_c(c + (size_t)j * cs, nR + 2, xr[i], xu[i]);
 }
 
 // delayed central kernel: the back cell takes the spline value at its
@@ -233,13 +500,13 @@ k_chains(double *__restrict__ f, const double *__restrict__ c, const int64_t *__
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nChains) return;
   int64_t pt = point[k];
-  int64_t j = pt / ((int64_t)nU * nR);
+  int j = (int)(pt / ((int64_t)nU * nR));
   int L = len[k];
   int h = head[k];
   double *v = vals + off[k];
   int back = h + L - 1;
   if (back >= L) back -= L;
-  v[back] = bout[k] ? 0.0 : eval_c(c + j * cs, nR + 2, bxr[k], bxu[k]);
+  v[back] = bout[k] ? 0.0 : eval_c(c + (size_t)j * cs, nR + 2, bxr[k], bxu[k]);
   f[pt] = v[h];
   head[k] = (h + 1 == L) ? 0 : h + 1;
 }
@@ -350,27 +617,192 @@ void launch_moment(Run &r, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
                      r.nU, r.nJ);
-  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kBlock), 0, s, r.P, r.gfac, r.dM, r.Mx, r.acc,
-                     r.mTot, r.nR, r.nJ);
+  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(1024), r.nR * sizeof(double), s, r.P, r.gfac,
+                     r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
+}
+
+void configure_tiles(Run &r, int maxLdsPerBlock) {
+  // r-sweep: TR rows of LS doubles per block; v-sweep: (nU+2) x TC doubles.
+  // Prefer tiles of about 52 KB so that three blocks share a 160 KB CU.
+  // three blocks per 160 KB CU
+  const int budget = 163840 / 3 < maxLdsPerBlock ? 163840 / 3 : maxLdsPerBlock;
+  // r-sweep: odd row stride TP (LS) of the [k][row] tile, TR = min(64, TP)
+  int tp = budget / (int)((r.nR + 2) * sizeof(double));
+  if (tp > 65) tp = 65;
+  if (tp % 2 == 0) tp--;
+  if (tp < 1) tp = 1;
+  r.LS = tp;
+  r.TR = tp > 64 ? 64 : tp;
+  r.TC = 64;
+  while (r.TC > 1 && (size_t)r.TC * (r.nU + 2) * sizeof(double) > (size_t)budget) r.TC /= 2;
+  // tile overrides (used by the autotuner): VLS_TILE_R = odd row stride of the
+  // r-sweep tile, VLS_TILE_V = columns per v-sweep block
+  if (const char *e = getenv("VLS_TILE_R")) {
+    int v = atoi(e) | 1;
+    if (v >= 1 && v <= 65) {
+      r.LS = v;
+      r.TR = v > 64 ? 64 : v;
+    }
+  }
+  if (const char *e = getenv("VLS_TILE_V")) {
+    int v = atoi(e);
+    if (v >= 1 && v <= 64) r.TC = v;
+  }
+  const size_t ldsR = (size_t)r.LS * (r.nR + 2) * sizeof(double);
+  const size_t ldsV = (size_t)r.TC * (r.nU + 2) * sizeof(double);
+  if (ldsR > (size_t)maxLdsPerBlock || ldsV > (size_t)maxLdsPerBlock)
+    throw Error(VLS_ERR_ARG, "grid too large for the LDS of one workgroup");
+  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_r,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsR));
+  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_v,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsV));
+}
+
+// Row version of the register line solver with coalesced memory access.
+// A block of 64 lanes takes 64 consecutive lines (rows of f, contiguous in
+// memory).  Right-hand sides arrive in chunks of kC elements per row through
+// a small LDS transpose (loads of whole row segments, then lane = row reads
+// its kC values); the loads of chunk c+1 are issued before the elimination
+// of chunk c so that their latency hides behind it.  Coefficients leave the
+// same way, a chunk at a time as the back substitution produces them.
+template <int M>
+__global__ void __launch_bounds__(64)
+k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
+            int64_t cs, int lzmask, ThomasAxis ax) {
+  constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
+  constexpr int nIn = M - 2;           // right-hand sides per line (= nR)
+  constexpr int nChIn = (nIn + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
+  __shared__ double chunk[64 * kP];
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int nLines = nJ * nU;
+  const int L0 = blockIdx.x * 64;
+  const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC elements per pass
+  double y[M];
+  double tmp[kC];
+  auto load_chunk = [&](int ch) {
+#pragma unroll
+    for (int q = 0; q < kC; q++) {
+      const int row = q * 4 + rsub, e = ch * kC + ksub, L = L0 + row;
+      tmp[q] = (L < nLines && e < nIn) ? f[(size_t)L * nIn + e] : 0.0;
+    }
+  };
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double prev = ax.A03;
+  load_chunk(0);
+#pragma unroll
+  for (int ch = 0; ch < nChIn; ch++) {
+#pragma unroll
+    for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[q];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < nIn) y[1 + ch * kC + i] = chunk[lane * kP + i];
+    __syncthreads();
+    if (ch + 1 < nChIn) load_chunk(ch + 1);
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+      const int k = 1 + ch * kC + i;
+      if (k <= M - 2) {
+        double dv = dS, ev = rdS;
+        if (k <= kHead) {
+          dv = ax.d[k + lz];
+          ev = ax.rd[k + lz];
+        }
+        y[k] = div_known(y[k] - c16 * prev, dv, ev);
+        prev = y[k];
+      }
+    }
+  }
+  double tt = 0.0 - ax.hi2 * y[M - 3];
+  tt = tt - ax.p * y[M - 2];
+  y[M - 1] = div_known(tt, ax.q, ax.rq);
+  const double uS = ax.uS + (double)lz;
+  double cn = y[M - 1];
+  // back substitution and stores, last chunk first
+#pragma unroll
+  for (int ch = nChOut - 1; ch >= 0; ch--) {
+#pragma unroll
+    for (int i = kC - 1; i >= 0; i--) {
+      const int k = ch * kC + i;
+      if (k >= 1 && k <= M - 2) {
+        double uv = uS;
+        if (k <= kHead) uv = ax.u[k + lz];
+        y[k] = y[k] - uv * cn;
+        cn = y[k];
+      }
+    }
+    if (ch == 0) y[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < M) chunk[lane * kP + i] = y[ch * kC + i];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kC; q++) {
+      const int row = q * 4 + rsub, k = ch * kC + ksub, L = L0 + row;
+      if (L < nLines && k < M) {
+        const int j = L / nU, u = L - j * nU;
+        c[(size_t)j * cs + (size_t)(u + 1) * M + k] = chunk[row * kP + ksub];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
+// standard grids.  Other shapes use the LDS-tiled kernels above.
+using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
+struct LineShape {
+  int nR, nU;
+  LineFn rows, cols;
+};
+#define VLS_LINE_SHAPE(R, U) \
+  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>}
+static const LineShape kLineShapes[] = {
+    VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
+    VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
+    VLS_LINE_SHAPE(201, 201),
+};
+#undef VLS_LINE_SHAPE
+
+static const LineShape *line_shape(const Run &r) {
+  if (getenv("VLS_NO_REGISTER_LINES")) return nullptr;
+  for (const LineShape &ls : kLineShapes)
+    if (ls.nR == r.nR && ls.nU == r.nU) return &ls;
+  return nullptr;
 }
 
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
-  hipLaunchKernelGGL(k_spline_r, dim3(blocks_for((int64_t)r.nJ * r.nU)), dim3(kBlock), 0, s, src,
-                     r.c, r.nR, r.nU, r.nJ, r.cs, r.tr);
+  if (const LineShape *ls = line_shape(r)) {
+    const int lines = r.nJ * r.nU;
+    hipLaunchKernelGGL(ls->rows, dim3((lines + 63) / 64), dim3(64), 0, s, src, r.c, r.nR, r.nU,
+                       r.nJ, r.cs, 0, r.tr);
+    time_mark(t, s, "spline_r");
+    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
+                       r.nU, r.nJ, r.cs, 0, r.tv);
+    time_mark(t, s, "spline_v");
+    return;
+  }
+  dim3 gr((r.nU + r.TR - 1) / r.TR, r.nJ);
+  hipLaunchKernelGGL(k_spline_r, gr, dim3(64), (size_t)r.LS * (r.nR + 2) * sizeof(double), s, src,
+                     r.c, r.nR, r.nU, r.nJ, r.cs, r.TR, r.LS, 0, r.tr);
   time_mark(t, s, "spline_r");
-  hipLaunchKernelGGL(k_spline_v, dim3(blocks_for(r.nR + 2), r.nJ), dim3(kBlock), 0, s, r.c, r.nR,
-                     r.nU, r.nJ, r.cs, r.tv);
+  dim3 gv((r.nR + 2 + r.TC - 1) / r.TC, r.nJ);
+  hipLaunchKernelGGL(k_spline_v, gv, dim3(64), (size_t)r.TC * (r.nU + 2) * sizeof(double), s,
+                     r.c, r.nR, r.nU, r.nJ, r.cs, r.TC, 0, r.tv);
   time_mark(t, s, "spline_v");
 }
 
 void launch_kick(Run &r, hipStream_t s) {
-  hipLaunchKernelGGL(k_kick, dim3(blocks_for(r.nVal)), dim3(kBlock), 0, s, r.f, r.c, r.U_Val,
-                     r.acc, r.nR, r.nU, r.nJ, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
+  hipLaunchKernelGGL(k_kick, g, dim3(kBlock), 0, s, r.f, r.c, r.U_Val, r.acc, r.nR, r.nU, r.nJ,
+                     r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
 void launch_drift(Run &r, hipStream_t s) {
-  hipLaunchKernelGGL(k_drift, dim3(blocks_for(r.nVal)), dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu,
-                     r.flag, r.nR, r.nU, r.nVal, r.cs);
+  dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
+  hipLaunchKernelGGL(k_drift, g, dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.flag, r.nR, r.nU,
+                     r.cs);
 }
 
 void launch_chains(Run &r, hipStream_t s) {
