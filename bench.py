@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the spherical Vlasov-Poisson spline step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the UDF uniform-density sphere (M=1, R0=2, alpha=0.5, smoothing
+0.5, G=1) on its default grid, 201 log-r nodes (0.01..25) x 201 linear v_r
+nodes (-2..2) x 200 quadratic j cells (0..1.6), dt=0.005, FP64: 8,080,200
+phase-space points.  A step is one full makeOneStep (moment, radial scan,
+kick spline + evaluation, drift spline + evaluation, delay lines), with f
+resident in HBM.  value = points x K / wall seconds of K steps.
+
+Prints one JSON line with, besides the throughput, a roofline object for the
+dominant kernel (algorithmic bytes per launch over its average duration,
+measured with HIP events on the solver's stream during the timed steps) and a
+cpu_baseline object: the reference executable, built from its own sources
+into oracle/_ref, timed on this host's cores on a bounded sample of the same
+workload (or the oracle, as a port, if the reference was not built).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+WORKLOAD = {
+    "init.type": "UDF", "init.M": 1, "init.R0": 2, "init.alpha": 0.5,
+    "init.smoothingRadius": 0.5, "solver.G": 1,
+    "grid.Pmin": 0.01, "grid.Pmax": 25, "grid.Pres": 200, "grid.Pscale": "logarithmic",
+    "grid.Umin": [-2, 0], "grid.Umax": [2, 1.6], "grid.Ures": [200, 200],
+    "grid.Uscale": ["linear", "quadratic"], "solver.dt": 0.005,
+}
+
+
+def algorithmic_bytes(name, nR, nU, nJ):
+    """Compulsory HBM bytes of one launch of a kernel of the step: every
+    operand array read once and written once; static foot and delay-line
+    tables excluded."""
+    n = nR * nU * nJ
+    coef = (nR + 2) * (nU + 2) * nJ
+    rows = (nR + 2) * nU * nJ
+    table = {
+        "moment": 8 * n,                 # read f
+        "spline_r": 8 * n + 8 * rows,    # read f, write the r-solved rows
+        "spline_v": 8 * rows + 8 * coef,  # read them, write all coefficients
+        "kick": 8 * coef + 8 * n,        # read coefficients, write f
+        "drift": 8 * coef + 8 * n,       # read coefficients, write f
+    }
+    return table.get(name)
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(steps: int):
+    """Reference executable (or the oracle) on a bounded sample: the same
+    grid and initial conditions, `steps` steps, stats/snapshots/restarts off."""
+    from oracle import oracle as O
+
+    threads = cpu_threads()
+    npts = 201 * 201 * 200
+    cfg = dict(WORKLOAD)
+    cfg.update({"solver.Tmax": 0.005 * steps, "solver.statsEvery": -1,
+                "solver.snapshotEvery": -1, "solver.restartEvery": -1})
+    if O.REF_EXE.exists():
+        with tempfile.TemporaryDirectory() as wd:
+            out = O.run_reference(cfg, wd, threads=threads, timeout=900)
+        m = re.search(r"Computed (\d+) timesteps in ([0-9.eE+-]+)s", out)
+        if not m:
+            raise RuntimeError("cannot parse the reference's timing line")
+        nsteps, secs = int(m.group(1)), float(m.group(2))
+        return {"value": npts * nsteps / secs, "unit": "point-updates/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"reference vlasolve built from its sources (clang -O3, OpenMP), "
+                          f"same 201x201x200 UDF grid, {nsteps} steps in {secs:.2f} s "
+                          f"(its own step-loop timer), OMP_NUM_THREADS={threads}"}
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    orc = O.Oracle(WORKLOAD, nthreads=threads)
+    orc.step(1)
+    t0 = time.perf_counter()
+    orc.step(steps)
+    secs = time.perf_counter() - t0
+    return {"value": npts * steps / secs, "unit": "point-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CPU oracle (C restatement, OpenMP), same 201x201x200 UDF grid, "
+                      f"{steps} steps in {secs:.2f} s, {threads} threads"}
+
+
+def traffic_for(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel), from profiles/traffic.json."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        ent = d.get("kernels", {}).get(kernel)
+        if ent and d.get("workload") == "udf_201x201x200":
+            return float(ent["hbm_bytes_per_launch"])
+    except (ValueError, KeyError, TypeError):
+        return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the CPU baseline sample (default 150, ~10-20 s of CPU work)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import vlasolve_amd as V
+
+    solver = V.Solver(WORKLOAD)
+    ctx = solver.context
+    npts = solver.nR * solver.nU * solver.nJ
+    solver.step(args.warmup)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    solver.step(args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    times = ctx.kernel_times(cap=16 * (args.steps + 1))
+    ctx.set_timing(False)
+    value = npts * args.steps / wall
+    if args.calibrate_copy:
+        V.copy_probe(args.calibrate_copy << 20)
+
+    per = {}
+    for name, ms in times:
+        tot, cnt = per.get(name, (0.0, 0))
+        per[name] = (tot + ms, cnt + 1)
+    dominant = max((k for k in per if algorithmic_bytes(k, 1, 1, 1)), key=lambda k: per[k][0])
+    tot, cnt = per[dominant]
+    avg_s = tot / cnt / 1e3
+    alg = algorithmic_bytes(dominant, solver.nR, solver.nU, solver.nJ)
+    achieved = alg / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_for(dominant), "kernel": dominant,
+                "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": alg,
+                "kernels_us_per_step": {k: round(v[0] / args.steps * 1e3, 2)
+                                        for k, v in sorted(per.items())}}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_steps or 150)
+        except Exception as e:  # reported, not fatal for the GPU figure
+            cpu = {"value": None, "error": str(e)[:200]}
+
+    line = {
+        "metric": "phase-space cell-updates/s at 200^3 FP64 (vs CPU); adjoint/forward cost ratio",
+        "value": value, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "UDF sphere, 201 log-r x 201 v x 200 quadratic-j (Pres 200, "
+                               "Ures 200 200), dt 0.005, FP64, delayed central kernel",
+                   "points": npts, "global_batch": 1, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -144,6 +144,8 @@ int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n);
 int vls_thomas_table(int dim, double hinv, double *u, double *d, double *rd, double consts[8]);
 /* One-line description of the current device (CUs, clock, LDS, L2). */
 int vls_device_info(char *buf, int len);
+/* One device-to-device copy of `bytes` bytes, to calibrate memory counters. */
+int vls_copy_probe(size_t bytes);
 int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
                        int64_t *mismatches);
 
@@ -168,6 +170,21 @@ int vlsd_write_restart(vlsd_solver *s, const char *fname_no_ext);   /* .vls */
 int vlsd_snapshot(vlsd_solver *s);                  /* snapshot_%4.4f.ND */
 vls_ctx *vlsd_context(vlsd_solver *s);
 const char *vlsd_last_error(void);
+
+/* Host set-up only (no device): the same initial conditions, renormalisation,
+ * static feet and delay lines as vlsd_create, for inspection and for
+ * machines without a GPU.  dims = {nR, nU, nJ}; feet are the free-streaming
+ * foot positions and VLS_FOOT_* flags of every point; chains as in
+ * vls_set_chains, cells front first with their positions and values. */
+typedef struct vlsd_host vlsd_host;
+int vlsd_host_create(int argc, const char *const *argv, vlsd_host **out);
+void vlsd_host_destroy(vlsd_host *h);
+int vlsd_host_info(vlsd_host *h, int64_t dims[3], int64_t *nChains, int64_t *nCells,
+                   double *kernelRadius);
+int vlsd_host_f(vlsd_host *h, double *f);
+int vlsd_host_feet(vlsd_host *h, double *driftR, double *driftU, uint8_t *flag);
+int vlsd_host_chains(vlsd_host *h, int64_t *point, int32_t *len, double *cellR, double *cellU,
+                     double *val);
 
 #ifdef __cplusplus
 }

@@ -971,18 +971,24 @@ void orc_moments(orc_state *s, double *dM, double *Mexcl, double *mTot)
   free(m);
 }
 
-/* vlasovSolver21::statistics + potentialEnergy (vlasovSolver21.hxx:627-689) */
+/* vlasovSolver21::statistics + potentialEnergy (vlasovSolver21.hxx:627-689).
+ * potentialEnergy gathers integrate1D with addSpare=1 (quadrature21.hxx:36,
+ * grid_handler.hxx:350-372): its array is {0, dM[0], ..., dM[nR-1]}, so shell
+ * k (between R[k] and R[k+1]) meets the mass below it, M(<R[k]), at its own
+ * midpoint.  The last iteration (index nR) reads one past R but multiplies
+ * dM[nR-1] = 0 (the last node has no width), so it adds nothing. */
 void orc_stats(orc_state *s, double out[7])
 {
-  double *Mr = malloc(s->nR * sizeof(double));
-  integrate1D(s, s->f, Mr);
-  double M = Mr[0] + 0.0, Ep = 0;
+  double *dM = malloc(s->nR * sizeof(double));
+  integrate1D(s, s->f, dM);
+  double M = 0.0 + 0.0, Ep = 0; /* Mr[0] + kernelMass */
   for (long i = 1; i < s->nR; i++) {
     double r = 0.5 * (s->R_Val[i - 1] + s->R_Val[i]);
-    Ep -= (Mr[i] * s->p.G * M) / r;
-    M += Mr[i];
+    Ep -= (dM[i - 1] * s->p.G * M) / r;
+    M += dM[i - 1];
   }
-  free(Mr);
+  M += dM[s->nR - 1];
+  free(dM);
   double Ek = integrate(s, s->f, IG_EK);
   double E = Ek + Ep;
   double S = integrate(s, s->f, IG_S);

@@ -48,3 +48,7 @@ SMALL_CASES = {
 # LDS-tiled spline kernels
 UDF_ODD = dict(UDF_C1, **{"grid.Pres": 40, "grid.Ures": [36, 10]})
 SMALL_CASES["udf_odd_shape"] = (UDF_ODD, 30)
+
+# Osipkov-Merritt anisotropic Hernquist sphere (the Ra > 0 branch of the DF)
+HERNQUIST_OM = dict(HERNQUIST_SMALL, **{"init.Ra": 1.5, "grid.Pres": 32, "grid.Ures": [32, 16]})
+SMALL_CASES["hernquist_om"] = (HERNQUIST_OM, 40)

@@ -26,7 +26,9 @@ EXPORTS = (
     "vls_moments", "vls_stats", "vls_download_coef", "vls_set_timing", "vls_kernel_times",
     "vlsd_create", "vlsd_destroy", "vlsd_info", "vlsd_step", "vlsd_get_f", "vlsd_stats",
     "vlsd_moments", "vlsd_run", "vlsd_write_restart", "vlsd_snapshot", "vlsd_context",
-    "vlsd_last_error",
+    "vlsd_last_error", "vls_thomas_table", "vls_check_division", "vls_device_info",
+    "vls_copy_probe", "vlsd_host_create", "vlsd_host_destroy", "vlsd_host_info", "vlsd_host_f",
+    "vlsd_host_feet", "vlsd_host_chains",
 )
 
 
@@ -116,6 +118,18 @@ def to_argv(cfg) -> list[str]:
         vals = list(v) if isinstance(v, (list, tuple)) else [v]
         out += [repr(x) if isinstance(x, float) else str(x) for x in vals]
     return out
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(512)
+    _check(lib().vls_device_info(buf, 512))
+    return buf.value.decode()
+
+
+def copy_probe(nbytes: int) -> None:
+    """One device-to-device copy of nbytes (counter calibration)."""
+    lib().vls_copy_probe.argtypes = [C.c_size_t]
+    _check(lib().vls_copy_probe(int(nbytes)))
 
 
 class Solver:

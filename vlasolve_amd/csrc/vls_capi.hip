@@ -161,6 +161,22 @@ int vls_device_info(char *buf, int len) {
   });
 }
 
+// One device-to-device copy of `bytes` bytes (the runtime's copy kernel), to
+// calibrate memory counters against a transfer of known size.
+int vls_copy_probe(size_t bytes) {
+  return guarded([&] {
+    void *a = nullptr, *b = nullptr;
+    VLS_HIP_CHECK(hipMalloc(&a, bytes));
+    VLS_HIP_CHECK(hipMalloc(&b, bytes));
+    VLS_HIP_CHECK(hipMemset(a, 1, bytes));
+    VLS_HIP_CHECK(hipDeviceSynchronize());
+    VLS_HIP_CHECK(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice));
+    VLS_HIP_CHECK(hipDeviceSynchronize());
+    (void)hipFree(a);
+    (void)hipFree(b);
+  });
+}
+
 // The data-independent Thomas table of one axis (dim nodes, knot spacing
 // 1/hinv): u, d, rd have dim+2 entries; consts = {c16, A01, A02, A03, hi2, p,
 // q, rq}.

@@ -558,7 +558,8 @@ k_stats_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
 
 // Serial sums over j (moment) and over r (integrands), then over j; the
 // potential energy of vlasovSolver21::potentialEnergy
-// (src/vlasovSolver21.hxx:627-642).  out = {M, E, Ek, Ep, S, L1, L2}.
+// (src/vlasovSolver21.hxx:627-642, with the gathered array shifted by one,
+// src/quadrature21.hxx:36).  out = {M, E, Ek, Ep, S, L1, L2}.
 __global__ void __launch_bounds__(kBlock)
 k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val, double G,
               double *__restrict__ dM, double *__restrict__ q, double *__restrict__ out, int nR,
@@ -578,12 +579,15 @@ k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double M = dM[0] + 0.0, Ep = 0.0;
+    // shell k (R[k]..R[k+1]) meets M(<R[k]) at its midpoint: the gathered
+    // array of potentialEnergy is shifted by one (addSpare=1)
+    double M = 0.0 + 0.0, Ep = 0.0;
     for (int i = 1; i < nR; i++) {
       double rr = 0.5 * (R_Val[i - 1] + R_Val[i]);
-      Ep -= (dM[i] * G * M) / rr;
-      M += dM[i];
+      Ep -= (dM[i - 1] * G * M) / rr;
+      M += dM[i - 1];
     }
+    M += dM[nR - 1];
     double tot[4];
     for (int k = 0; k < 4; k++) {
       double s = 0.0;
