@@ -55,6 +55,7 @@ def algorithmic_bytes(name, nR, nU, nJ):
         "spline_r": 8 * n + 8 * rows,    # read f, write the r-solved rows
         "spline_v": 8 * rows + 8 * coef,  # read them, write all coefficients
         "kick": 8 * coef + 8 * n,        # read coefficients, write f
+        "kick_line": 8 * rows + 8 * n,   # fused v-solve + kick: read the r-solved rows, write f
         "drift": 8 * coef + 8 * n,       # read coefficients, write f
     }
     return table.get(name)

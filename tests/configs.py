@@ -52,3 +52,8 @@ SMALL_CASES["udf_odd_shape"] = (UDF_ODD, 30)
 # Osipkov-Merritt anisotropic Hernquist sphere (the Ra > 0 branch of the DF)
 HERNQUIST_OM = dict(HERNQUIST_SMALL, **{"init.Ra": 1.5, "grid.Pres": 32, "grid.Ures": [32, 16]})
 SMALL_CASES["hernquist_om"] = (HERNQUIST_OM, 40)
+
+# Strong kick: accelerations that move f by tens of v cells in one step, so
+# that the target rows of the kick spread widely across neighbouring r nodes
+# (physically extreme, numerically valid).
+UDF_STRONG_KICK = dict(UDF_C1, **{"solver.G": 1000, "solver.dt": 0.02})

@@ -81,3 +81,44 @@ def test_c2_shape_bit_exact():
     dMg, ag, mg = gpu.moments()
     dMo, ao, mo = orc.last_moments()
     assert same(dMg, dMo) and same(ag, ao) and mg == mo
+
+
+def _bit_exact_run(cfg, nsteps, label):
+    gpu = V.Solver(cfg)
+    orc = O.Oracle(cfg, nthreads=0)
+    for k in (1, nsteps - 1):
+        gpu.step(k)
+        orc.step(k)
+        fg, fo = gpu.f(), orc.f()
+        bad = np.count_nonzero(fg != fo)
+        assert bad == 0, f"{label}: {bad} points differ"
+    dMg, ag, mg = gpu.moments()
+    dMo, ao, mo = orc.last_moments()
+    assert same(dMg, dMo) and same(ag, ao) and mg == mo
+
+
+def test_kick_scratch_path_bit_exact(monkeypatch):
+    """The fused kick's global-scratch path (taken by waves whose kick targets
+    spread wider than its LDS ring), forced for every wave."""
+    from tests.configs import UDF_C1
+
+    print("case udf_c1 with VLS_KICK_SCRATCH", flush=True)
+    monkeypatch.setenv("VLS_KICK_SCRATCH", "1")
+    _bit_exact_run(UDF_C1, 12, "scratch path")
+
+
+def test_strong_kick_bit_exact():
+    """Kicks of tens of v cells per step: ring and scratch paths side by side."""
+    from tests.configs import UDF_STRONG_KICK
+
+    print("case udf_strong_kick", flush=True)
+    _bit_exact_run(UDF_STRONG_KICK, 10, "strong kick")
+
+
+def test_unfused_kick_bit_exact(monkeypatch):
+    """The separate v-solve and kick kernels (shapes without register lines)."""
+    from tests.configs import UDF_C1
+
+    print("case udf_c1 with VLS_NO_FUSED_KICK", flush=True)
+    monkeypatch.setenv("VLS_NO_FUSED_KICK", "1")
+    _bit_exact_run(UDF_C1, 12, "unfused kick")

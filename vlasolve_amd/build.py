@@ -32,7 +32,10 @@ HOST_SOURCES = ["host/params.cpp", "host/grid.cpp", "host/solver.cpp", "host/dri
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable"]
 HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-                      "-packed-fp32-ops", "-munsafe-fp-atomics"]
+                      "-packed-fp32-ops", "-munsafe-fp-atomics",
+                      # the register line solvers are fully unrolled loops of up to ~200
+                      # steps whose bodies exceed the default limit of a forced unroll
+                      "-mllvm", "-pragma-unroll-threshold=1000000"]
 HOST_FLAGS = COMMON + ["-fopenmp"]
 
 

@@ -287,9 +287,10 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       // those reads inside the allocation.
       r.c = dalloc<double>(r, (size_t)r.nJ * r.cs + kCoefPad(r));
       VLS_HIP_CHECK(hipMemsetAsync(r.c, 0, ((size_t)r.nJ * r.cs + kCoefPad(r)) * sizeof(double), s));
+      if (size_t n = vls::kick_scratch_doubles(r)) r.gsc = dalloc<double>(r, n);
+      r.kickScratch = getenv("VLS_KICK_SCRATCH") != nullptr;
       r.xr = dalloc<double>(r, r.nVal);
       r.xu = dalloc<double>(r, r.nVal);
-      r.flag = dalloc<uint8_t>(r, r.nVal);
       r.P = dalloc<double>(r, (size_t)r.nJ * r.nR);
       r.dM = dalloc<double>(r, r.nR);
       r.Mx = dalloc<double>(r, r.nR);
@@ -382,10 +383,13 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
         throw vls::Error(VLS_ERR_ARG, "vls_set_drift: foot outside the spline domain at point " +
                                           std::to_string(i));
     }
+    // the drift kernel reads the point's kind from the sign of xr
+    std::vector<double> xf(xr, xr + r.nVal);
+    for (int64_t i = 0; i < r.nVal; i++)
+      if (flag[i] != VLS_FOOT_EVAL) xf[i] = flag[i] == VLS_FOOT_OUT ? -1.0 : -2.0;
     hipStream_t s = ctx->c.stream;
-    h2d(r.xr, xr, r.nVal, s);
+    h2d(r.xr, xf.data(), r.nVal, s);
     h2d(r.xu, xu, r.nVal, s);
-    h2d(r.flag, flag, r.nVal, s);
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.have_drift = true;
   });
@@ -473,9 +477,7 @@ int vls_step(vls_ctx *ctx, int nSteps) {
         }
         vls::launch_moment(r, s);
         vls::time_mark(t, s, "moment");
-        vls::launch_spline(r, r.f, s, t);
-        vls::launch_kick(r, s);
-        vls::time_mark(t, s, "kick");
+        vls::launch_spline_kick(r, s, t);
         vls::launch_spline(r, r.f, s, t);
         vls::launch_chains(r, s);
         vls::time_mark(t, s, "chains");

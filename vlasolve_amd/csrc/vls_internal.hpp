@@ -62,9 +62,10 @@ struct Run {
   // state
   double *f = nullptr;      // [nJ][nU][nR]
   double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
-  // static feet
+  double *gsc = nullptr;    // fused kick: per-wave scratch for wide kicks
+  bool kickScratch = false; // fused kick: always take the scratch path
+  // static feet (xr < 0 marks points outside the grid or on delay lines)
   double *xr = nullptr, *xu = nullptr;
-  uint8_t *flag = nullptr;
   // moment
   double *P = nullptr;      // [nJ][nR] per-slice partials
   double *dM = nullptr, *Mx = nullptr, *acc = nullptr, *mTot = nullptr;
@@ -99,6 +100,8 @@ void configure_tiles(Run &r, int maxLdsPerBlock);
 void launch_moment(Run &r, hipStream_t s);
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 void launch_kick(Run &r, hipStream_t s);
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t);  // kick spline + kick
+size_t kick_scratch_doubles(const Run &r);
 void launch_drift(Run &r, hipStream_t s);
 void launch_chains(Run &r, hipStream_t s);
 void launch_stats(Run &r, hipStream_t s);

@@ -84,22 +84,24 @@ k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
 
 // dM[r] = serial sum over j; exclusive radial prefix (gridHandler::accumulate
 // with addSpare=1, src/grid_handler.hxx:350-407); acceleration
-// M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).  One block.
-__global__ void __launch_bounds__(1024)
+// M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).  One block: a
+// thread per r sums its column of partials with kB loads in flight; one
+// thread then runs the serial prefix on registers, kB entries at a time.
+__global__ void __launch_bounds__(256)
 k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, double *__restrict__ dM,
               double *__restrict__ Mx, double *__restrict__ acc, double *__restrict__ mTot, int nR,
               int nJ) {
+  constexpr int kB = 64;  // loads in flight per thread (one wave per SIMD)
   extern __shared__ double sdm[];  // [nR]
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
-    constexpr int kJ = 8;
     double s = 0.0;
     int j = 0;
-    for (; j + kJ <= nJ; j += kJ) {
-      double v[kJ];
+    for (; j + kB <= nJ; j += kB) {
+      double v[kB];
 #pragma unroll
-      for (int k = 0; k < kJ; k++) v[k] = P[(size_t)(j + k) * nR + r];
+      for (int k = 0; k < kB; k++) v[k] = P[(size_t)(j + k) * nR + r];
 #pragma unroll
-      for (int k = 0; k < kJ; k++) s += v[k];
+      for (int k = 0; k < kB; k++) s += v[k];
     }
     for (; j < nJ; j++) s += P[(size_t)j * nR + r];
     dM[r] = s;
@@ -107,9 +109,24 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    constexpr int kS = 16;
     double prev = 0.0;
-    for (int r = 0; r < nR; r++) {
-      double d = sdm[r];
+    int r = 0;
+    for (; r + kS <= nR; r += kS) {
+      double v[kS];
+#pragma unroll
+      for (int k = 0; k < kS; k++) v[k] = sdm[r + k];
+#pragma unroll
+      for (int k = 0; k < kS; k++) {
+        const double d = v[k];
+        v[k] = prev;
+        prev += d;
+      }
+#pragma unroll
+      for (int k = 0; k < kS; k++) sdm[r + k] = v[k];
+    }
+    for (; r < nR; r++) {
+      const double d = sdm[r];
       sdm[r] = prev;
       prev += d;
     }
@@ -409,6 +426,220 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
   if (live) out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
 }
 
+// ---------------------------------------------------------------- fused kick
+// The v-direction solves of the kick's spline (create_2D second loop,
+// src/localSpline.hxx:1252-1254) fused with the kick itself
+// (src/vlasovSolver21_spline.hxx:750-765, evaluateNxPy
+// src/localSpline.hxx:423-441).  The kick evaluates the spline on r nodes
+// only: point (u, r) with target row n needs, for the four padded rows
+// m = n..n+3, the column combination g_m(r) = (c[m][r] + c[m][r+1]*4) +
+// c[m][r+2], in the reference's operation order.
+//
+// A wave takes 64 consecutive padded columns and writes the points of the
+// first kKW = 62 of them.  Lane = column: the column is solved in registers,
+// and the back substitution, which produces c_m from the last row down, forms
+// g_m with two lane shifts and keeps the last kRing rows of g in a ring in
+// LDS (each lane reads only its own column of it).  Point (u, r) is evaluated
+// at the step that produces row m = u + D, D being the smallest offset n - u
+// over the wave's points: its rows are ready by then, and they are still in
+// the ring if the offsets of the wave spread over at most kRing - 4 rows.  A
+// wave whose offsets spread wider keeps all of g in a global scratch area and
+// evaluates after the sweep.  Either way the kick's coefficients never reach
+// HBM.  The offsets come from the same expressions as the evaluation and are
+// taken during the forward elimination.
+constexpr int kKW = 62, kEB = 8;
+
+// Long columns keep their first kYL forward values in LDS instead of VGPRs
+// (they are the last ones the back substitution needs) and use a 32-row ring,
+// so that the column, the ring evaluation and the sweep fit the 512 VGPRs of
+// a wave.
+template <int Mv>
+struct KickLineCfg {
+  static constexpr int kRing = Mv > 160 ? 32 : 64;
+  static constexpr int kYL = Mv > 160 ? 32 : 0;
+};
+
+template <int Mv, int S>
+__global__ void __launch_bounds__(64)
+k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *__restrict__ U_Val,
+            const double *__restrict__ acc, double *__restrict__ gsc, int nR, int64_t cs,
+            double Ulo, double Uhi, double start_v, double hinv_v, int forceScratch, int lzmask,
+            ThomasAxis ax) {
+  constexpr int nU = Mv - 2;
+  constexpr int kRing = KickLineCfg<Mv>::kRing, kYL = KickLineCfg<Mv>::kYL;
+  // ring slot s holds row m with m % kRing == s; slots kRing..kRing+2 repeat
+  // slots 0..2 so that the four rows n..n+3 of a point are read from slots
+  // (n % kRing) + 0..3 without wrapping
+  __shared__ double ring[(kRing + 3) * 64];
+  __shared__ double ylds[(kYL > 0 ? kYL : 1) * 64];
+  __shared__ double Us[nU];
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int j = blockIdx.y;
+  const int k = blockIdx.x * kKW + lane;      // padded column solved by this lane
+  const bool outLane = lane < kKW && k < nR;  // writes the points f[j][.][k]
+  const double *col = c + (size_t)j * cs + (k < S ? k : S - 1);
+  for (int u = lane; u < nU; u += 64) Us[u] = U_Val[u];
+  const double a = outLane ? acc[k] : 0.0;
+  __syncthreads();
+  double *fo = f + (size_t)j * nU * nR + (outLane ? k : 0);
+
+  // forward elimination (rows 1..Mv-2)
+  double y[Mv];
+#pragma unroll
+  for (int m = 1; m <= Mv - 2; m++) y[m] = col[(int64_t)m * S];
+  // target-row offsets n - u of the lane's points (while the loads fly)
+  int dlo = 1 << 30, dhi = -(1 << 30);
+  if (outLane) {
+#pragma unroll 4
+    for (int u = 0; u < nU; u++) {
+      const double nu = Us[u] + a;
+      if (!(nu <= Ulo) && !(nu >= Uhi)) {
+        const int n = (int)trunc((nu - start_v) * hinv_v);
+        dlo = min(dlo, n - u);
+        dhi = max(dhi, n - u);
+      }
+    }
+  }
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double prev = ax.A03;
+#pragma unroll
+  for (int m = 1; m <= Mv - 2; m++) {
+    double dv = dS, ev = rdS;
+    if (m <= kHead) {
+      dv = ax.d[m + lz];
+      ev = ax.rd[m + lz];
+    }
+    const double yk = div_known(y[m] - c16 * prev, dv, ev);
+    if (m <= kYL)
+      ylds[(m - 1) * 64 + lane] = yk;
+    else
+      y[m] = yk;
+    prev = yk;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    dlo = min(dlo, __shfl_xor(dlo, o));
+    dhi = max(dhi, __shfl_xor(dhi, o));
+  }
+  if (dlo > dhi) dlo = dhi = 0;  // no point of the wave evaluates the spline
+  const int D = dlo;
+  const bool useRing = !forceScratch && dhi - dlo <= kRing - 4 - (kEB - 1);
+  double *gs = gsc + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (Mv + 1) * 64 + lane;
+
+  auto ring_at = [&](int row) { return ring[(row & (kRing - 1)) * 64 + lane]; };
+  auto ring_put = [&](int m, double g) {  // m known at compile time
+    ring[(m & (kRing - 1)) * 64 + lane] = g;
+    if ((m & (kRing - 1)) < 3) ring[(kRing + (m & (kRing - 1))) * 64 + lane] = g;
+  };
+  auto scratch_at = [&](int row) { return gs[(size_t)row * 64]; };
+  // point u of this lane's column (u wave-uniform)
+  auto eval = [&](int u, auto gload) {
+    const double newU = Us[u] + a;
+    double val = 0;
+    if (!(newU <= Ulo) && !(newU >= Uhi)) {
+      double yy = (newU - start_v) * hinv_v;
+      int n;
+      split(yy, n, yy);
+      const double g0 = gload(n), g1 = gload(n + 1), g2 = gload(n + 2), g3 = gload(n + 3);
+      double result = 1.0 / 36.0;
+      result *= BV2(1. - yy) * g0 + BV1(1. - yy) * g1 + BV1(yy) * g2 + BV2(yy) * g3;
+      val = result;
+    }
+    fo[(size_t)u * nR] = val;
+  };
+  // g_m from the lane's c_m and its two right neighbours
+  auto gcomb = [&](double cm) {
+    const double c1 = __shfl_down(cm, 1), c2 = __shfl_down(cm, 2);
+    return (cm + c1 * 4) + c2;
+  };
+  // Lanes without a point of their own store into their scratch slot, so that
+  // the stores of a block need no lane branch.
+  double *fdump = gs;
+  // The points whose steps are rows m0..m0+kEB-1, once row m0 is in: straight
+  // code, so that the kEB evaluations overlap each other and the sweep.
+  auto eval_block = [&](int m0) {
+    double val[kEB];
+#pragma unroll
+    for (int i = 0; i < kEB; i++) {
+      const int u = m0 + i - D;
+      const int uc = min(max(u, 0), nU - 1);
+      const double newU = Us[uc] + a;
+      const bool in = !(newU <= Ulo) && !(newU >= Uhi);
+      double yy = in ? (newU - start_v) * hinv_v : 0.0;
+      int n;
+      split(yy, n, yy);
+      const double *rb = ring + (n & (kRing - 1)) * 64 + lane;
+      const double g0 = rb[0], g1 = rb[64], g2 = rb[128], g3 = rb[192];
+      double result = 1.0 / 36.0;
+      result *= BV2(1. - yy) * g0 + BV1(1. - yy) * g1 + BV1(yy) * g2 + BV2(yy) * g3;
+      val[i] = in ? result : 0.0;
+    }
+    if (m0 - D >= 0 && m0 + kEB - 1 - D < nU) {
+#pragma unroll
+      for (int i = 0; i < kEB; i++) {
+        double *dst = outLane ? fo + (size_t)(m0 + i - D) * nR : fdump;
+        *dst = val[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kEB; i++) {
+        const int u = m0 + i - D;
+        if (outLane && u >= 0 && u < nU) fo[(size_t)u * nR] = val[i];
+      }
+    }
+  };
+
+  double tt = 0.0 - ax.hi2 * y[Mv - 3];
+  tt = tt - ax.p * y[Mv - 2];
+  const double cLast = div_known(tt, ax.q, ax.rq);
+  const double uS = ax.uS + (double)lz;
+  if (useRing) {
+    // Points whose step would be above the last row have targets outside
+    // the grid (n <= nU-1 < u + D): zero.
+    if (outLane)
+      for (int u = max(Mv - D, 0); u < nU; u++) fo[(size_t)u * nR] = 0.0;
+    // back substitution, last row first; a block of kEB rows is evaluated as
+    // soon as it is complete
+    double cn = cLast, c2 = 0.0;
+    ring_put(Mv - 1, gcomb(cn));
+#pragma unroll
+    for (int m = Mv - 2; m >= 1; m--) {
+      double uv = uS;
+      if (m <= kHead) uv = ax.u[m + lz];
+      const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
+      c2 = cn;
+      cn = ym - uv * cn;
+      ring_put(m, gcomb(cn));
+      if ((Mv - 1 - m) % kEB == kEB - 1) eval_block(m);
+    }
+    ring_put(0, gcomb((ax.A03 - ax.A01 * cn) - ax.A02 * c2));
+    // rows 0..top not covered by a block, and the points whose step would
+    // come after row 0 (D < 0); the ring then holds rows 0..kRing-1
+    if (outLane) {
+      const int top = Mv - 1 < kEB ? Mv - 1 : (Mv - 1) % kEB;
+      for (int u = min(top - D, nU - 1); u >= 0; u--) eval(u, ring_at);
+    }
+  } else {
+    // all of g to the scratch area, then every point
+    double cn = cLast, c2 = 0.0;
+    gs[(size_t)(Mv - 1) * 64] = gcomb(cn);
+#pragma unroll
+    for (int m = Mv - 2; m >= 1; m--) {
+      double uv = uS;
+      if (m <= kHead) uv = ax.u[m + lz];
+      const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
+      c2 = cn;
+      cn = ym - uv * cn;
+      gs[(size_t)m * 64] = gcomb(cn);
+    }
+    gs[0] = gcomb((ax.A03 - ax.A01 * cn) - ax.A02 * c2);
+    gs[(size_t)Mv * 64] = 0.0;  // the row past the padded column, read with weight 0
+    if (outLane)
+      for (int u = 0; u < nU; u++) eval(u, scratch_at);
+  }
+}
+
 // ---------------------------------------------------------------- kick
 // vlasovSolver21_spline.hxx:750-765 with localSpline::evaluateNxPy
 // (src/localSpline.hxx:423-441).
@@ -468,23 +699,29 @@ __device__ __forceinline__ double eval_c(const double *d, int64_t st, double xr,
   return result;
 }
 
-// drift loop, src/vlasovSolver21_spline.hxx:788-796 (delay-line points are
-// written by k_chains)
+// drift loop, src/vlasovSolver21_spline.hxx:788-796.  The foot table carries
+// the point's kind in xr: an evaluated foot has xr > -1, kFootOut (-1) is a
+// foot outside the grid (f = 0), kFootChain (-2) a delay-line point, written
+// by k_chains.  Blocks are spread so that each XCD (block id mod 8) walks a
+// contiguous range of points: the 4x4 stencils of neighbouring points share
+// coefficient rows, which then stay in that XCD's L2.
+constexpr double kFootOut = -1.0, kFootChain = -2.0;
+
 __global__ void __launch_bounds__(kBlock)
 k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ xr,
-        const double *__restrict__ xu, const uint8_t *__restrict__ flag, int nR, int nU,
-        int64_t cs) {
-  const int il = blockIdx.x * blockDim.x + threadIdx.x;
-  if (il >= nU * nR) return;
-  const int j = blockIdx.y;
-  const size_t i = (size_t)j * nU * nR + il;
-  uint8_t fl = flag[i];
-  if (fl == VLS_FOOT_CHAIN) return;
-  if (fl == VLS_FOOT_OUT) {
-    f[i] = 0;
+        const double *__restrict__ xu, int64_t nVal, int nSlice, int64_t cs, int stride,
+        int perXcd) {
+  const int b = blockIdx.x;
+  const int64_t lb = (int64_t)(b & 7) * perXcd + (b >> 3);  // XCD-contiguous block
+  const int64_t i = lb * kBlock + threadIdx.x;
+  if (i >= nVal) return;
+  const double x = xr[i];
+  if (x <= kFootOut) {  // an evaluated foot has trunc(xr) >= 0, so xr > -1
+    if (x == kFootOut) f[i] = 0;
     return;
   }
-  f[i] = eval_c(c + (size_t)j * cs, nR + 2, xr[i], xu[i]);
+  const int64_t j = i / nSlice;
+  f[i] = eval_c(c + j * cs, stride, x, xu[i]);
 }
 
 // delayed central kernel: the back cell takes the spline value at its
@@ -621,7 +858,7 @@ void launch_moment(Run &r, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
                      r.nU, r.nJ);
-  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(1024), r.nR * sizeof(double), s, r.P, r.gfac,
+  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(256), r.nR * sizeof(double), s, r.P, r.gfac,
                      r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
 }
 
@@ -663,12 +900,14 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
 }
 
 // Row version of the register line solver with coalesced memory access.
-// A block of 64 lanes takes 64 consecutive lines (rows of f, contiguous in
-// memory).  Right-hand sides arrive in chunks of kC elements per row through
-// a small LDS transpose (loads of whole row segments, then lane = row reads
-// its kC values); the loads of chunk c+1 are issued before the elimination
-// of chunk c so that their latency hides behind it.  Coefficients leave the
-// same way, a chunk at a time as the back substitution produces them.
+// A block of 64 lanes takes up to 64 consecutive rows u0.. of slice j (rows
+// of f, contiguous in memory).  Right-hand sides arrive in chunks of kC
+// elements per row through a small LDS transpose (loads of whole row
+// segments, then lane = row reads its kC values); the loads of chunk c+1 are
+// issued before the elimination of chunk c so that their latency hides
+// behind it.  Coefficients leave the same way, a chunk at a time as the back
+// substitution produces them.  Loads are clamped into the block's rows (the
+// extra lanes of a partial block solve a copy and store nothing).
 template <int M>
 __global__ void __launch_bounds__(64)
 k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
@@ -679,16 +918,20 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   __shared__ double chunk[64 * kP];
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
-  const int nLines = nJ * nU;
-  const int L0 = blockIdx.x * 64;
+  const int j = blockIdx.y;
+  const int u0 = blockIdx.x * 64;
+  const int rows = min(64, nU - u0);
   const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC elements per pass
+  const double *src = f + ((size_t)j * nU + u0) * nIn;
+  double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double y[M];
   double tmp[kC];
   auto load_chunk = [&](int ch) {
+    const int e = min(ch * kC + ksub, nIn - 1);
 #pragma unroll
     for (int q = 0; q < kC; q++) {
-      const int row = q * 4 + rsub, e = ch * kC + ksub, L = L0 + row;
-      tmp[q] = (L < nLines && e < nIn) ? f[(size_t)L * nIn + e] : 0.0;
+      const int row = min(q * 4 + rsub, rows - 1);
+      tmp[q] = src[row * nIn + e];
     }
   };
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
@@ -741,27 +984,37 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < M) chunk[lane * kP + i] = y[ch * kC + i];
     __syncthreads();
+    // all LDS reads first, so that their latencies overlap, then the stores
+    const int k = ch * kC + ksub;
+    double sv[kC];
 #pragma unroll
-    for (int q = 0; q < kC; q++) {
-      const int row = q * 4 + rsub, k = ch * kC + ksub, L = L0 + row;
-      if (L < nLines && k < M) {
-        const int j = L / nU, u = L - j * nU;
-        c[(size_t)j * cs + (size_t)(u + 1) * M + k] = chunk[row * kP + ksub];
+    for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
+    __syncthreads();
+    if (rows == 64 && (ch + 1) * kC <= M) {
+#pragma unroll
+      for (int q = 0; q < kC; q++) dst[(q * 4 + rsub) * M + k] = sv[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < kC; q++) {
+        const int row = q * 4 + rsub;
+        if (row < rows && k < M) dst[row * M + k] = sv[q];
       }
     }
-    __syncthreads();
   }
 }
 
 // Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
 // standard grids.  Other shapes use the LDS-tiled kernels above.
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
+using KickFn = void (*)(const double *, double *, const double *, const double *, double *, int,
+                        int64_t, double, double, double, double, int, int, ThomasAxis);
 struct LineShape {
   int nR, nU;
   LineFn rows, cols;
+  KickFn kick;
 };
 #define VLS_LINE_SHAPE(R, U) \
-  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>}
+  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>, k_kick_line<(U) + 2, (R) + 2>}
 static const LineShape kLineShapes[] = {
     VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
     VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
@@ -778,9 +1031,8 @@ static const LineShape *line_shape(const Run &r) {
 
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   if (const LineShape *ls = line_shape(r)) {
-    const int lines = r.nJ * r.nU;
-    hipLaunchKernelGGL(ls->rows, dim3((lines + 63) / 64), dim3(64), 0, s, src, r.c, r.nR, r.nU,
-                       r.nJ, r.cs, 0, r.tr);
+    hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
+                       r.nU, r.nJ, r.cs, 0, r.tr);
     time_mark(t, s, "spline_r");
     hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
@@ -797,6 +1049,35 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   time_mark(t, s, "spline_v");
 }
 
+// Doubles of global scratch the fused kick needs (0: no fused kick for this
+// shape): one (nU+3) x 64 area per wave.
+static bool fused_kick(const Run &r) {
+  return line_shape(r) != nullptr && !getenv("VLS_NO_FUSED_KICK");
+}
+
+size_t kick_scratch_doubles(const Run &r) {
+  if (!fused_kick(r)) return 0;
+  const size_t tiles = (r.nR + kKW - 1) / kKW;
+  return tiles * r.nJ * (size_t)(r.nU + 3) * 64;
+}
+
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t) {
+  if (fused_kick(r) && r.gsc) {
+    const LineShape *ls = line_shape(r);
+    hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR,
+                       r.nU, r.nJ, r.cs, 0, r.tr);
+    time_mark(t, s, "spline_r");
+    hipLaunchKernelGGL(ls->kick, dim3((r.nR + kKW - 1) / kKW, r.nJ), dim3(64), 0, s, r.c, r.f,
+                       r.U_Val, r.acc, r.gsc, r.nR, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
+                       r.kickScratch ? 1 : 0, 0, r.tv);
+    time_mark(t, s, "kick_line");
+    return;
+  }
+  launch_spline(r, r.f, s, t);
+  launch_kick(r, s);
+  time_mark(t, s, "kick");
+}
+
 void launch_kick(Run &r, hipStream_t s) {
   dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
   hipLaunchKernelGGL(k_kick, g, dim3(kBlock), 0, s, r.f, r.c, r.U_Val, r.acc, r.nR, r.nU, r.nJ,
@@ -804,9 +1085,10 @@ void launch_kick(Run &r, hipStream_t s) {
 }
 
 void launch_drift(Run &r, hipStream_t s) {
-  dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
-  hipLaunchKernelGGL(k_drift, g, dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.flag, r.nR, r.nU,
-                     r.cs);
+  const int64_t nb = (r.nVal + kBlock - 1) / kBlock;
+  const int perXcd = (int)((nb + 7) / 8);
+  hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.nVal,
+                     r.nU * r.nR, r.cs, r.nR + 2, perXcd);
 }
 
 void launch_chains(Run &r, hipStream_t s) {
