@@ -53,12 +53,56 @@ def algorithmic_bytes(name, nR, nU, nJ):
     table = {
         "moment": 8 * n,                 # read f
         "spline_r": 8 * n + 8 * rows,    # read f, write the r-solved rows
+        "spline_r_moment": 8 * n + 8 * rows,  # the same, with the moment partials of f
         "spline_v": 8 * rows + 8 * coef,  # read them, write all coefficients
         "kick": 8 * coef + 8 * n,        # read coefficients, write f
         "kick_line": 8 * rows + 8 * n,   # fused v-solve + kick: read the r-solved rows, write f
         "drift": 8 * coef + 8 * n,       # read coefficients, write f
     }
     return table.get(name)
+
+
+# The adjoint's workload: PLDF power-law sphere (n=1, alpha=1, eta=0.1, M=1,
+# smoothing 0.5) on 129 log-r x 129 v x 64 quadratic-j nodes, dt=0.005,
+# 2000 steps, checkpoints every 50 steps.
+ADJ_WORKLOAD = {
+    "init.type": "PLDF", "init.n": 1, "init.alpha": 1, "init.eta": 0.1, "init.M": 1,
+    "init.smoothingRadius": 0.5, "grid.Pres": 128, "grid.Ures": [128, 64], "grid.Pmax": 30,
+    "grid.Umin": [-1.5, 0], "grid.Umax": [1.5, 1.6], "solver.dt": 0.005,
+}
+ADJ_STEPS, ADJ_CK = 2000, 50
+
+
+def adjoint_ratio(V):
+    """wall(gradient of a mass-profile loss: recorded forward run with
+    checkpoints + reverse sweep recomputing each segment) / wall(forward)."""
+    import numpy as np
+
+    s = V.Solver(ADJ_WORKLOAD)
+    ctx = s.context
+    f0 = s.f()
+    head0, vals0 = ctx.get_chains()
+    ctx.step(20)  # warm-up
+    ctx.synchronize()
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, vals0)
+    t0 = time.perf_counter()
+    ctx.step(ADJ_STEPS)
+    ctx.synchronize()
+    fwd = time.perf_counter() - t0
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, vals0)
+    t0 = time.perf_counter()
+    ctx.record(ADJ_STEPS, ADJ_CK)
+    M = ctx.mass_profile()
+    g0, _ = ctx.adjoint(dL_dM=2 * (M - 1.01 * M))
+    grad = time.perf_counter() - t0
+    if not np.isfinite(g0).all():
+        raise RuntimeError("non-finite gradient")
+    return {"workload": "PLDF n=1 eta=0.1, 129 log-r x 129 v x 64 quadratic-j, dt 0.005",
+            "steps": ADJ_STEPS, "ck_every": ADJ_CK, "forward_s": round(fwd, 4),
+            "gradient_s": round(grad, 4), "ratio": round(grad / fwd, 3),
+            "loss": "sum_r (M(<r) - M_target)^2 of the final state"}
 
 
 def cpu_threads():
@@ -127,6 +171,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default 150, ~10-20 s of CPU work)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -169,6 +214,13 @@ def main():
                 "kernels_us_per_step": {k: round(v[0] / args.steps * 1e3, 2)
                                         for k, v in sorted(per.items())}}
 
+    adj = None
+    if not args.no_adjoint:
+        try:
+            adj = adjoint_ratio(V)
+        except Exception as e:  # reported, not fatal for the forward figure
+            adj = {"ratio": None, "error": str(e)[:200]}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -186,6 +238,7 @@ def main():
                    "points": npts, "global_batch": 1, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(line))
 

@@ -149,6 +149,37 @@ int vls_copy_probe(size_t bytes);
 int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
                        int64_t *mismatches);
 
+/* Sizes of one run: {nR, nU, nJ, number of delay lines, delay-line cells}. */
+int vls_run_info(vls_ctx *ctx, int run, int64_t out[5]);
+/* Overwrite the delay-line state (the inverse of vls_get_chains): ring heads
+ * and ring values in the layout of vls_set_chains.  Either may be NULL. */
+int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *vals);
+/* Enclosed-mass profile of the current f: M[r] = M(<R_r), the exclusive
+ * radial sum of the moment (quadrature21::integrate1D and
+ * gridHandler::accumulate, src/quadrature21.hxx:88-142,
+ * src/grid_handler.hxx:350-407).  Refreshes the moments of vls_moments. */
+int vls_mass_profile(vls_ctx *ctx, int run, double *M);
+
+/* Discrete adjoint of the step.  No counterpart in the reference (it has no
+ * derivative); it differentiates exactly the discrete map that vls_step
+ * applies (linking steps).
+ *
+ * vls_record advances run `run` by nSteps steps like vls_step, keeping f and
+ * the delay-line state every ckEvery steps in HBM (checkpoints).
+ * vls_adjoint then returns the gradient of a loss L(f_N) with respect to the
+ * f and the delay-line values of the first recorded step, given dL_dM
+ * ([nR], the loss's derivative in the mass profile of f_N as returned by
+ * vls_mass_profile; may be NULL) and dL_df ([nJ][nU][nR], its direct
+ * derivative in f_N; may be NULL).  The states between checkpoints are
+ * recomputed segment by segment; the context ends in the state after the
+ * recorded run.  grad_vals0 (may be NULL) has the ring layout of
+ * vls_set_chains at the first recorded step.  The drift's transpose uses
+ * floating-point atomics, so the last bits of the gradient may vary from call
+ * to call. */
+int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery);
+int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
+                double *grad_f0, double *grad_vals0);
+
 /* ---------------------------------------------------------------- layer 2 */
 
 typedef struct vlsd_solver vlsd_solver;

@@ -88,6 +88,13 @@ def lib():
         L.orc_advance_drift.argtypes = [d, d, d, d, d, P(d), P(d)]
         L.orc_gen_scale.restype = C.c_long
         L.orc_gen_scale.argtypes = [d, d, C.c_long, C.c_int, C.c_int, P(d)]
+        L.orc_set_chain_vals.argtypes = [C.c_void_p, P(d)]
+        L.orc_tl_init.restype = C.c_int
+        L.orc_tl_init.argtypes = [C.c_void_p, P(d), P(d)]
+        L.orc_tl_step.restype = C.c_int
+        L.orc_tl_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_tl_get.argtypes = [C.c_void_p, P(d)]
+        L.orc_tl_mass.argtypes = [C.c_void_p, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -276,6 +283,34 @@ class Oracle:
         L.orc_get_chains(self._h, _ptr(point, C.c_int64), _ptr(off, C.c_int64),
                          _ptr(ln, C.c_int32), _ptr(R), _ptr(U), _ptr(V))
         return dict(point=point, off=off, len=ln, R=R, U=U, val=V)
+
+    # ---- tangent-linear model (checks the adjoint of the HIP path)
+    def set_chain_vals(self, vals):
+        v = np.ascontiguousarray(vals, np.float64)
+        lib().orc_set_chain_vals(self._h, _ptr(v))
+
+    def tl_init(self, df, dvals=None):
+        d = np.ascontiguousarray(df, np.float64).reshape(-1)
+        assert d.size == self.nR * self.nU * self.nJ
+        dv = None if dvals is None else np.ascontiguousarray(dvals, np.float64)
+        if lib().orc_tl_init(self._h, _ptr(d), _ptr(dv) if dv is not None else None) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def tl_step(self, n=1):
+        if lib().orc_tl_step(self._h, int(n)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def tl_get(self):
+        out = np.empty(self.shape)
+        lib().orc_tl_get(self._h, _ptr(out))
+        return out
+
+    def tl_mass(self):
+        """exclusive-scan mass profile M(<R_r) of f and its tangent"""
+        M = np.empty(self.nR)
+        dM = np.empty(self.nR)
+        lib().orc_tl_mass(self._h, _ptr(M), _ptr(dM))
+        return M, dM
 
     def spline_build(self, slice2d):
         a = np.ascontiguousarray(slice2d, np.float64)

@@ -94,6 +94,7 @@ typedef struct {
   long j;
   long n, cap;
   double *R, *U, *val;
+  double *tval; /* tangent of val (tangent-linear model), or NULL */
 } chain_t;
 
 struct orc_state {
@@ -123,6 +124,9 @@ struct orc_state {
   long *chainStart;   /* per slice j: first chain id (chains sorted by j) */
   /* last step's moments */
   double *dM, *Mr, mTot;
+  /* tangent-linear model: tangents of f, of the coefficients and of the
+   * acceleration (NULL until orc_tl_init) */
+  double *tf, *tcoef, *tMr;
   double time;
   long stepCount;
 };
@@ -499,18 +503,33 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv)
     free(tmp);
     return;
   }
-  /* device order: P[j][r] = serial sum over u, then dM[r] = serial sum over j */
+  /* device order: the rows u of a slice form waves of 64 and groups of 16
+   * (u = 64w + 16q + t); G_wq = serial sum over the group's rows,
+   * W_w = ((G_w0 + G_w1) + G_w2) + G_w3, P[j][r] = ((W_0 + W_1) + ...) over
+   * the waves, then dM[r] = serial sum over j */
   double *P = calloc((size_t)nJ * nR, sizeof(double));
+  const long nW = (nU + 63) / 64;
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < nJ; j++) {
     double Ct1 = 8 * M_PI * M_PI * s->J_Val[j] * get_dJ(s, j);
     for (long rr = 0; rr < nR - 1; rr++) {
-      double acc = 0;
-      for (long u = 0; u < nU - 1; u++) {
-        double tval = moment_term(s, f, j, u, rr, Ct1);
-        if (tval == tval) acc += tval;
+      double Pj = 0.0;
+      for (long w = 0; w < nW; w++) {
+        double G[4];
+        for (int q = 0; q < 4; q++) {
+          double acc = 0.0;
+          for (long t = 0; t < 16; t++) {
+            long u = 64 * w + 16 * q + t;
+            if (u >= nU - 1) continue;
+            double tval = moment_term(s, f, j, u, rr, Ct1);
+            if (tval == tval) acc += tval;
+          }
+          G[q] = acc;
+        }
+        double W = ((G[0] + G[1]) + G[2]) + G[3];
+        Pj = w == 0 ? W : Pj + W;
       }
-      P[(size_t)j * nR + rr] = acc;
+      P[(size_t)j * nR + rr] = Pj;
     }
   }
   for (r = 0; r < nR; r++) {
@@ -915,6 +934,164 @@ fail:
   return NULL;
 }
 
+/* ------------------------------------------------------------------ */
+/* Tangent-linear model of the step (linking, delayed or reflective kernel):
+ * the derivative of the discrete map in a direction (df, dvals).  Every
+ * phase of the step is linear in (f, chain values) except the kick, whose
+ * target moves with the acceleration a(f); its tangent adds
+ * d(evaluateNxPy)/dnewU * da.  Not part of the reference: this is the check
+ * for the adjoint of the HIP path. */
+
+/* d/dnewU of evalNxPy for the linear v scale: hinv/36 * sum B_k'(t) G_k */
+static double evalNxPy_dU(const orc_state *s, const double *data, int Nx, double yy)
+{
+  const long st = s->stride1;
+  Nx++;
+  double y = (yy - s->sstart[1]) * s->sstep_inv[1];
+  y = modf(y, &yy);
+  int Ny = 1 + (int)yy;
+  const long ref[4] = {Nx + (Ny - 1) * st, Nx + Ny * st, Nx + (Ny + 1) * st, Nx + (Ny + 2) * st};
+  double G[4];
+  for (int k = 0; k < 4; k++) G[k] = data[ref[k] - 1] + data[ref[k]] * 4 + data[ref[k] + 1];
+  const double w = 1 - y;
+  const double d0 = -3 * w * w, d1 = -(3 + 6 * w - 9 * w * w), d2 = 3 + 6 * y - 9 * y * y,
+               d3 = 3 * y * y;
+  return s->sstep_inv[1] / 36.0 * (d0 * G[0] + d1 * G[1] + d2 * G[2] + d3 * G[3]);
+}
+
+/* overwrite the delay-line values, cells front first in chain order */
+void orc_set_chain_vals(orc_state *s, const double *vals)
+{
+  long off = 0;
+  for (long c = 0; c < s->nChains; c++) {
+    memcpy(s->chains[c].val, vals + off, s->chains[c].n * sizeof(double));
+    off += s->chains[c].n;
+  }
+}
+
+int orc_tl_init(orc_state *s, const double *df, const double *dvals)
+{
+  if (s->p.noLinking || s->p.Uscale[0] != ORC_LINEAR) {
+    set_err("tangent-linear model: linking steps and a linear v scale only");
+    return -1;
+  }
+  if (!s->tf) {
+    s->tf = malloc(s->nVal * sizeof(double));
+    s->tcoef = calloc((size_t)s->nJ * s->cstride + 2 * (size_t)(s->sdim[0] + 2) + 8, sizeof(double));
+    s->tMr = malloc(s->nR * sizeof(double));
+  }
+  memcpy(s->tf, df, s->nVal * sizeof(double));
+  long off = 0;
+  for (long c = 0; c < s->nChains; c++) {
+    chain_t *ch = &s->chains[c];
+    free(ch->tval);
+    ch->tval = malloc(ch->n * sizeof(double));
+    for (long k = 0; k < ch->n; k++) ch->tval[k] = dvals ? dvals[off + k] : 0.0;
+    off += ch->n;
+  }
+  return 0;
+}
+
+void orc_tl_get(const orc_state *s, double *df) { memcpy(df, s->tf, s->nVal * sizeof(double)); }
+
+/* exclusive-scan mass profile M(<R_r) of f and its tangent */
+void orc_tl_mass(orc_state *s, double *M, double *dM)
+{
+  double *tmp = malloc(s->nR * sizeof(double));
+  integrate1D(s, s->f, tmp);
+  accumulate(tmp, M, s->nR);
+  integrate1D(s, s->tf, tmp);
+  accumulate(tmp, dM, s->nR);
+  free(tmp);
+}
+
+static void tl_build(orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++)
+    build_slice(s, s->tmpArr + (size_t)thread_id() * s->tmpStride, s->tf + j * s->nU * s->nR,
+                s->tcoef + (size_t)j * s->cstride);
+}
+
+static void tl_step1(orc_state *s)
+{
+  const long nR = s->nR, nU = s->nU, nJ = s->nJ;
+  /* moment and acceleration, and their tangents */
+  moment_and_accel(s);
+  double *tdM = malloc(nR * sizeof(double));
+  integrate1D(s, s->tf, tdM);
+  accumulate(tdM, s->tMr, nR);
+  for (long i = 0; i < nR; i++) s->tMr[i] *= s->p.G / (s->R_Val[i] * s->R_Val[i]) * s->p.dt;
+  free(tdM);
+  /* kick */
+  build_all(s);
+  tl_build(s);
+  const double Ulo = s->U_Val[0], Uhi = s->U_Val[nU - 1];
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < nJ; j++) {
+    const double *c = s->coef + (size_t)j * s->cstride;
+    const double *tc = s->tcoef + (size_t)j * s->cstride;
+    for (long u = 0; u < nU; u++)
+      for (long r = 0; r < nR; r++) {
+        long i = (j * nU + u) * nR + r;
+        double newU = s->U_Val[u] + s->Mr[r];
+        if ((newU <= Ulo) || (newU >= Uhi)) {
+          s->f[i] = 0;
+          s->tf[i] = 0;
+        } else {
+          s->f[i] = evalNxPy(s, c, (int)r, newU);
+          s->tf[i] = evalNxPy(s, tc, (int)r, newU) + evalNxPy_dU(s, c, (int)r, newU) * s->tMr[r];
+        }
+      }
+  }
+  /* drift with the delay lines */
+  build_all(s);
+  tl_build(s);
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < nJ; j++) {
+    const double *c = s->coef + (size_t)j * s->cstride;
+    const double *tc = s->tcoef + (size_t)j * s->cstride;
+    if (s->p.kernelType == ORC_DELAYED)
+      for (long k = s->chainStart[j]; k < s->chainStart[j + 1]; k++) {
+        chain_t *ch = &s->chains[k];
+        double R = ch->R[ch->n - 1], U = ch->U[ch->n - 1];
+        int out = (U <= Ulo) || (U >= Uhi);
+        ch->val[ch->n - 1] = out ? 0 : eval_xy(s, c, R, U);
+        ch->tval[ch->n - 1] = out ? 0 : eval_xy(s, tc, R, U);
+      }
+    for (long u = 0; u < nU; u++)
+      for (long r = 0; r < nR; r++) {
+        long i = (j * nU + u) * nR + r;
+        if (s->driftR[i] < s->kernelRadius) {
+          chain_t *ch = &s->chains[s->chainOfPoint[i]];
+          double t0 = ch->tval[0];
+          for (long k = 0; k + 1 < ch->n; k++) ch->tval[k] = ch->tval[k + 1];
+          ch->tval[ch->n - 1] = t0;
+          s->tf[i] = t0;
+          s->f[i] = chain_read_roll(ch);
+        } else if (s->driftOut[i]) {
+          s->f[i] = 0;
+          s->tf[i] = 0;
+        } else {
+          s->f[i] = evalC(s->stride1, c, s->driftR_I[i], s->driftR_F[i], s->driftU_I[i], s->driftU_F[i]);
+          s->tf[i] = evalC(s->stride1, tc, s->driftR_I[i], s->driftR_F[i], s->driftU_I[i], s->driftU_F[i]);
+        }
+      }
+  }
+  s->time += s->p.dt;
+  s->stepCount++;
+}
+
+int orc_tl_step(orc_state *s, int nsteps)
+{
+  if (!s->tf) {
+    set_err("orc_tl_step: call orc_tl_init first");
+    return -1;
+  }
+  for (int n = 0; n < nsteps; n++) tl_step1(s);
+  return 0;
+}
+
 void orc_destroy(orc_state *s)
 {
   if (!s) return;
@@ -922,6 +1099,7 @@ void orc_destroy(orc_state *s)
     free(s->chains[c].R);
     free(s->chains[c].U);
     free(s->chains[c].val);
+    free(s->chains[c].tval);
   }
   free(s->chains);
   free(s->R_Val); free(s->R_V); free(s->U_Val); free(s->U_V); free(s->J_Val); free(s->J_V);
@@ -929,6 +1107,7 @@ void orc_destroy(orc_state *s)
   free(s->driftR); free(s->driftU); free(s->driftOut);
   free(s->driftR_I); free(s->driftU_I); free(s->driftR_F); free(s->driftU_F);
   free(s->chainOfPoint); free(s->chainStart); free(s->dM); free(s->Mr);
+  free(s->tf); free(s->tcoef); free(s->tMr);
   free(s);
 }
 

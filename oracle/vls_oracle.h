@@ -46,8 +46,9 @@ typedef struct orc_params {
   double Ra;                                        /* Hernquist */
   /* Reduction order.  nthreads >= 1: the reference's order with that many
    * OpenMP threads under a static schedule (quadrature21.hxx:96-139).
-   * nthreads == 0: the device order (per-slice serial partials over u, then a
-   * serial sum over j), which the HIP kernels use. */
+   * nthreads == 0: the device order, which the HIP kernels use: per slice
+   * and r, serial sums over groups of 16 rows u, combined four to a wave of
+   * 64 rows and then over the waves, then a serial sum over j. */
   int nthreads;
 } orc_params;
 
@@ -96,6 +97,17 @@ void orc_advance_drift(double R, double U, double J, double dt, double R0, doubl
                        double *newU);
 /* scale generation (scale.hxx:114-151); returns the number of values */
 long orc_gen_scale(double start, double stop, long N, int sct, int cell, double *out);
+
+/* Tangent-linear model (directional derivative of the step; linking steps
+ * and a linear v scale).  dvals: tangents of the delay-line cells in the
+ * order of orc_get_chains, or NULL for zero.  orc_tl_step advances the state
+ * and its tangent together; orc_tl_mass returns the exclusive-scan mass
+ * profile M(<R_r) of the current f and its tangent. */
+int orc_tl_init(orc_state *s, const double *df, const double *dvals);
+void orc_set_chain_vals(orc_state *s, const double *vals);
+int orc_tl_step(orc_state *s, int nsteps);
+void orc_tl_get(const orc_state *s, double *df);
+void orc_tl_mass(orc_state *s, double *M, double *dM);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,177 @@
+"""The discrete adjoint of the HIP step, through the C ABI.
+
+Checks: (1) against the oracle's tangent-linear model (an independent,
+forward-mode derivative of the same discrete map, itself pinned against
+finite differences on the CPU): the adjoint identity <w, J v> = <J^T w, v>
+for random directions, over several checkpoint segments; (2) against central
+finite differences of the GPU forward run, to the 1e-6 the adjoint is
+specified to; (3) the torch.autograd harness; (4) the gradient with respect
+to an initial-condition parameter (the PLDF virial ratio eta), through the
+host set-up's derivative."""
+import numpy as np
+import pytest
+
+from tests.configs import UDF_TINY, PLDF_SMALL
+
+pytestmark = pytest.mark.gpu
+
+V = pytest.importorskip("vlasolve_amd")
+from oracle import oracle as O  # noqa: E402
+
+
+def ring_to_front(slots, head, off, ln):
+    out = np.empty_like(slots)
+    for k in range(len(head)):
+        L, o = ln[k], off[k]
+        idx = (head[k] + np.arange(L)) % L
+        out[o:o + L] = slots[o + idx]
+    return out
+
+
+def chain_layout(cfg):
+    ch = O.Oracle(cfg, nthreads=1).chains()
+    return ch["off"], ch["len"]
+
+
+@pytest.mark.parametrize("n,ck", [(6, 4), (5, 1)])
+def test_adjoint_matches_tangent_linear(n, ck):
+    print(f"case udf_tiny adjoint n={n} ck={ck}", flush=True)
+    cfg = UDF_TINY
+    solver = V.Solver(cfg)
+    ctx = solver.context
+    f0 = solver.f()
+    head0, slots0 = ctx.get_chains()
+    off, ln = chain_layout(cfg)
+    assert len(head0) > 0, "the case must exercise the delay lines"
+    ctx.record(n, ck)
+    M = ctx.mass_profile()
+    rng = np.random.default_rng(11)
+    w = rng.standard_normal(M.shape)
+    g0, gv = ctx.adjoint(dL_dM=w)
+    gv_front = ring_to_front(gv, head0, off, ln)
+
+    orc = O.Oracle(cfg, nthreads=0)
+    assert np.array_equal(orc.f(), f0)
+    df = rng.standard_normal(f0.shape) * np.abs(f0).max()
+    dv = rng.standard_normal(gv.shape) * np.abs(slots0).max()
+    orc.tl_init(df, dv)
+    orc.tl_step(n)
+    Mo, dMo = orc.tl_mass()
+    assert np.allclose(Mo, M, rtol=1e-12, atol=1e-15)
+    lhs = float(w @ dMo)
+    rhs = float((g0 * df).sum() + (gv_front * dv).sum())
+    print(f"<w, J v> = {lhs:.15e}  <J^T w, v> = {rhs:.15e}")
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # the context is back in the state after the recorded run
+    assert np.array_equal(ctx.download_f(0, f0.shape), _after(cfg, n))
+
+
+def _after(cfg, n):
+    o = O.Oracle(cfg, nthreads=0)
+    o.step(n)
+    return o.f()
+
+
+def _loss_run(ctx, f0, head0, slots0, n, Mt):
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, slots0)
+    ctx.step(n)
+    M = ctx.mass_profile()
+    return float(((M - Mt) ** 2).sum()), M
+
+
+def test_adjoint_against_gpu_finite_differences():
+    print("case udf_tiny finite differences", flush=True)
+    cfg = UDF_TINY
+    n = 8
+    solver = V.Solver(cfg)
+    ctx = solver.context
+    f0 = solver.f()
+    head0, slots0 = ctx.get_chains()
+    rng = np.random.default_rng(5)
+    _, Mref = _loss_run(ctx, f0, head0, slots0, n, 0.0)
+    Mt = Mref * (1 + 0.01 * rng.standard_normal(Mref.shape))
+    # gradient
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, slots0)
+    ctx.record(n, 3)
+    M = ctx.mass_profile()
+    g0, gv = ctx.adjoint(dL_dM=2 * (M - Mt))
+    v = rng.standard_normal(f0.shape) * np.abs(f0).max()
+    vv = rng.standard_normal(slots0.shape) * np.abs(slots0).max()
+    h = 1e-7
+    Lp, _ = _loss_run(ctx, f0 + h * v, head0, slots0 + h * vv, n, Mt)
+    Lm, _ = _loss_run(ctx, f0 - h * v, head0, slots0 - h * vv, n, Mt)
+    fd = (Lp - Lm) / (2 * h)
+    ad = float((g0 * v).sum() + (gv * vv).sum())
+    print(f"finite differences {fd:.12e}  adjoint {ad:.12e}")
+    assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def test_autograd_harness():
+    torch = pytest.importorskip("torch")
+    from vlasolve_amd.autograd import MassProfileRun
+
+    print("case udf_tiny torch.autograd", flush=True)
+    cfg = UDF_TINY
+    n = 4
+    solver = V.Solver(cfg)
+    ctx = solver.context
+    f0 = solver.f()
+    head0, slots0 = ctx.get_chains()
+    Fn = MassProfileRun()
+    f0_t = torch.tensor(f0, requires_grad=True)
+    v0_t = torch.tensor(slots0, requires_grad=True)
+    M = Fn.apply(f0_t, v0_t, ctx, n, 2, head0)
+    Mt = M.detach() * 1.01
+    loss = ((M - Mt) ** 2).sum()
+    loss.backward()
+    # the same gradient straight from the C ABI
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, slots0)
+    ctx.record(n, 2)
+    M2 = ctx.mass_profile()
+    g0, gv = ctx.adjoint(dL_dM=2 * (M2 - Mt.numpy()))
+    assert np.allclose(f0_t.grad.numpy(), g0, rtol=1e-12, atol=1e-300)
+    assert np.allclose(v0_t.grad.numpy(), gv, rtol=1e-12, atol=1e-300)
+
+
+def test_gradient_wrt_pldf_eta():
+    """dL/deta = <dL/df0, df0/deta> + <dL/dvals0, dvals0/deta>, the
+    set-up's derivative by central differences of the host initial
+    conditions, against central differences of the whole run in eta."""
+    print("case pldf eta gradient", flush=True)
+    base = dict(PLDF_SMALL, **{"grid.Pres": 32, "grid.Ures": [32, 12]})
+    eta, n, h = 0.1, 6, 1e-6
+
+    def cfg_at(e):
+        return dict(base, **{"init.eta": e})
+
+    solver = V.Solver(cfg_at(eta))
+    ctx = solver.context
+    f0 = solver.f()
+    head0, slots0 = ctx.get_chains()
+    off, ln = chain_layout(cfg_at(eta))
+    _, Mref = _loss_run(ctx, f0, head0, slots0, n, 0.0)
+    Mt = Mref * 1.02
+
+    def run_loss(e):
+        s = V.Solver(cfg_at(e))
+        c = s.context
+        c.step(n)
+        M = c.mass_profile()
+        return float(((M - Mt) ** 2).sum())
+
+    fd = (run_loss(eta + h) - run_loss(eta - h)) / (2 * h)
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, slots0)
+    ctx.record(n, n)
+    M = ctx.mass_profile()
+    g0, gv = ctx.adjoint(dL_dM=2 * (M - Mt))
+    hp, hm = V.HostSetup(cfg_at(eta + h)), V.HostSetup(cfg_at(eta - h))
+    df = (hp.f() - hm.f()) / (2 * h)
+    dv_front = (hp.chains()["val"] - hm.chains()["val"]) / (2 * h)
+    gv_front = ring_to_front(gv, head0, off, ln)
+    ad = float((g0 * df).sum() + (gv_front * dv_front).sum())
+    print(f"dL/deta finite differences {fd:.10e}  adjoint {ad:.10e}")
+    assert abs(fd - ad) <= 1e-6 * abs(fd)

@@ -28,7 +28,8 @@ EXPORTS = (
     "vlsd_moments", "vlsd_run", "vlsd_write_restart", "vlsd_snapshot", "vlsd_context",
     "vlsd_last_error", "vls_thomas_table", "vls_check_division", "vls_device_info",
     "vls_copy_probe", "vlsd_host_create", "vlsd_host_destroy", "vlsd_host_info", "vlsd_host_f",
-    "vlsd_host_feet", "vlsd_host_chains",
+    "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
+    "vls_record", "vls_adjoint",
 )
 
 
@@ -93,6 +94,11 @@ def lib():
         L.vlsd_snapshot.argtypes = [vp]
         L.vlsd_context.argtypes = [vp]
         L.vlsd_context.restype = vp
+        L.vls_run_info.argtypes = [vp, C.c_int, P(i64)]
+        L.vls_put_chains.argtypes = [vp, C.c_int, P(i32), P(d)]
+        L.vls_mass_profile.argtypes = [vp, C.c_int, P(d)]
+        L.vls_record.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.vls_adjoint.argtypes = [vp, C.c_int, P(d), P(d), P(d), P(d)]
         _lib = L
     return _lib
 
@@ -260,6 +266,47 @@ class Context:
 
     def set_timing(self, on=True):
         _check(lib().vls_set_timing(self._h, int(bool(on))))
+
+    def info(self, run=0):
+        """{nR, nU, nJ, chains, cells} of a run"""
+        out = (C.c_int64 * 5)()
+        _check(lib().vls_run_info(self._h, run, out))
+        return dict(zip(("nR", "nU", "nJ", "chains", "cells"), list(out)))
+
+    def get_chains(self, run=0):
+        """(ring heads, ring values) of the delay lines"""
+        inf = self.info(run)
+        head = np.zeros(inf["chains"], np.int32)
+        vals = np.zeros(inf["cells"])
+        _check(lib().vls_get_chains(self._h, run, _ptr(head, C.c_int32), _ptr(vals)))
+        return head, vals
+
+    def put_chains(self, head=None, vals=None, run=0):
+        h = None if head is None else np.ascontiguousarray(head, np.int32)
+        v = None if vals is None else np.ascontiguousarray(vals, np.float64)
+        _check(lib().vls_put_chains(self._h, run, _ptr(h, C.c_int32) if h is not None else None,
+                                    _ptr(v) if v is not None else None))
+
+    def mass_profile(self, run=0):
+        """M(<R_r) of the current f"""
+        out = np.empty(self.info(run)["nR"])
+        _check(lib().vls_mass_profile(self._h, run, _ptr(out)))
+        return out
+
+    def record(self, n_steps, ck_every, run=0):
+        _check(lib().vls_record(self._h, run, int(n_steps), int(ck_every)))
+
+    def adjoint(self, dL_dM=None, dL_df=None, run=0):
+        """(grad_f0, grad_vals0) of a loss of the recorded run's final state"""
+        inf = self.info(run)
+        shape = (inf["nJ"], inf["nU"], inf["nR"])
+        gm = None if dL_dM is None else np.ascontiguousarray(dL_dM, np.float64)
+        gf = None if dL_df is None else np.ascontiguousarray(dL_df, np.float64).reshape(shape)
+        g0 = np.empty(shape)
+        gv = np.empty(inf["cells"])
+        _check(lib().vls_adjoint(self._h, run, _ptr(gm) if gm is not None else None,
+                                 _ptr(gf) if gf is not None else None, _ptr(g0), _ptr(gv)))
+        return g0, gv
 
     def kernel_times(self, cap=4096):
         names = (C.c_char_p * cap)()

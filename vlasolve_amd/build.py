@@ -26,7 +26,7 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 CLANGXX = str(ROCM / "llvm" / "bin" / "clang++")
 ARCH = os.environ.get("VLS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["vls_kernels.hip", "vls_capi.hip"]
+HIP_SOURCES = ["vls_kernels.hip", "vls_capi.hip", "vls_adjoint.hip"]
 HOST_SOURCES = ["host/params.cpp", "host/grid.cpp", "host/solver.cpp", "host/driver.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",

@@ -56,7 +56,29 @@ void h2d(T *dst, const T *src, size_t n, hipStream_t s) {
   if (n) VLS_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+template <class T>
+void d2d(T *dst, const T *src, size_t n, hipStream_t s) {
+  if (n) VLS_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
 vls::Run &run_of(vls_ctx *ctx, int run);
+
+// One step of a run on stream s (vlasovSolver21_spline::makeOneStep,
+// src/vlasovSolver21_spline.hxx:680-801): with linking, kick (moment, spline,
+// evaluation) then drift (spline, evaluation with the delay lines); without,
+// half drift, kick, half drift.
+void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
+  if (r.sp.no_linking) {
+    vls::launch_spline(r, r.f, s, t);
+    vls::launch_drift(r, s);
+    vls::time_mark(t, s, "drift");
+  }
+  vls::launch_spline_kick(r, s, t);
+  vls::launch_spline(r, r.f, s, t);
+  vls::launch_drift(r, s);
+  vls::time_mark(t, s, "drift");
+  r.stepped = true;
+}
 }  // namespace
 
 struct vls_ctx {
@@ -252,7 +274,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
   vls_ctx *ctx = new vls_ctx();
   int rc = guarded([&] {
     VLS_HIP_CHECK(hipGetDevice(&ctx->c.device));
-    int maxLds = 0;
+    int &maxLds = ctx->c.maxLds;
     VLS_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeSharedMemPerBlockOptin,
                                         ctx->c.device));
     if (maxLds <= 0)
@@ -383,13 +405,19 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
         throw vls::Error(VLS_ERR_ARG, "vls_set_drift: foot outside the spline domain at point " +
                                           std::to_string(i));
     }
-    // the drift kernel reads the point's kind from the sign of xr
-    std::vector<double> xf(xr, xr + r.nVal);
-    for (int64_t i = 0; i < r.nVal; i++)
-      if (flag[i] != VLS_FOOT_EVAL) xf[i] = flag[i] == VLS_FOOT_OUT ? -1.0 : -2.0;
+    // the drift kernel reads the point's kind from xr (-1 outside, -2 delay
+    // line) and a delay-line point's chain index from xu (-1: none yet)
+    std::vector<double> xf(xr, xr + r.nVal), uf(xu, xu + r.nVal);
+    for (int64_t i = 0; i < r.nVal; i++) {
+      if (flag[i] == VLS_FOOT_OUT) xf[i] = -1.0;
+      if (flag[i] == VLS_FOOT_CHAIN) xf[i] = -2.0, uf[i] = -1.0;
+    }
+    r.hflag.assign(flag, flag + r.nVal);
     hipStream_t s = ctx->c.stream;
     h2d(r.xr, xf.data(), r.nVal, s);
-    h2d(r.xu, xu, r.nVal, s);
+    h2d(r.xu, uf.data(), r.nVal, s);
+    vls::mark_chains(r, s);
+    VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.have_drift = true;
   });
@@ -404,11 +432,15 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     if (nChains > 0 && (!point || !len || !head || !back_xr || !back_xu || !back_out || !vals))
       throw vls::Error(VLS_ERR_ARG, "null chain table");
     std::vector<int64_t> off(nChains);
+    std::vector<uint8_t> seen(nChains ? r.nVal : 0, 0);
     int64_t cells = 0;
     for (int64_t k = 0; k < nChains; k++) {
       if (len[k] < 2) throw vls::Error(VLS_ERR_ARG, "chain shorter than 2 cells");
       if (head[k] < 0 || head[k] >= len[k]) throw vls::Error(VLS_ERR_ARG, "chain head out of range");
       if (point[k] < 0 || point[k] >= r.nVal) throw vls::Error(VLS_ERR_ARG, "chain point out of range");
+      if (!r.hflag.empty() && r.hflag[point[k]] != VLS_FOOT_CHAIN)
+        throw vls::Error(VLS_ERR_ARG, "chain attached to a point whose foot is not a delay line");
+      if (seen[point[k]]++) throw vls::Error(VLS_ERR_ARG, "two chains attached to one point");
       if (!back_out[k] && !foot_in_domain(back_xr[k], back_xu[k], r))
         throw vls::Error(VLS_ERR_ARG, "chain back cell outside the spline domain");
       off[k] = cells;
@@ -434,6 +466,8 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     h2d(r.ch_xu, back_xu, nChains, s);
     h2d(r.ch_out, back_out, nChains, s);
     h2d(r.ch_vals, vals, cells, s);
+    if (r.have_drift) vls::mark_chains(r, s);
+    VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.have_chains = true;
   });
@@ -467,25 +501,8 @@ int vls_step(vls_ctx *ctx, int nSteps) {
     for (auto &r : ctx->c.runs)
       if (!r.have_f || !r.have_drift)
         throw vls::Error(VLS_ERR_STATE, "vls_step: f and drift feet must be set first");
-    for (int n = 0; n < nSteps; n++) {
-      for (auto &r : ctx->c.runs) {
-        if (r.sp.no_linking) {
-          vls::launch_spline(r, r.f, s, t);
-          vls::launch_chains(r, s);
-          vls::launch_drift(r, s);
-          vls::time_mark(t, s, "drift");
-        }
-        vls::launch_moment(r, s);
-        vls::time_mark(t, s, "moment");
-        vls::launch_spline_kick(r, s, t);
-        vls::launch_spline(r, r.f, s, t);
-        vls::launch_chains(r, s);
-        vls::time_mark(t, s, "chains");
-        vls::launch_drift(r, s);
-        vls::time_mark(t, s, "drift");
-        r.stepped = true;
-      }
-    }
+    for (int n = 0; n < nSteps; n++)
+      for (auto &r : ctx->c.runs) step_once(r, s, t);
     VLS_HIP_CHECK(hipGetLastError());
   });
 }
@@ -537,6 +554,170 @@ int vls_set_timing(vls_ctx *ctx, int on) {
   return guarded([&] {
     if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
     ctx->c.timing.on = on != 0;
+  });
+}
+
+int vls_run_info(vls_ctx *ctx, int run, int64_t out[5]) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!out) throw vls::Error(VLS_ERR_ARG, "null output");
+    out[0] = r.nR;
+    out[1] = r.nU;
+    out[2] = r.nJ;
+    out[3] = r.nChains;
+    out[4] = r.nCells;
+  });
+}
+
+int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *vals) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    hipStream_t s = ctx->c.stream;
+    if (r.nChains && head) {
+      std::vector<int32_t> len(r.nChains);
+      VLS_HIP_CHECK(hipMemcpyAsync(len.data(), r.ch_len, r.nChains * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+      for (int64_t k = 0; k < r.nChains; k++)
+        if (head[k] < 0 || head[k] >= len[k]) throw vls::Error(VLS_ERR_ARG, "chain head out of range");
+      h2d(r.ch_head, head, r.nChains, s);
+    }
+    if (r.nCells && vals) h2d(r.ch_vals, vals, r.nCells, s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_mass_profile(vls_ctx *ctx, int run, double *M) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!M) throw vls::Error(VLS_ERR_ARG, "null output");
+    if (!r.have_f) throw vls::Error(VLS_ERR_STATE, "vls_mass_profile: f not set");
+    hipStream_t s = ctx->c.stream;
+    vls::launch_moment(r, s);
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipMemcpyAsync(M, r.Mx, r.nR * sizeof(double), hipMemcpyDeviceToHost, s));
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (nSteps < 1 || ckEvery < 1) throw vls::Error(VLS_ERR_ARG, "vls_record: need nSteps, ckEvery >= 1");
+    if (!r.have_f || !r.have_drift)
+      throw vls::Error(VLS_ERR_STATE, "vls_record: f and drift feet must be set first");
+    if (r.sp.no_linking)
+      throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
+    hipStream_t s = ctx->c.stream;
+    auto &A = r.adj;
+    const int nCk = (nSteps + ckEvery - 1) / ckEvery;
+    const int seg = std::min(ckEvery, nSteps);
+    const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+    const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+    if (!A.fb) {
+      A.fb = dalloc<double>(r, nv);
+      A.fsb = dalloc<double>(r, nv);
+      A.cb = dalloc<double>(r, coef);
+      A.gb = dalloc<double>(r, coef);
+      A.abp = dalloc<double>(r, (size_t)r.nJ * r.nR);
+      A.dMb = dalloc<double>(r, r.nR);
+      A.Mbar = dalloc<double>(r, r.nR);
+      A.endF = dalloc<double>(r, nv);
+      vls::configure_adjoint(ctx->c.maxLds, r.nR);
+    }
+    // the chain-sized arrays follow the current delay lines
+    const bool chainsChanged = A.cells != (int64_t)nc || A.chains != (int64_t)nch;
+    if (chainsChanged) {
+      A.ringb = dalloc<double>(r, nc);
+      A.endVals = dalloc<double>(r, nc);
+      A.endHead = dalloc<int32_t>(r, nch);
+      A.cells = nc;
+      A.chains = nch;
+    }
+    if (A.capCk < nCk || chainsChanged) {
+      A.ckF = dalloc<double>(r, nv * nCk);
+      A.ckVals = dalloc<double>(r, nc * nCk);
+      A.ckHead = dalloc<int32_t>(r, nch * nCk);
+      A.capCk = nCk;
+    }
+    if (A.capSeg < seg) {
+      A.seg = dalloc<double>(r, nv * seg);
+      A.capSeg = seg;
+    }
+    for (int n = 0; n < nSteps; n++) {
+      if (n % ckEvery == 0) {
+        const int k = n / ckEvery;
+        d2d(A.ckF + nv * k, r.f, nv, s);
+        d2d(A.ckVals + nc * k, r.ch_vals, nc, s);
+        d2d(A.ckHead + nch * k, r.ch_head, nch, s);
+      }
+      step_once(r, s, nullptr);
+    }
+    d2d(A.endF, r.f, nv, s);
+    d2d(A.endVals, r.ch_vals, nc, s);
+    d2d(A.endHead, r.ch_head, nch, s);
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    A.nSteps = nSteps;
+    A.ckEvery = ckEvery;
+    A.nCk = nCk;
+    A.recorded = true;
+  });
+}
+
+int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
+                double *grad_f0, double *grad_vals0) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    auto &A = r.adj;
+    if (!A.recorded) throw vls::Error(VLS_ERR_STATE, "vls_adjoint: no recorded run (vls_record)");
+    if (!grad_f0) throw vls::Error(VLS_ERR_ARG, "null gradient output");
+    hipStream_t s = ctx->c.stream;
+    const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+    const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+    // lambda_N: the loss's derivative in f_N, directly and through M(<r)
+    if (dL_df)
+      h2d(A.fb, dL_df, nv, s);
+    else
+      vls::launch_fill(A.fb, (int64_t)nv, 0.0, s);
+    if (dL_dM) {
+      h2d(A.Mbar, dL_dM, r.nR, s);
+      vls::launch_adj_mass(r, nullptr, A.Mbar, A.dMb, A.fb, true, s);
+    }
+    vls::launch_fill(A.ringb, (int64_t)nc, 0.0, s);
+    for (int sg = A.nCk - 1; sg >= 0; sg--) {
+      const int n0 = sg * A.ckEvery, n1 = std::min(n0 + A.ckEvery, A.nSteps);
+      // recompute the segment's states from its checkpoint
+      d2d(r.f, A.ckF + nv * sg, nv, s);
+      d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
+      d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
+      for (int n = n0; n < n1; n++) {
+        d2d(A.seg + nv * (n - n0), r.f, nv, s);
+        if (n + 1 < n1) step_once(r, s, nullptr);
+      }
+      // reverse through the segment
+      for (int n = n1 - 1; n >= n0; n--) {
+        d2d(r.f, A.seg + nv * (n - n0), nv, s);
+        vls::launch_moment(r, s);             // acceleration of f_n
+        vls::launch_spline(r, r.f, s, nullptr);  // kick coefficients of f_n
+        vls::launch_fill(A.cb, (int64_t)coef, 0.0, s);
+        vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
+        vls::launch_adj_spline(r, A.cb, A.fsb, s);
+        vls::launch_adj_kick(r, A.fsb, A.gb, A.cb, A.abp, s);
+        vls::launch_adj_spline(r, A.cb, A.fb, s);
+        vls::launch_adj_mass(r, A.abp, nullptr, A.dMb, A.fb, true, s);
+      }
+    }
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipMemcpyAsync(grad_f0, A.fb, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (grad_vals0 && nc)
+      VLS_HIP_CHECK(hipMemcpyAsync(grad_vals0, A.ringb, nc * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+    // back to the state after the recorded run
+    d2d(r.f, A.endF, nv, s);
+    d2d(r.ch_vals, A.endVals, nc, s);
+    d2d(r.ch_head, A.endHead, nch, s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

@@ -64,8 +64,9 @@ struct Run {
   double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
   double *gsc = nullptr;    // fused kick: per-wave scratch for wide kicks
   bool kickScratch = false; // fused kick: always take the scratch path
-  // static feet (xr < 0 marks points outside the grid or on delay lines)
+  // static feet (xr <= -1 marks points outside the grid or on delay lines)
   double *xr = nullptr, *xu = nullptr;
+  std::vector<uint8_t> hflag;  // host copy of the VLS_FOOT_* kinds
   // moment
   double *P = nullptr;      // [nJ][nR] per-slice partials
   double *dM = nullptr, *Mx = nullptr, *acc = nullptr, *mTot = nullptr;
@@ -84,12 +85,28 @@ struct Run {
   double *st_q = nullptr;     // [4][nJ]
   double *st_out = nullptr;   // [8]
 
+  // adjoint: recorded run and work arrays (allocated by vls_record)
+  struct Adjoint {
+    int nSteps = 0, ckEvery = 0, nCk = 0, capCk = 0, capSeg = 0;
+    int64_t cells = -1, chains = -1;  // delay-line sizes the chain arrays were made for
+    double *ckF = nullptr;      // [capCk][nVal] f at steps 0, ckEvery, ...
+    double *ckVals = nullptr;   // [capCk][nCells] delay-line values there
+    int32_t *ckHead = nullptr;  // [capCk][nChains] ring heads there
+    double *endF = nullptr, *endVals = nullptr;  // state after the recorded run
+    int32_t *endHead = nullptr;
+    double *seg = nullptr;      // [capSeg][nVal] the states of one segment
+    double *fb = nullptr, *fsb = nullptr, *cb = nullptr, *gb = nullptr;
+    double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
+    bool recorded = false;
+  } adj;
+
   bool have_f = false, have_drift = false, have_chains = false, stepped = false;
   std::vector<void *> allocs;
 };
 
 struct Context {
   int device = 0;
+  int maxLds = 0;  // LDS bytes per workgroup (opt-in limit)
   hipStream_t stream = nullptr;
   std::vector<Run> runs;
   Timing timing;
@@ -97,16 +114,28 @@ struct Context {
 
 // kernel launchers (vls_kernels.hip)
 void configure_tiles(Run &r, int maxLdsPerBlock);
-void launch_moment(Run &r, hipStream_t s);
+void launch_moment(Run &r, hipStream_t s);       // partials of f, then the scan
+void launch_moment_scan(Run &r, hipStream_t s);  // scan of the partials in P
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 void launch_kick(Run &r, hipStream_t s);
-void launch_spline_kick(Run &r, hipStream_t s, Timing *t);  // kick spline + kick
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t);  // moment, kick spline, kick
 size_t kick_scratch_doubles(const Run &r);
 void launch_drift(Run &r, hipStream_t s);
-void launch_chains(Run &r, hipStream_t s);
+void mark_chains(Run &r, hipStream_t s);  // attach the delay lines to their points
 void launch_stats(Run &r, hipStream_t s);
 
 void time_mark(Timing *t, hipStream_t s, const char *name);
+
+// adjoint launchers (vls_adjoint.hip)
+void configure_adjoint(int maxLdsPerBlock, int nR);
+void launch_fill(double *p, int64_t n, double v, hipStream_t s);
+void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
+                      int64_t stepOff, hipStream_t s);
+void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
+void launch_adj_kick(Run &r, const double *fsb, double *gb, double *cb, double *abp,
+                     hipStream_t s);
+void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
+                     bool accumulate, hipStream_t s);
 
 }  // namespace vls
 

@@ -36,50 +36,54 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
 
 // ---------------------------------------------------------------- moment
 // Per (j, r) partial of quadrature21::integrate1D (src/quadrature21.hxx:104-133):
-// sum over u < nU-1 of ((Ct1_j*dU_u)*dR_r) * avg4, serial in u.  The row
-// loads of the next kU iterations are issued before their additions so that
-// the memory latency overlaps the serial sum.
+// the terms ((Ct1_j*dU_u)*dR_r) * avg4 over u < nU-1, summed in a fixed tree
+// that the fused row solver below also follows: serial sums over groups of 16
+// rows (u = 64w + 16q + t), combined ((G0 + G1) + G2) + G3 into a wave of 64
+// rows, and the waves left to right.  One thread per (j, r); the row loads of
+// the next group are issued before its additions.
 __global__ void __launch_bounds__(kBlock)
 k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
                  const double *__restrict__ dU, const double *__restrict__ dR,
                  double *__restrict__ P, int nR, int nU, int nJ) {
-  constexpr int kU = 8;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (r >= nR) return;
-  double acc = 0.0;
+  double Pj = 0.0;
   if (r < nR - 1) {
     const double ct = Ct1[j];
     const double dr = dR[r];
     const double *p = f + (size_t)j * nU * nR + r;
     double a0 = p[0], a1 = p[1];
-    int u = 0;
-    for (; u + kU <= nU - 1; u += kU) {
-      double b0[kU], b1[kU];
+    const int nW = (nU + 63) / 64;
+    for (int w = 0; w < nW; w++) {
+      double G[4];
 #pragma unroll
-      for (int k = 0; k < kU; k++) {
-        const double *q = p + (size_t)(u + k + 1) * nR;
-        b0[k] = q[0];
-        b1[k] = q[1];
-      }
+      for (int q = 0; q < 4; q++) {
+        const int ub = 64 * w + 16 * q;
+        double b0[16], b1[16];
 #pragma unroll
-      for (int k = 0; k < kU; k++) {
-        double avg = (((a0 + a1) + b0[k]) + b1[k]) * 0.25;
-        acc += ((ct * dU[u + k]) * dr) * avg;
-        a0 = b0[k];
-        a1 = b1[k];
+        for (int t = 0; t < 16; t++) {
+          const int u = min(ub + t, nU - 2);
+          b0[t] = p[(size_t)(u + 1) * nR];
+          b1[t] = p[(size_t)(u + 1) * nR + 1];
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          if (ub + t < nU - 1) {
+            double avg = (((a0 + a1) + b0[t]) + b1[t]) * 0.25;
+            acc += ((ct * dU[ub + t]) * dr) * avg;
+            a0 = b0[t];
+            a1 = b1[t];
+          }
+        }
+        G[q] = acc;
       }
-    }
-    for (; u < nU - 1; u++) {
-      const double *q = p + (size_t)(u + 1) * nR;
-      double b0 = q[0], b1 = q[1];
-      double avg = (((a0 + a1) + b0) + b1) * 0.25;
-      acc += ((ct * dU[u]) * dr) * avg;
-      a0 = b0;
-      a1 = b1;
+      const double W = ((G[0] + G[1]) + G[2]) + G[3];
+      Pj = w == 0 ? W : Pj + W;
     }
   }
-  P[(size_t)j * nR + r] = acc;
+  P[(size_t)j * nR + r] = Pj;
 }
 
 // dM[r] = serial sum over j; exclusive radial prefix (gridHandler::accumulate
@@ -450,13 +454,13 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 constexpr int kKW = 62, kEB = 8;
 
 // Long columns keep their first kYL forward values in LDS instead of VGPRs
-// (they are the last ones the back substitution needs) and use a 32-row ring,
-// so that the column, the ring evaluation and the sweep fit the 512 VGPRs of
-// a wave.
+// (they are the last ones the back substitution needs) and use a 16-row ring,
+// so that the column leaves room in the 512 VGPRs of a wave for several
+// evaluations in flight, within the LDS share of four waves per CU.
 template <int Mv>
 struct KickLineCfg {
-  static constexpr int kRing = Mv > 160 ? 32 : 64;
-  static constexpr int kYL = Mv > 160 ? 32 : 0;
+  static constexpr int kRing = Mv > 160 ? 16 : 64;
+  static constexpr int kYL = Mv > 160 ? 56 : 0;
 };
 
 template <int Mv, int S>
@@ -699,53 +703,68 @@ __device__ __forceinline__ double eval_c(const double *d, int64_t st, double xr,
   return result;
 }
 
-// drift loop, src/vlasovSolver21_spline.hxx:788-796.  The foot table carries
-// the point's kind in xr: an evaluated foot has xr > -1, kFootOut (-1) is a
-// foot outside the grid (f = 0), kFootChain (-2) a delay-line point, written
-// by k_chains.  Blocks are spread so that each XCD (block id mod 8) walks a
-// contiguous range of points: the 4x4 stencils of neighbouring points share
-// coefficient rows, which then stay in that XCD's L2.
+// drift loop, src/vlasovSolver21_spline.hxx:788-796, with the delayed central
+// kernel.  The foot table carries the point's kind in xr: an evaluated foot
+// has xr > -1 (trunc(xr) >= 0), kFootOut (-1) is a foot outside the grid
+// (f = 0), kFootChain (-2) a point fed by a delay line whose index is in xu.
+// A delay-line point takes the spline value at its back cell (or 0 when that
+// cell's v is outside the grid; src/vlasovSolver21_spline.hxx:776-786) into
+// the ring slot behind the front, reads the front (sphericalKernel::readAt,
+// src/sphericalKernel.hxx:155-161) and turns the ring by one
+// (cellChain::roll, src/cellChain.hxx:49-56).  Blocks are spread so that each
+// XCD (block id mod 8) walks a contiguous range of points: the 4x4 stencils
+// of neighbouring points share coefficient rows, which then stay in that
+// XCD's L2.
 constexpr double kFootOut = -1.0, kFootChain = -2.0;
+
+struct ChainTables {
+  const int64_t *off;
+  const int32_t *len;
+  int32_t *head;
+  const double *bxr, *bxu;
+  const uint8_t *bout;
+  double *vals;
+  int64_t n;
+};
 
 __global__ void __launch_bounds__(kBlock)
 k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ xr,
         const double *__restrict__ xu, int64_t nVal, int nSlice, int64_t cs, int stride,
-        int perXcd) {
+        int perXcd, ChainTables ch) {
   const int b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7) * perXcd + (b >> 3);  // XCD-contiguous block
   const int64_t i = lb * kBlock + threadIdx.x;
   if (i >= nVal) return;
   const double x = xr[i];
-  if (x <= kFootOut) {  // an evaluated foot has trunc(xr) >= 0, so xr > -1
-    if (x == kFootOut) f[i] = 0;
-    return;
-  }
   const int64_t j = i / nSlice;
-  f[i] = eval_c(c + j * cs, stride, x, xu[i]);
+  const double *cj = c + j * cs;
+  if (x > kFootOut) {
+    f[i] = eval_c(cj, stride, x, xu[i]);
+  } else if (x == kFootOut) {
+    f[i] = 0;
+  } else {
+    const double kd = xu[i];
+    if (!(kd >= 0) || kd >= (double)ch.n) return;  // no delay line attached
+    const int64_t k = (int64_t)kd;
+    const int L = ch.len[k];
+    const int h = ch.head[k];
+    double *v = ch.vals + ch.off[k];
+    int back = h + L - 1;
+    if (back >= L) back -= L;
+    v[back] = ch.bout[k] ? 0.0 : eval_c(cj, stride, ch.bxr[k], ch.bxu[k]);
+    f[i] = v[h];
+    ch.head[k] = (h + 1 == L) ? 0 : h + 1;
+  }
 }
 
-// delayed central kernel: the back cell takes the spline value at its
-// position (src/vlasovSolver21_spline.hxx:776-786), the grid point reads the
-// front (sphericalKernel::readAt, src/sphericalKernel.hxx:155-161) and the
-// ring turns by one (cellChain::roll, src/cellChain.hxx:49-56).
+// Attach delay line k to its point: xr = kFootChain, xu = k.
 __global__ void __launch_bounds__(kBlock)
-k_chains(double *__restrict__ f, const double *__restrict__ c, const int64_t *__restrict__ point,
-         const int64_t *__restrict__ off, const int32_t *__restrict__ len,
-         int32_t *__restrict__ head, const double *__restrict__ bxr, const double *__restrict__ bxu,
-         const uint8_t *__restrict__ bout, double *__restrict__ vals, int64_t nChains, int nR,
-         int nU, int64_t cs) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nChains) return;
-  int64_t pt = point[k];
-  int j = (int)(pt / ((int64_t)nU * nR));
-  int L = len[k];
-  int h = head[k];
-  double *v = vals + off[k];
-  int back = h + L - 1;
-  if (back >= L) back -= L;
-  v[back] = bout[k] ? 0.0 : eval_c(c + (size_t)j * cs, nR + 2, bxr[k], bxu[k]);
-  f[pt] = v[h];
-  head[k] = (h + 1 == L) ? 0 : h + 1;
+k_mark_chains(double *__restrict__ xr, double *__restrict__ xu, const int64_t *__restrict__ point,
+              int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  xr[point[k]] = kFootChain;
+  xu[point[k]] = (double)k;
 }
 
 // ---------------------------------------------------------------- statistics
@@ -858,6 +877,10 @@ void launch_moment(Run &r, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
                      r.nU, r.nJ);
+  launch_moment_scan(r, s);
+}
+
+void launch_moment_scan(Run &r, hipStream_t s) {
   hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(256), r.nR * sizeof(double), s, r.P, r.gfac,
                      r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
 }
@@ -900,29 +923,45 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
 }
 
 // Row version of the register line solver with coalesced memory access.
-// A block of 64 lanes takes up to 64 consecutive rows u0.. of slice j (rows
-// of f, contiguous in memory).  Right-hand sides arrive in chunks of kC
-// elements per row through a small LDS transpose (loads of whole row
-// segments, then lane = row reads its kC values); the loads of chunk c+1 are
-// issued before the elimination of chunk c so that their latency hides
-// behind it.  Coefficients leave the same way, a chunk at a time as the back
-// substitution produces them.  Loads are clamped into the block's rows (the
-// extra lanes of a partial block solve a copy and store nothing).
-template <int M>
-__global__ void __launch_bounds__(64)
+// Wave w of block b takes up to 64 consecutive rows u0 = 64 (b*W + w) ... of
+// slice j (rows of f, contiguous in memory), W = blockDim.x / 64.
+// Right-hand sides arrive in chunks of kC elements per row through a small
+// LDS transpose (loads of whole row segments, then lane = row reads its kC
+// values); the loads of chunk c+1 are issued before the elimination of chunk
+// c so that their latency hides behind it.  Coefficients leave the same way,
+// a chunk at a time as the back substitution produces them.  Loads are
+// clamped into the wave's rows (the extra lanes of a partial wave solve a
+// copy and store nothing).
+//
+// MOM: the block holds a whole slice (W = ceil(nU/64) <= 4 waves) and also
+// forms the slice's moment partials P[j][r] from the same data, in the tree
+// order of k_moment_partial: each lane makes the terms of its row pair
+// (u, u+1) (row u+1 is the next lane's, or the next wave's first row), the
+// terms go through LDS, a thread per (wave, group, r) sums its 16 rows, and
+// one thread per r combines the groups and the waves.
+constexpr int kRowsMaxW = 4;
+
+template <int M, bool MOM>
+__global__ void __launch_bounds__(256)
 k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
-            int64_t cs, int lzmask, ThomasAxis ax) {
+            int64_t cs, int lzmask, ThomasAxis ax, const double *__restrict__ Ct1,
+            const double *__restrict__ dU, const double *__restrict__ dR, double *__restrict__ P) {
   constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
   constexpr int nIn = M - 2;           // right-hand sides per line (= nR)
   constexpr int nChIn = (nIn + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
-  __shared__ double chunk[64 * kP];
-  const int lane = threadIdx.x;
+  __shared__ double chunkAll[kRowsMaxW * 64 * kP];
+  __shared__ double termAll[MOM ? kRowsMaxW * 64 * kP : 1];
+  __shared__ double gsum[MOM ? kRowsMaxW * 4 * kC : 1];
+  const int W = blockDim.x >> 6;
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int lz = lane & lzmask;
   const int j = blockIdx.y;
-  const int u0 = blockIdx.x * 64;
-  const int rows = min(64, nU - u0);
+  const int u0 = (blockIdx.x * W + w) * 64;
+  const int rows = max(min(64, nU - u0), 1);
   const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC elements per pass
-  const double *src = f + ((size_t)j * nU + u0) * nIn;
+  double *chunk = chunkAll + w * 64 * kP;
+  const double *src = f + ((size_t)j * nU + min(u0, nU - 1)) * nIn;
   double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double y[M];
   double tmp[kC];
@@ -934,6 +973,12 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
       tmp[q] = src[row * nIn + e];
     }
   };
+  // moment: this lane's row pair, its weight, and the values carried from
+  // the previous chunk's last element
+  const int u = u0 + lane;
+  const bool pairOk = MOM && u < nU - 1;
+  const double ctdu = pairOk ? Ct1[j] * dU[u] : 0.0;
+  double aPrev = 0.0, bPrev = 0.0;
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
   load_chunk(0);
@@ -945,8 +990,40 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nIn) y[1 + ch * kC + i] = chunk[lane * kP + i];
+    if (MOM) {
+      // row u+1: the next lane's row, or the next wave's first row
+      const double *nxt = lane < 63 ? chunk + (lane + 1) * kP : chunkAll + (w + 1) * 64 * kP;
+      double *T = termAll + (w * 64 + lane) * kP;
+#pragma unroll
+      for (int i = 0; i < kC; i++) {
+        const int e = ch * kC + i;  // column of the new values; term r = e - 1
+        if (e < nIn) {
+          const double a1 = y[1 + e];
+          const double b1 = (lane < 63 || w + 1 < W) ? nxt[i] : 0.0;
+          double t = 0.0;
+          if (pairOk && e >= 1) {
+            const double avg = (((aPrev + a1) + bPrev) + b1) * 0.25;
+            t = (ctdu * dR[e - 1]) * avg;
+          }
+          T[i] = t;
+          aPrev = a1;
+          bPrev = b1;
+        } else {
+          T[i] = 0.0;
+        }
+      }
+    }
     __syncthreads();
     if (ch + 1 < nChIn) load_chunk(ch + 1);
+    if (MOM) {
+      // a thread per (group of 16 rows, r) of its wave: serial over the rows
+      const int i = lane & 15, q = lane >> 4;
+      const double *T = termAll + (w * 64 + 16 * q) * kP + i;
+      double g = 0.0;
+#pragma unroll
+      for (int t = 0; t < 16; t++) g += T[t * kP];
+      gsum[(w * 4 + q) * kC + i] = g;
+    }
 #pragma unroll
     for (int i = 0; i < kC; i++) {
       const int k = 1 + ch * kC + i;
@@ -960,7 +1037,22 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
         prev = y[k];
       }
     }
+    if (MOM) {
+      __syncthreads();
+      if (threadIdx.x < kC) {
+        const int i = threadIdx.x, r = ch * kC + i - 1;
+        double Pj = 0.0;
+        for (int ww = 0; ww < W; ww++) {
+          const double *g = gsum + ww * 4 * kC + i;
+          const double Wsum = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
+          Pj = ww == 0 ? Wsum : Pj + Wsum;
+        }
+        if (r >= 0 && r < nR - 1) P[(size_t)j * nR + r] = Pj;
+        if (r == nR - 1) P[(size_t)j * nR + r] = 0.0;
+      }
+    }
   }
+  if (MOM && threadIdx.x == 0 && nChIn * kC - 1 == nR - 1) P[(size_t)j * nR + nR - 1] = 0.0;
   double tt = 0.0 - ax.hi2 * y[M - 3];
   tt = tt - ax.p * y[M - 2];
   y[M - 1] = div_known(tt, ax.q, ax.rq);
@@ -990,14 +1082,16 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 #pragma unroll
     for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
     __syncthreads();
-    if (rows == 64 && (ch + 1) * kC <= M) {
+    if (u0 < nU) {
+      if (rows == 64 && (ch + 1) * kC <= M) {
 #pragma unroll
-      for (int q = 0; q < kC; q++) dst[(q * 4 + rsub) * M + k] = sv[q];
-    } else {
+        for (int q = 0; q < kC; q++) dst[(q * 4 + rsub) * M + k] = sv[q];
+      } else {
 #pragma unroll
-      for (int q = 0; q < kC; q++) {
-        const int row = q * 4 + rsub;
-        if (row < rows && k < M) dst[row * M + k] = sv[q];
+        for (int q = 0; q < kC; q++) {
+          const int row = q * 4 + rsub;
+          if (row < rows && k < M) dst[row * M + k] = sv[q];
+        }
       }
     }
   }
@@ -1006,15 +1100,19 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 // Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
 // standard grids.  Other shapes use the LDS-tiled kernels above.
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
+using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis,
+                        const double *, const double *, const double *, double *);
 using KickFn = void (*)(const double *, double *, const double *, const double *, double *, int,
                         int64_t, double, double, double, double, int, int, ThomasAxis);
 struct LineShape {
   int nR, nU;
-  LineFn rows, cols;
+  RowsFn rows, rowsMoment;
+  LineFn cols;
   KickFn kick;
 };
-#define VLS_LINE_SHAPE(R, U) \
-  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>, k_kick_line<(U) + 2, (R) + 2>}
+#define VLS_LINE_SHAPE(R, U)                                                          \
+  {R, U, k_line_rows<(R) + 2, false>, k_line_rows<(R) + 2, true>,                     \
+   k_line<(U) + 2, false, (R) + 2>, k_kick_line<(U) + 2, (R) + 2>}
 static const LineShape kLineShapes[] = {
     VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
     VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
@@ -1032,7 +1130,7 @@ static const LineShape *line_shape(const Run &r) {
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   if (const LineShape *ls = line_shape(r)) {
     hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
-                       r.nU, r.nJ, r.cs, 0, r.tr);
+                       r.nU, r.nJ, r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
     time_mark(t, s, "spline_r");
     hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
@@ -1064,15 +1162,29 @@ size_t kick_scratch_doubles(const Run &r) {
 void launch_spline_kick(Run &r, hipStream_t s, Timing *t) {
   if (fused_kick(r) && r.gsc) {
     const LineShape *ls = line_shape(r);
-    hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR,
-                       r.nU, r.nJ, r.cs, 0, r.tr);
-    time_mark(t, s, "spline_r");
+    const int nW = (r.nU + 63) / 64;
+    if (nW <= kRowsMaxW) {
+      // the row solves also form the moment partials of f
+      hipLaunchKernelGGL(ls->rowsMoment, dim3(1, r.nJ), dim3(64 * nW), 0, s, r.f, r.c, r.nR,
+                         r.nU, r.nJ, r.cs, 0, r.tr, r.Ct1, r.dU, r.dR, r.P);
+      time_mark(t, s, "spline_r_moment");
+      launch_moment_scan(r, s);
+      time_mark(t, s, "moment_scan");
+    } else {
+      launch_moment(r, s);
+      time_mark(t, s, "moment");
+      hipLaunchKernelGGL(ls->rows, dim3(nW, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR, r.nU, r.nJ,
+                         r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
+      time_mark(t, s, "spline_r");
+    }
     hipLaunchKernelGGL(ls->kick, dim3((r.nR + kKW - 1) / kKW, r.nJ), dim3(64), 0, s, r.c, r.f,
                        r.U_Val, r.acc, r.gsc, r.nR, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
                        r.kickScratch ? 1 : 0, 0, r.tv);
     time_mark(t, s, "kick_line");
     return;
   }
+  launch_moment(r, s);
+  time_mark(t, s, "moment");
   launch_spline(r, r.f, s, t);
   launch_kick(r, s);
   time_mark(t, s, "kick");
@@ -1087,15 +1199,15 @@ void launch_kick(Run &r, hipStream_t s) {
 void launch_drift(Run &r, hipStream_t s) {
   const int64_t nb = (r.nVal + kBlock - 1) / kBlock;
   const int perXcd = (int)((nb + 7) / 8);
+  ChainTables ch{r.ch_off, r.ch_len, r.ch_head, r.ch_xr, r.ch_xu, r.ch_out, r.ch_vals, r.nChains};
   hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.nVal,
-                     r.nU * r.nR, r.cs, r.nR + 2, perXcd);
+                     r.nU * r.nR, r.cs, r.nR + 2, perXcd, ch);
 }
 
-void launch_chains(Run &r, hipStream_t s) {
+void mark_chains(Run &r, hipStream_t s) {
   if (r.nChains == 0) return;
-  hipLaunchKernelGGL(k_chains, dim3(blocks_for(r.nChains)), dim3(kBlock), 0, s, r.f, r.c,
-                     r.ch_point, r.ch_off, r.ch_len, r.ch_head, r.ch_xr, r.ch_xu, r.ch_out,
-                     r.ch_vals, r.nChains, r.nR, r.nU, r.cs);
+  hipLaunchKernelGGL(k_mark_chains, dim3(blocks_for(r.nChains)), dim3(kBlock), 0, s, r.xr, r.xu,
+                     r.ch_point, r.nChains);
 }
 
 void launch_stats(Run &r, hipStream_t s) {
