@@ -105,6 +105,49 @@ def adjoint_ratio(V):
             "loss": "sum_r (M(<r) - M_target)^2 of the final state"}
 
 
+def ensemble_cfgs(n=32, seed=2024):
+    """32 UDF runs: alpha ~ U[0.1, 1], R0 ~ U[1.5, 3], each axis drawn from
+    {64, 96, 128, 192, 256} cells (j at most 128)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = [64, 96, 128, 192, 256]
+    out = []
+    for _ in range(n):
+        pr, pu = int(rng.choice(sizes)), int(rng.choice(sizes))
+        pj = int(rng.choice([s for s in sizes if s <= 128]))
+        out.append(dict(WORKLOAD, **{"init.alpha": float(rng.uniform(0.1, 1.0)),
+                                     "init.R0": float(rng.uniform(1.5, 3.0)),
+                                     "grid.Pres": pr, "grid.Ures": [pu, pj]}))
+    return out
+
+
+ENS_STEPS = 500
+
+
+def ensemble_speedup(V):
+    """Throughput of the 32-run uneven ensemble stepped as one graph launch a
+    step (runs concurrent) over the same runs stepped one after the other."""
+    cfgs = ensemble_cfgs()
+    e = V.Ensemble(cfgs)
+    pts = sum(e.context.info(k)["nR"] * e.context.info(k)["nU"] * e.context.info(k)["nJ"]
+              for k in range(e.n))
+    res = {}
+    for mode in (False, True):
+        e.set_graph(mode)
+        e.step(5)
+        e.context.synchronize()
+        t0 = time.perf_counter()
+        e.step(ENS_STEPS)
+        e.context.synchronize()
+        res[mode] = time.perf_counter() - t0
+    return {"workload": "32 UDF runs, alpha~U[0.1,1], R0~U[1.5,3], axes from {64..256} cells "
+                        "(j <= 128), seed 2024", "steps": ENS_STEPS, "points": pts,
+            "sequential_s": round(res[False], 4), "graph_s": round(res[True], 4),
+            "point_updates_per_s": pts * ENS_STEPS / res[True],
+            "speedup": round(res[False] / res[True], 3)}
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -172,6 +215,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default 150, ~10-20 s of CPU work)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-run ensemble")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -221,6 +265,13 @@ def main():
         except Exception as e:  # reported, not fatal for the forward figure
             adj = {"ratio": None, "error": str(e)[:200]}
 
+    ens = None
+    if not args.no_ensemble:
+        try:
+            ens = ensemble_speedup(V)
+        except Exception as e:  # reported, not fatal for the forward figure
+            ens = {"speedup": None, "error": str(e)[:200]}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -239,6 +290,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 

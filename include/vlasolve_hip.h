@@ -116,6 +116,13 @@ int vls_get_chains(vls_ctx *ctx, int run, int32_t *head, double *vals);
  * stream.  Does not wait for completion. */
 int vls_step(vls_ctx *ctx, int nSteps);
 int vls_synchronize(vls_ctx *ctx);
+/* Ensembles (nRuns > 1): by default vls_step captures one step of all runs
+ * as a HIP graph, each run on a stream of its own so that runs of different
+ * sizes execute concurrently, and replays it once per step (the graph is
+ * rebuilt after vls_set_chains).  on = 0 steps the runs one after the other
+ * on the context stream.  No counterpart in the reference, which runs one
+ * model per process. */
+int vls_set_graph(vls_ctx *ctx, int on);
 
 /* Moments used by the last step's kick: dM[nR] (quadrature21::integrate1D,
  * src/quadrature21.hxx:88-142), accel[nR] = M(<r) G/r^2 dt
@@ -201,6 +208,17 @@ int vlsd_write_restart(vlsd_solver *s, const char *fname_no_ext);   /* .vls */
 int vlsd_snapshot(vlsd_solver *s);                  /* snapshot_%4.4f.ND */
 vls_ctx *vlsd_context(vlsd_solver *s);
 const char *vlsd_last_error(void);
+
+/* An ensemble: nRuns independent runs, each from its own reference-style
+ * argument list (argc[k], argv[k]; initial conditions, not restarts), set up
+ * on the host like vlsd_create and loaded into one device context whose runs
+ * may have different grids.  The context (vlsd_ensemble_context) is stepped
+ * with vls_step; it belongs to the ensemble. */
+typedef struct vlsd_ensemble vlsd_ensemble;
+int vlsd_ensemble_create(int nRuns, const int *argc, const char *const *const *argv,
+                         vlsd_ensemble **out);
+void vlsd_ensemble_destroy(vlsd_ensemble *e);
+vls_ctx *vlsd_ensemble_context(vlsd_ensemble *e);
 
 /* Host set-up only (no device): the same initial conditions, renormalisation,
  * static feet and delay lines as vlsd_create, for inspection and for

@@ -29,7 +29,8 @@ EXPORTS = (
     "vlsd_last_error", "vls_thomas_table", "vls_check_division", "vls_device_info",
     "vls_copy_probe", "vlsd_host_create", "vlsd_host_destroy", "vlsd_host_info", "vlsd_host_f",
     "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
-    "vls_record", "vls_adjoint",
+    "vls_record", "vls_adjoint", "vls_set_graph", "vlsd_ensemble_create", "vlsd_ensemble_destroy",
+    "vlsd_ensemble_context",
 )
 
 
@@ -99,6 +100,12 @@ def lib():
         L.vls_mass_profile.argtypes = [vp, C.c_int, P(d)]
         L.vls_record.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.vls_adjoint.argtypes = [vp, C.c_int, P(d), P(d), P(d), P(d)]
+        L.vls_set_graph.argtypes = [vp, C.c_int]
+        L.vlsd_ensemble_create.argtypes = [C.c_int, P(C.c_int), P(P(C.c_char_p)), P(vp)]
+        L.vlsd_ensemble_destroy.argtypes = [vp]
+        L.vlsd_ensemble_destroy.restype = None
+        L.vlsd_ensemble_context.argtypes = [vp]
+        L.vlsd_ensemble_context.restype = vp
         _lib = L
     return _lib
 
@@ -314,6 +321,49 @@ class Context:
         n = C.c_int(cap)
         _check(lib().vls_kernel_times(self._h, names, _ptr(ms), C.byref(n)))
         return [(names[k].decode(), float(ms[k])) for k in range(min(n.value, cap))]
+
+
+class Ensemble:
+    """Several independent runs (grids may differ) in one device context.
+
+    ``Ensemble([cfg0, cfg1, ...]).context`` is a Context whose run k is cfg k;
+    ``step`` advances all runs (concurrently, through a HIP graph)."""
+
+    def __init__(self, cfgs, verbose: int = 0):
+        argvs = []
+        for cfg in cfgs:
+            a = ["vlasolve_hip"] + to_argv(cfg) + ["-verbose", str(verbose)]
+            argvs.append((C.c_char_p * len(a))(*[x.encode() for x in a]))
+        n = len(argvs)
+        argc = (C.c_int * n)(*[len(a) for a in argvs])
+        argv = (C.POINTER(C.c_char_p) * n)(*[C.cast(a, C.POINTER(C.c_char_p)) for a in argvs])
+        h = C.c_void_p()
+        _check(lib().vlsd_ensemble_create(n, argc, argv, C.byref(h)), driver=True)
+        self._h = h
+        self.n = n
+        self.context = Context(lib().vlsd_ensemble_context(self._h), owner=self)
+
+    def step(self, n=1):
+        self.context.step(n)
+
+    def f(self, run):
+        inf = self.context.info(run)
+        return self.context.download_f(run, (inf["nJ"], inf["nU"], inf["nR"]))
+
+    def set_graph(self, on=True):
+        _check(lib().vls_set_graph(self.context._h, int(bool(on))))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.context._h = None
+            lib().vlsd_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HostSetup:

@@ -1,7 +1,9 @@
 // C ABI of the solver surface (include/vlasolve_hip.h, layer 2).
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "vls_host.hpp"
 
@@ -125,7 +127,50 @@ struct vlsd_host {
   vlh::Solver21 solver;
 };
 
+struct vlsd_ensemble {
+  std::vector<std::unique_ptr<vlh::Solver21>> runs;
+  vls_ctx *ctx = nullptr;
+  ~vlsd_ensemble() {
+    if (ctx) vls_destroy(ctx);
+  }
+};
+
 extern "C" {
+
+int vlsd_ensemble_create(int nRuns, const int *argc, const char *const *const *argv,
+                         vlsd_ensemble **out) {
+  if (!out) return VLS_ERR_ARG;
+  *out = nullptr;
+  vlsd_ensemble *e = new vlsd_ensemble();
+  int rc = guarded([&] {
+    if (nRuns < 1 || !argc || !argv) throw vlh::Failure("vlsd_ensemble_create: need runs");
+    std::vector<vls_geometry> geo(nRuns);
+    std::vector<vls_solver_params> sp(nRuns);
+    for (int k = 0; k < nRuns; k++) {
+      vlh::Params prm(argc[k], argv[k]);
+      vlh::verbose = prm.get<int>("verbose", "default", 0);
+      if (!prm.get<std::string>("reload", "default", "").empty())
+        throw vlh::Failure("vlsd_ensemble_create: runs start from initial conditions");
+      e->runs.emplace_back(new vlh::Solver21());
+      e->runs.back()->init(prm, "", false);
+      geo[k] = e->runs.back()->grid().geometry();
+      sp[k] = e->runs.back()->deviceParams();
+    }
+    int rc2 = vls_create(geo.data(), sp.data(), nRuns, &e->ctx);
+    if (rc2) throw vlh::Failure(vls_last_error(), rc2);
+    for (int k = 0; k < nRuns; k++) e->runs[k]->uploadTo(e->ctx, k);
+  });
+  if (rc) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return VLS_OK;
+}
+
+void vlsd_ensemble_destroy(vlsd_ensemble *e) { delete e; }
+
+vls_ctx *vlsd_ensemble_context(vlsd_ensemble *e) { return e ? e->ctx : nullptr; }
 
 int vlsd_host_create(int argc, const char *const *argv, vlsd_host **out) {
   if (!out) return VLS_ERR_ARG;

@@ -323,20 +323,29 @@ void Solver21::renormalize(double norm) {
   for (auto &v : chains_.val) v *= fac;
 }
 
-void Solver21::upload() {
-  const Grid21 &g = grid_;
-  if (ctx_) {
-    vls_destroy(ctx_);
-    ctx_ = nullptr;
-  }
-  vls_geometry geo = g.geometry();
+vls_solver_params Solver21::deviceParams() const {
   vls_solver_params sp{};
   sp.G = sp_.G;
   sp.dt = sp_.dt;
   sp.no_linking = noLinking_ ? 1 : 0;
   sp.kernel_type = kernelType_;
+  return sp;
+}
+
+void Solver21::upload() {
+  if (ctx_) {
+    vls_destroy(ctx_);
+    ctx_ = nullptr;
+  }
+  vls_geometry geo = grid_.geometry();
+  vls_solver_params sp = deviceParams();
   check(vls_create(&geo, &sp, 1, &ctx_));
-  check(vls_upload_f(ctx_, 0, f_.data()));
+  uploadTo(ctx_, 0);
+}
+
+void Solver21::uploadTo(vls_ctx *ctx, int run) const {
+  const Grid21 &g = grid_;
+  check(vls_upload_f(ctx, run, f_.data()));
   std::vector<double> xr(g.nVal, 0.0), xu(g.nVal, 0.0);
   std::vector<uint8_t> flag(g.nVal);
 #pragma omp parallel for schedule(static)
@@ -349,7 +358,7 @@ void Solver21::upload() {
       xu[i] = g.splineCoord(driftU_[i], 1);
     }
   }
-  check(vls_set_drift(ctx_, 0, xr.data(), xu.data(), flag.data()));
+  check(vls_set_drift(ctx, run, xr.data(), xu.data(), flag.data()));
   const int64_t nc = (int64_t)chains_.point.size();
   std::vector<double> bxr(nc), bxu(nc), ringVals(chains_.val.size());
   std::vector<uint8_t> bout(nc);
@@ -363,7 +372,7 @@ void Solver21::upload() {
     for (int32_t k = 0; k < chains_.len[c]; k++)
       ringVals[chains_.off[c] + (chains_.head[c] + k) % chains_.len[c]] = chains_.val[chains_.off[c] + k];
   }
-  check(vls_set_chains(ctx_, 0, nc, chains_.point.data(), chains_.len.data(), chains_.head.data(),
+  check(vls_set_chains(ctx, run, nc, chains_.point.data(), chains_.len.data(), chains_.head.data(),
                        bxr.data(), bxu.data(), bout.data(), ringVals.data()));
 }
 

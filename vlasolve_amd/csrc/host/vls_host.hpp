@@ -158,6 +158,10 @@ class Solver21 {
   double time() const { return time_; }
   long stepCount() const { return stepCount_; }
   vls_ctx *context() { return ctx_; }
+  // device parameters of this run, and loading the host set-up (f, feet,
+  // delay lines) into run `run` of a context whose geometry is grid().geometry()
+  vls_solver_params deviceParams() const;
+  void uploadTo(vls_ctx *ctx, int run) const;
   const SolverParams &solverParams() const { return sp_; }
 
  private:

@@ -79,6 +79,51 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
   vls::time_mark(t, s, "drift");
   r.stepped = true;
 }
+
+// Capture one step of every run of an ensemble: each run's kernels go to a
+// stream of their own, forked from and joined back into the context stream,
+// so that small runs fill the GPU together; replayed once per step.
+void build_ensemble_graph(vls::Context &C) {
+  const int n = (int)C.runs.size();
+  if ((int)C.runStreams.size() < n) {
+    for (int k = (int)C.runStreams.size(); k < n; k++) {
+      hipStream_t st;
+      VLS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      C.runStreams.push_back(st);
+      hipEvent_t ev;
+      VLS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      C.joinEv.push_back(ev);
+    }
+  }
+  if (!C.forkEv) VLS_HIP_CHECK(hipEventCreateWithFlags(&C.forkEv, hipEventDisableTiming));
+  hipGraph_t g = nullptr;
+  VLS_HIP_CHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    VLS_HIP_CHECK(hipEventRecord(C.forkEv, C.stream));
+    for (int k = 0; k < n; k++) {
+      VLS_HIP_CHECK(hipStreamWaitEvent(C.runStreams[k], C.forkEv, 0));
+      step_once(C.runs[k], C.runStreams[k], nullptr);
+      VLS_HIP_CHECK(hipEventRecord(C.joinEv[k], C.runStreams[k]));
+      VLS_HIP_CHECK(hipStreamWaitEvent(C.stream, C.joinEv[k], 0));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(C.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  VLS_HIP_CHECK(hipStreamEndCapture(C.stream, &g));
+  hipError_t e = hipGraphInstantiate(&C.graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    C.graph = nullptr;
+    throw vls::Error(VLS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+}
+
+void drop_graph(vls::Context &C) {
+  if (C.graph) (void)hipGraphExecDestroy(C.graph);
+  C.graph = nullptr;
+}
 }  // namespace
 
 struct vls_ctx {
@@ -365,11 +410,24 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
 void vls_destroy(vls_ctx *ctx) {
   if (!ctx) return;
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  drop_graph(ctx->c);
+  for (auto st : ctx->c.runStreams) (void)hipStreamDestroy(st);
+  for (auto e : ctx->c.joinEv) (void)hipEventDestroy(e);
+  if (ctx->c.forkEv) (void)hipEventDestroy(ctx->c.forkEv);
   for (auto &r : ctx->c.runs)
     for (void *p : r.allocs) (void)hipFree(p);
   for (auto e : ctx->c.timing.ev) (void)hipEventDestroy(e);
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
+}
+
+int vls_set_graph(vls_ctx *ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    drop_graph(ctx->c);
+    ctx->c.graphOn = on != 0;
+  });
 }
 
 int vls_upload_f(vls_ctx *ctx, int run, const double *f) {
@@ -466,6 +524,7 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     h2d(r.ch_xu, back_xu, nChains, s);
     h2d(r.ch_out, back_out, nChains, s);
     h2d(r.ch_vals, vals, cells, s);
+    drop_graph(ctx->c);  // the delay-line tables moved
     if (r.have_drift) vls::mark_chains(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
@@ -501,8 +560,16 @@ int vls_step(vls_ctx *ctx, int nSteps) {
     for (auto &r : ctx->c.runs)
       if (!r.have_f || !r.have_drift)
         throw vls::Error(VLS_ERR_STATE, "vls_step: f and drift feet must be set first");
-    for (int n = 0; n < nSteps; n++)
-      for (auto &r : ctx->c.runs) step_once(r, s, t);
+    vls::Context &C = ctx->c;
+    if (C.runs.size() > 1 && C.graphOn && !t->on) {
+      // an ensemble: the runs advance concurrently, one graph launch a step
+      if (!C.graph) build_ensemble_graph(C);
+      for (int n = 0; n < nSteps; n++) VLS_HIP_CHECK(hipGraphLaunch(C.graph, s));
+      for (auto &r : C.runs) r.stepped = true;
+    } else {
+      for (int n = 0; n < nSteps; n++)
+        for (auto &r : C.runs) step_once(r, s, t);
+    }
     VLS_HIP_CHECK(hipGetLastError());
   });
 }

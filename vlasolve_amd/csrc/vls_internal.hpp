@@ -110,6 +110,13 @@ struct Context {
   hipStream_t stream = nullptr;
   std::vector<Run> runs;
   Timing timing;
+  // ensembles: one stream per run, the step of all runs captured as a graph
+  // (fork from `stream`, each run on its own stream, join back)
+  std::vector<hipStream_t> runStreams;
+  hipEvent_t forkEv = nullptr;
+  std::vector<hipEvent_t> joinEv;
+  hipGraphExec_t graph = nullptr;
+  bool graphOn = true;
 };
 
 // kernel launchers (vls_kernels.hip)
