@@ -132,8 +132,10 @@ def test_autograd_harness():
     ctx.record(n, 2)
     M2 = ctx.mass_profile()
     g0, gv = ctx.adjoint(dL_dM=2 * (M2 - Mt.numpy()))
-    assert np.allclose(f0_t.grad.numpy(), g0, rtol=1e-12, atol=1e-300)
-    assert np.allclose(v0_t.grad.numpy(), gv, rtol=1e-12, atol=1e-300)
+    # the drift's transpose sums with atomics, so two calls agree to rounding
+    # relative to the gradient's scale, not bit for bit
+    assert np.abs(f0_t.grad.numpy() - g0).max() <= 1e-12 * np.abs(g0).max()
+    assert np.abs(v0_t.grad.numpy() - gv).max() <= 1e-12 * np.abs(gv).max()
 
 
 def test_gradient_wrt_pldf_eta():

@@ -768,7 +768,7 @@ k_mark_chains(double *__restrict__ xr, double *__restrict__ xu, const int64_t *_
 }
 
 // ---------------------------------------------------------------- statistics
-// Per (j, r) partials, serial in u, of the moment and of the integrands of
+// Per (j, r) partials of the moment (in its tree order) and, serial in u, of the integrands of
 // quadrature21::integrate (src/quadrature21.hxx:57-85; integrands
 // src/integrand21.hxx:82-145).
 __global__ void __launch_bounds__(kBlock)
@@ -788,20 +788,34 @@ k_stats_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
     const double J = J_Val[j];
     const double *p = f + (int64_t)j * nU * nR + r;
     double a0 = p[0], a1 = p[1];
-    for (int u = 0; u < nU - 1; u++) {
-      const double *q = p + (int64_t)(u + 1) * nR;
-      double b0 = q[0], b1 = q[1];
-      double v = (((a0 + a1) + b0) + b1) * 0.25;
-      double du = dU[u];
-      double w = (ct * du) * dr;
-      double uc = U_Val[u] + 0.5 * du;
-      m += w * v;
-      ek += w * (0.5 * v * (uc * uc + J * J / (rc * rc)));
-      s += w * ((v <= 0) ? 0.0 : -v * log(v));
-      l1 += w * fabs(v);
-      l2 += w * (v * v);
-      a0 = b0;
-      a1 = b1;
+    // the mass in the moment's tree order (groups of 16 rows, waves of 64),
+    // the other integrands serial in u
+    const int nW = (nU + 63) / 64;
+    for (int wv = 0; wv < nW; wv++) {
+      double G[4];
+      for (int q4 = 0; q4 < 4; q4++) {
+        double g = 0.0;
+        for (int t = 0; t < 16; t++) {
+          const int u = 64 * wv + 16 * q4 + t;
+          if (u >= nU - 1) break;
+          const double *q = p + (int64_t)(u + 1) * nR;
+          double b0 = q[0], b1 = q[1];
+          double v = (((a0 + a1) + b0) + b1) * 0.25;
+          double du = dU[u];
+          double w = (ct * du) * dr;
+          double uc = U_Val[u] + 0.5 * du;
+          g += w * v;
+          ek += w * (0.5 * v * (uc * uc + J * J / (rc * rc)));
+          s += w * ((v <= 0) ? 0.0 : -v * log(v));
+          l1 += w * fabs(v);
+          l2 += w * (v * v);
+          a0 = b0;
+          a1 = b1;
+        }
+        G[q4] = g;
+      }
+      const double Wm = ((G[0] + G[1]) + G[2]) + G[3];
+      m = wv == 0 ? Wm : m + Wm;
     }
   }
   const int64_t o = (int64_t)j * nR + r, n = (int64_t)nJ * nR;
