@@ -956,14 +956,14 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
 constexpr int kRowsMaxW = 4;
 
 template <int M, bool MOM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(MOM ? 256 : 64)
 k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
             int64_t cs, int lzmask, ThomasAxis ax, const double *__restrict__ Ct1,
             const double *__restrict__ dU, const double *__restrict__ dR, double *__restrict__ P) {
   constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
   constexpr int nIn = M - 2;           // right-hand sides per line (= nR)
   constexpr int nChIn = (nIn + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
-  __shared__ double chunkAll[kRowsMaxW * 64 * kP];
+  __shared__ double chunkAll[(MOM ? kRowsMaxW : 1) * 64 * kP];
   __shared__ double termAll[MOM ? kRowsMaxW * 64 * kP : 1];
   __shared__ double gsum[MOM ? kRowsMaxW * 4 * kC : 1];
   const int W = blockDim.x >> 6;
