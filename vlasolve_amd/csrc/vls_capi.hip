@@ -80,6 +80,22 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
   r.stepped = true;
 }
 
+// One step that also keeps what the adjoint of the step needs: the kick's
+// acceleration (accOut, [nR]) and spline coefficients (cK, a padded
+// coefficient array).  The kick runs unfused (k_spline / k_kick, the same
+// values as the fused kernel, bit for bit).
+void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut) {
+  vls::launch_moment(r, s);
+  d2d(accOut, r.acc, r.nR, s);
+  double *keep = r.c;
+  r.c = cK;
+  vls::launch_spline(r, r.f, s, nullptr);
+  vls::launch_kick(r, s);
+  r.c = keep;
+  vls::launch_spline(r, r.f, s, nullptr);
+  vls::launch_drift(r, s);
+}
+
 // Capture one step of every run of an ensemble: each run's kernels go to a
 // stream of their own, forked from and joined back into the context stream,
 // so that small runs fill the GPU together; replayed once per step.
@@ -344,6 +360,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.cs = (int64_t)(r.nR + 2) * (r.nU + 2);
       r.start_v = G.spline_start[1];
       r.hinv_v = G.spline_hinv[1];
+      r.hU.assign(G.U_Val, G.U_Val + r.nU);
       r.Ulo = G.U_Val[0];
       r.Uhi = G.U_Val[r.nU - 1];
       vls::configure_tiles(r, maxLds);
@@ -474,6 +491,10 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
     hipStream_t s = ctx->c.stream;
     h2d(r.xr, xf.data(), r.nVal, s);
     h2d(r.xu, uf.data(), r.nVal, s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    r.hxr.swap(xf);
+    r.hxu.swap(uf);
+    r.adj.csrValid = false;  // the adjoint's transposed foot table follows the feet
     vls::mark_chains(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
@@ -675,6 +696,11 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       throw vls::Error(VLS_ERR_STATE, "vls_record: f and drift feet must be set first");
     if (r.sp.no_linking)
       throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
+    // the kick's transpose finds the points of a coefficient row from
+    // n ~ u + a hinv, which needs v nodes on the uniform spline knots
+    for (int u = 0; u < r.nU; u++)
+      if (std::fabs((r.hU[u] - r.start_v) * r.hinv_v - u) > 0.25)
+        throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint needs a uniform v grid");
     hipStream_t s = ctx->c.stream;
     auto &A = r.adj;
     const int nCk = (nSteps + ckEvery - 1) / ckEvery;
@@ -708,8 +734,20 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.capCk = nCk;
     }
     if (A.capSeg < seg) {
-      A.seg = dalloc<double>(r, nv * seg);
+      A.segC = dalloc<double>(r, coef * seg);
+      A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
       A.capSeg = seg;
+    }
+    if (!A.csrValid) {
+      std::vector<int64_t> off;
+      std::vector<uint32_t> ent;
+      vls::build_drift_transpose(r, r.hxr, r.hxu, off, ent);
+      A.csrOff = dalloc<int64_t>(r, off.size());
+      A.csrEnt = dalloc<uint32_t>(r, ent.size());
+      h2d(A.csrOff, off.data(), off.size(), s);
+      h2d(A.csrEnt, ent.data(), ent.size(), s);
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+      A.csrValid = true;
     }
     for (int n = 0; n < nSteps; n++) {
       if (n % ckEvery == 0) {
@@ -754,23 +792,19 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
     vls::launch_fill(A.ringb, (int64_t)nc, 0.0, s);
     for (int sg = A.nCk - 1; sg >= 0; sg--) {
       const int n0 = sg * A.ckEvery, n1 = std::min(n0 + A.ckEvery, A.nSteps);
-      // recompute the segment's states from its checkpoint
+      // recompute the segment from its checkpoint, keeping each step's kick
+      // coefficients and acceleration
       d2d(r.f, A.ckF + nv * sg, nv, s);
       d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
       d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
-      for (int n = n0; n < n1; n++) {
-        d2d(A.seg + nv * (n - n0), r.f, nv, s);
-        if (n + 1 < n1) step_once(r, s, nullptr);
-      }
+      for (int n = n0; n < n1; n++)
+        step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0));
       // reverse through the segment
       for (int n = n1 - 1; n >= n0; n--) {
-        d2d(r.f, A.seg + nv * (n - n0), nv, s);
-        vls::launch_moment(r, s);             // acceleration of f_n
-        vls::launch_spline(r, r.f, s, nullptr);  // kick coefficients of f_n
-        vls::launch_fill(A.cb, (int64_t)coef, 0.0, s);
+        const double *cK = A.segC + coef * (n - n0), *accK = A.segAcc + (size_t)r.nR * (n - n0);
         vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
         vls::launch_adj_spline(r, A.cb, A.fsb, s);
-        vls::launch_adj_kick(r, A.fsb, A.gb, A.cb, A.abp, s);
+        vls::launch_adj_kick(r, A.fsb, cK, accK, A.gb, A.cb, A.abp, s);
         vls::launch_adj_spline(r, A.cb, A.fb, s);
         vls::launch_adj_mass(r, A.abp, nullptr, A.dMb, A.fb, true, s);
       }

@@ -94,11 +94,17 @@ struct Run {
     int32_t *ckHead = nullptr;  // [capCk][nChains] ring heads there
     double *endF = nullptr, *endVals = nullptr;  // state after the recorded run
     int32_t *endHead = nullptr;
-    double *seg = nullptr;      // [capSeg][nVal] the states of one segment
+    double *segC = nullptr;     // [capSeg][coef] the kick coefficients of each state
+    double *segAcc = nullptr;   // [capSeg][nR] the kick accelerations
     double *fb = nullptr, *fsb = nullptr, *cb = nullptr, *gb = nullptr;
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
+    int64_t *csrOff = nullptr;  // transposed foot table of the drift
+    uint32_t *csrEnt = nullptr;
+    bool csrValid = false;
     bool recorded = false;
   } adj;
+  std::vector<double> hxr, hxu;  // host copies of the folded foot table
+  std::vector<double> hU;        // host copy of U_Val
 
   bool have_f = false, have_drift = false, have_chains = false, stepped = false;
   std::vector<void *> allocs;
@@ -139,8 +145,10 @@ void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s);
 void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
-void launch_adj_kick(Run &r, const double *fsb, double *gb, double *cb, double *abp,
-                     hipStream_t s);
+void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *term,
+                     double *cb, double *abp, hipStream_t s);
+void build_drift_transpose(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
+                           std::vector<int64_t> &off, std::vector<uint32_t> &ent);
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
                      bool accumulate, hipStream_t s);
 
