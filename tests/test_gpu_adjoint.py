@@ -66,6 +66,40 @@ def test_adjoint_matches_tangent_linear(n, ck):
     assert np.array_equal(ctx.download_f(0, f0.shape), _after(cfg, n))
 
 
+@pytest.mark.parametrize("pres,ures", [(15, [15, 6]), (20, [18, 5])])
+def test_adjoint_generic_lines(pres, ures):
+    """Shapes without register line solvers: the streaming line adjoints.
+    With 18 padded nodes per line the last row's two coefficients fall on
+    both sides of a 16-element chunk boundary."""
+    from tests.configs import UDF_C1
+
+    cfg = dict(UDF_C1, **{"grid.Pres": pres, "grid.Ures": ures})
+    n, ck = 5, 2
+    print(f"case udf {pres}x{ures} adjoint n={n} ck={ck}", flush=True)
+    solver = V.Solver(cfg)
+    ctx = solver.context
+    f0 = solver.f()
+    head0, slots0 = ctx.get_chains()
+    off, ln = chain_layout(cfg)
+    ctx.record(n, ck)
+    M = ctx.mass_profile()
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(M.shape)
+    g0, gv = ctx.adjoint(dL_dM=w)
+    gv_front = ring_to_front(gv, head0, off, ln)
+    orc = O.Oracle(cfg, nthreads=0)
+    df = rng.standard_normal(f0.shape) * np.abs(f0).max()
+    dv = rng.standard_normal(gv.shape) * (np.abs(slots0).max() if slots0.size else 1.0)
+    orc.tl_init(df, dv)
+    orc.tl_step(n)
+    Mo, dMo = orc.tl_mass()
+    assert np.allclose(Mo, M, rtol=1e-12, atol=1e-15)
+    lhs = float(w @ dMo)
+    rhs = float((g0 * df).sum() + (gv_front * dv).sum())
+    print(f"<w, J v> = {lhs:.15e}  <J^T w, v> = {rhs:.15e}")
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
 def _after(cfg, n):
     o = O.Oracle(cfg, nthreads=0)
     o.step(n)

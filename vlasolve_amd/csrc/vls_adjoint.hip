@@ -26,6 +26,7 @@
 // vlasovSolver21_spline::makeOneStep (src/vlasovSolver21_spline.hxx:680-801).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -45,6 +46,18 @@ __device__ __forceinline__ double BV1(double x) {
 }
 __device__ __forceinline__ double dBV2(double x) { return 3.0 * x * x; }
 __device__ __forceinline__ double dBV1(double x) { return 3.0 + 6.0 * x - 9.0 * x * x; }
+
+// Weight of node b (0..3) of a cubic B-spline segment at fraction x, the
+// same value as BV2(1-x), BV1(1-x), BV1(x), BV2(x) for b = 0..3 up to
+// rounding: with t = x for b >= 2 and 1 - x otherwise, t^3 for the outer
+// nodes and 1 + 3t + 3t^2 - 3t^3 for the inner ones.  Branch-free, for lanes
+// whose b differ.
+__device__ __forceinline__ double bweight(int b, double x) {
+  const double t = b >= 2 ? x : 1.0 - x;
+  const double t3 = t * t * t;
+  const double inner = fma(t, fma(t, fma(-3.0, t, 3.0), 3.0), 1.0);
+  return (b == 0 || b == 3) ? t3 : inner;
+}
 
 __device__ __forceinline__ void split(double x, int &n, double &t) {
   double ip = trunc(x);
@@ -76,142 +89,138 @@ __device__ __forceinline__ void eval_c_adj(double *cb, int st, double xr, double
   }
 }
 
-// drift^T with the delay lines.  head0/stepOff: the ring heads at step n are
-// (head0 + stepOff) mod len.
-__global__ void __launch_bounds__(kB)
-k_adj_drift(const double *__restrict__ fb, double *__restrict__ cb, const double *__restrict__ xr,
-            const double *__restrict__ xu, int64_t nVal, int nSlice, int64_t cs, int st,
-            const int64_t *__restrict__ off, const int32_t *__restrict__ len,
-            const int32_t *__restrict__ head0, int64_t stepOff, const double *__restrict__ bxr,
-            const double *__restrict__ bxu, const uint8_t *__restrict__ bout,
-            double *__restrict__ ringb, int64_t nChains) {
-  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (i >= nVal) return;
-  const double x = xr[i];
-  const double lam = fb[i];
-  double *cbj = cb + (i / nSlice) * cs;
-  if (x > -1.0) {
-    if (lam != 0.0) eval_c_adj(cbj, st, x, xu[i], lam);
-  } else if (x == -2.0) {
-    const double kd = xu[i];
-    if (!(kd >= 0) || kd >= (double)nChains) return;
-    const int64_t k = (int64_t)kd;
-    const int L = len[k];
-    const int h = (int)((head0[k] + stepOff) % L);
-    const int back = (h + L - 1) % L;
-    double *rb = ringb + off[k];
-    rb[h] += lam;  // f[point] read the front slot
-    const double xb = rb[back];  // the back slot was written from the spline
-    rb[back] = 0.0;
-    if (!bout[k] && xb != 0.0) eval_c_adj(cbj, st, bxr[k], bxu[k], xb);
-  }
+// Line adjoints for any line length M (the reverse of the LDS-tiled forward
+// solvers): the reverse-mode derivative of the Thomas elimination, run as two
+// streaming sweeps over the padded line, a chunk of kLC elements at a time.
+// The ascending sweep (c1 -= A01 c0, c2 -= A02 c0, c[t] -= u[t-1] c[t-1])
+// writes its values back in place; the descending one applies the last row
+// (tt = c[M-1] / q into c[M-3] and c[M-2]), scales by 1/d[k] and carries
+// c16 c[k] down to c[k-1].  Only the carry lives across chunks, so nothing
+// of the line has to fit on chip.
+constexpr int kLC = 16;
+
+// one element of the descending sweep: element k (1..M-2) of the line after
+// the ascending sweep, the carry from element k+1 in and out
+__device__ __forceinline__ double adj_down(const ThomasAxis &ax, int k, double v, double tt,
+                                           double &carry) {
+  v -= carry;
+  if (k == ax.M - 2) v = fma(-ax.p, tt, v);
+  if (k == ax.M - 3) v = fma(-ax.hi2, tt, v);
+  v *= ax.rd[k];
+  carry = ax.c16 * v;
+  return v;
 }
 
-// Reverse-mode derivative of the natural-spline line solve (the Thomas
-// elimination of the forward kernels): in place, c-bar of the M padded
-// coefficients in, rhs-bar of rows 1..M-2 out.
-template <class ACC>
-__device__ __forceinline__ void thomas_adjoint(const ThomasAxis &ax, ACC c) {
-  const int M = ax.M;
-  const double c0 = c(0);
-  c(1) -= ax.A01 * c0;
-  c(2) -= ax.A02 * c0;
-  for (int k = 1; k <= M - 2; k++) c(k + 1) -= ax.u[k] * c(k);
-  const double tt = c(M - 1) / ax.q;
-  c(M - 3) -= ax.hi2 * tt;
-  c(M - 2) -= ax.p * tt;
-  for (int k = M - 2; k >= 1; k--) {
-    const double r = c(k) / ax.d[k];
-    c(k) = r;
-    if (k >= 2) c(k - 1) -= ax.c16 * r;
-  }
-}
-
-// v lines (lane = padded column), in place
+// v lines (lane = padded column), in place: coalesced rows of 64 columns
 __global__ void __launch_bounds__(64)
 k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   if (k >= S) return;
+  const int M = ax.M;
   double *col = cb + blockIdx.y * cs + k;
-  thomas_adjoint(ax, [&](int m) -> double & { return col[(int64_t)m * S]; });
+  double v[kLC];
+  double c0 = 0.0, prev = 0.0;
+  for (int m0 = 0; m0 < M; m0 += kLC) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m0 + i < M) v[i] = col[(int64_t)(m0 + i) * S];
+#pragma unroll
+    for (int i = 0; i < kLC; i++) {
+      const int t = m0 + i;
+      if (t >= M) break;
+      if (t == 0) c0 = v[i];
+      if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
+      if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
+      if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+      prev = v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m0 + i < M) col[(int64_t)(m0 + i) * S] = v[i];
+  }
+  const double tt = prev * ax.rq;  // prev = c[M-1]
+  double carry = 0.0;
+  for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {  // elements m1, m1-1, ... down to max(1, m1-kLC+1)
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m1 - i >= 1) v[i] = col[(int64_t)(m1 - i) * S];
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m1 - i >= 1) v[i] = adj_down(ax, m1 - i, v[i], tt, carry);
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m1 - i >= 1) col[(int64_t)(m1 - i) * S] = v[i];
+  }
 }
 
-// r rows (lane = padded row 1..nU), rows staged through LDS so that the
-// global accesses are row segments; rhs-bar of row u+1 goes to out[j][u][.]
+// r rows (lane = padded row u+1 of 64 consecutive rows), rhs-bar of row u+1
+// to out[j][u][.]: chunks of kLC columns through an LDS transpose, so that
+// the global accesses are row segments; the ascending sweep writes back into
+// cb (a work array here), the descending one into out.
 __global__ void __launch_bounds__(64)
-k_adj_r(const double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64_t cs,
+k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64_t cs,
         ThomasAxis ax) {
-  extern __shared__ double tile[];  // [64][S + 1]
-  const int S = nR + 2, P = S + 1;
+  constexpr int kP = kLC + 1;
+  __shared__ double chunk[64 * kP];
+  const int M = nR + 2;
+  const int lane = threadIdx.x;
   const int j = blockIdx.y;
   const int u0 = blockIdx.x * 64;
   const int rows = min(64, nU - u0);
-  const double *src = cb + j * cs + (int64_t)(u0 + 1) * S;
-  for (int e = threadIdx.x; e < rows * S; e += 64) tile[(e / S) * P + e % S] = src[e];
-  __syncthreads();
-  if (threadIdx.x < rows) {
-    double *row = tile + threadIdx.x * P;
-    thomas_adjoint(ax, [&](int m) -> double & { return row[m]; });
-  }
-  __syncthreads();
-  double *dst = out + ((int64_t)j * nU + u0) * nR;
-  for (int e = threadIdx.x; e < rows * nR; e += 64) dst[e] = tile[(e / nR) * P + e % nR + 1];
-}
-
-// kick^T per (j, r): coefficient adjoint of the column combination g (into
-// gb, zeroed before; the thread owns column r) and the acceleration adjoint
-__global__ void __launch_bounds__(kB)
-k_adj_kick(const double *__restrict__ fsb, const double *__restrict__ c,
-           const double *__restrict__ U_Val, const double *__restrict__ acc,
-           double *__restrict__ gb, double *__restrict__ abp, int nR, int nU, int64_t cs,
-           double Ulo, double Uhi, double start_v, double hinv_v) {
-  const int r = blockIdx.x * kB + threadIdx.x;
-  const int j = blockIdx.y;
-  if (r >= nR) return;
-  const int S = nR + 2;
-  const double a = acc[r];
-  const double *cj = c + j * cs;
-  double *gj = gb + j * cs;
-  double ab = 0.0;
-  for (int u = 0; u < nU; u++) {
-    const double lam = fsb[((int64_t)j * nU + u) * nR + r];
-    if (lam == 0.0) continue;
-    const double newU = U_Val[u] + a;
-    if ((newU <= Ulo) || (newU >= Uhi)) continue;
-    double t;
-    int n;
-    split((newU - start_v) * hinv_v, n, t);
-    const double w[4] = {BV2(1. - t), BV1(1. - t), BV1(t), BV2(t)};
-    const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
-    double s = 0.0;
+  const int rsub = lane / kLC, ksub = lane % kLC;  // staging role: 4 rows x kLC columns per pass
+  double *src = cb + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  double *dst = out + ((size_t)j * nU + u0) * nR;
+  double v[kLC];
+  // rows x columns [e0, e0 + kLC) of a (rows x width) array <-> LDS, lane-row view
+  auto gather = [&](const double *a, int width, int e0) {
+    const int e = e0 + ksub;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (n + k > nU + 1) break;  // past the padded column: weight 0 (target on the last knot)
-      const double *rowc = cj + (int64_t)(n + k) * S + r;
-      const double g = (rowc[0] + rowc[1] * 4) + rowc[2];
-      s += dw[k] * g;
-      gj[(int64_t)(n + k) * S + r] += lam * w[k] / 36.0;
+    for (int q = 0; q < kLC; q++) {
+      const int row = min(q * 4 + rsub, rows - 1);
+      chunk[(q * 4 + rsub) * kP + ksub] = e < width ? a[(size_t)row * width + e] : 0.0;
     }
-    ab += lam * (hinv_v / 36.0) * s;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kLC; i++) v[i] = chunk[lane * kP + i];
+    __syncthreads();
+  };
+  auto scatter = [&](double *a, int width, int e0) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++) chunk[lane * kP + i] = v[i];
+    __syncthreads();
+    const int e = e0 + ksub;
+#pragma unroll
+    for (int q = 0; q < kLC; q++) {
+      const int row = q * 4 + rsub;
+      if (row < rows && e < width) a[(size_t)row * width + e] = chunk[row * kP + ksub];
+    }
+    __syncthreads();
+  };
+  double c0 = 0.0, prev = 0.0;
+  for (int m0 = 0; m0 < M; m0 += kLC) {
+    gather(src, M, m0);
+#pragma unroll
+    for (int i = 0; i < kLC; i++) {
+      const int t = m0 + i;
+      if (t >= M) break;
+      if (t == 0) c0 = v[i];
+      if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
+      if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
+      if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+      prev = v[i];
+    }
+    scatter(src, M, m0);
   }
-  abp[(int64_t)j * nR + r] = ab;
-}
-
-// c-bar[m][k] = gb[m][k] + 4 gb[m][k-1] + gb[m][k-2] over the columns r < nR
-__global__ void __launch_bounds__(kB)
-k_adj_kick_gather(const double *__restrict__ gb, double *__restrict__ cb, int nR, int nU,
-                  int64_t cs, int64_t n) {
-  const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (e >= n) return;
-  const int S = nR + 2;
-  const int64_t j = e / cs, rem = e - j * cs;
-  const int k = (int)(rem % S);
-  const double *g = gb + e;
-  double v = 0.0;
-  if (k < nR) v += g[0];
-  if (k >= 1 && k - 1 < nR) v += 4.0 * g[-1];
-  if (k >= 2 && k - 2 < nR) v += g[-2];
-  cb[e] = v;
+  const double tt = prev * ax.rq;
+  double carry = 0.0;
+  // descending over chunks of the output columns e = k - 1 (k = 1..M-2)
+  for (int e0 = ((nR - 1) / kLC) * kLC; e0 >= 0; e0 -= kLC) {
+    gather(src, M, e0 + 1);
+#pragma unroll
+    for (int i = kLC - 1; i >= 0; i--)
+      if (e0 + i < nR) v[i] = adj_down(ax, e0 + i + 1, v[i], tt, carry);
+    scatter(dst, nR, e0);
+  }
 }
 
 // scan^T: Mbar[r] = (sum_j abp[j][r]) * gfac[r] (or the given Mbar), then
@@ -282,49 +291,43 @@ k_adj_moment(double *__restrict__ fb, const double *__restrict__ dMb,
   fb[i] = accumulate ? fb[i] + s : s;
 }
 
-// drift^T as a gather: coefficient e (global index j*cs + m*S + k) takes
-// lam(p) * wy[a] * wx[b] / 36 for each entry (p, a*4 + b) of its list, p the
-// point's index inside slice j.  ent[e0..e1) are in increasing point order.
+// drift^T as a gather: coefficient e (index e of slice j's padded array,
+// m*S + k) takes lam(p) * wy[a] * wx[b] / 36 for each entry (p, a*4 + b) of
+// its list, p the point's index inside slice j; its entries are in
+// increasing point order.  A group of kG lanes takes one coefficient (most
+// lists hold 16 entries): the group's first two lanes read the list bounds
+// and pass them round, lane l reads entries l, l + kG, ..., so that a wave's
+// loads of the lists are contiguous and its gathers fall on the few rows of
+// f around the coefficient, and the group then sums its lanes in a fixed
+// tree.  The feet come interleaved (xr, xu), one 16-byte load per entry.
+constexpr int kG = 16;
 __global__ void __launch_bounds__(kB)
 k_adj_drift_gather(const double *__restrict__ fb, double *__restrict__ cb,
-                   const double *__restrict__ xr, const double *__restrict__ xu,
-                   const int64_t *__restrict__ off, const uint32_t *__restrict__ ent,
-                   int64_t nSlice, int64_t cs, int64_t nCoef) {
-  const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (e >= nCoef) return;
-  const int64_t j = e / cs;
+                   const double2 *__restrict__ xy, const int64_t *__restrict__ off,
+                   const uint32_t *__restrict__ ent, int64_t nSlice, int64_t cs) {
+  const int64_t e = ((int64_t)blockIdx.x * kB + threadIdx.x) / kG;
+  const int l = threadIdx.x & (kG - 1);
+  const int64_t j = blockIdx.y;
+  const int64_t eg = j * cs + min(e, cs - 1);
+  const int64_t o = l < 2 ? off[eg + l] : 0;
+  const int64_t q0 = __shfl(o, 0, kG), q1 = __shfl(o, 1, kG);
   const double *fj = fb + j * nSlice;
-  const double *xrj = xr + j * nSlice, *xuj = xu + j * nSlice;
-  // entries in batches of kQ, all loads of a batch issued before their use
-  constexpr int kQ = 8;
+  const double2 *xyj = xy + j * nSlice;
   double acc = 0.0;
-  const int64_t q0 = off[e], q1 = off[e + 1];
-  for (int64_t q = q0; q < q1; q += kQ) {
-    uint32_t v[kQ];
-    double lam[kQ], xx[kQ], yy[kQ];
-#pragma unroll
-    for (int t = 0; t < kQ; t++) v[t] = ent[q + t < q1 ? q + t : q0];
-#pragma unroll
-    for (int t = 0; t < kQ; t++) {
-      const int64_t p = v[t] >> 4;
-      lam[t] = fj[p];
-      xx[t] = xrj[p];
-      yy[t] = xuj[p];
-    }
-#pragma unroll
-    for (int t = 0; t < kQ; t++) {
-      if (q + t >= q1 || lam[t] == 0.0) continue;
-      int nx, ny;
-      double x, y;
-      split(xx[t], nx, x);
-      split(yy[t], ny, y);
-      const int a = (v[t] >> 2) & 3, b = v[t] & 3;
-      const double wx = b == 0 ? BV2(1. - x) : b == 1 ? BV1(1. - x) : b == 2 ? BV1(x) : BV2(x);
-      const double wy = a == 0 ? BV2(1. - y) : a == 1 ? BV1(1. - y) : a == 2 ? BV1(y) : BV2(y);
-      acc += lam[t] / 36.0 * wy * wx;
-    }
+  for (int64_t q = q0 + l; q < q1; q += kG) {
+    const uint32_t v = ent[q];
+    const int64_t p = v >> 4;
+    const double lam = fj[p];
+    const double2 ft = xyj[p];
+    int nx, ny;
+    double x, y;
+    split(ft.x, nx, x);
+    split(ft.y, ny, y);
+    acc += lam * (1.0 / 36.0) * bweight((v >> 2) & 3, y) * bweight(v & 3, x);
   }
-  cb[e] = acc;
+#pragma unroll
+  for (int o2 = kG / 2; o2 >= 1; o2 >>= 1) acc += __shfl_xor(acc, o2);
+  if (e < cs && l == 0) cb[eg] = acc;
 }
 
 // drift^T of the delay lines, one thread per chain (after the gather)
@@ -405,31 +408,24 @@ k_adj_kick_acc(const double *__restrict__ term, double *__restrict__ abp, int nR
 
 // kick^T, coefficient part: cb[j][m][k] = g(m, k) + 4 g(m, k-1) + g(m, k-2),
 // g(m, r) = sum over the points u of column r whose rows n..n+3 include m of
-// fb*[u] B_{m-n}(t) / 36.  The target row of point u is n = u + a_r hinv up to
-// rounding, so the points of row m lie in a short window of u.
+// fb*[u] B_{m-n}(t) / 36.  The target row of point u is n = u + a_r hinv, up
+// to rounding and to the v nodes' offsets from the knots (at most a quarter
+// of a cell, checked when the run is recorded), so the points of row m are
+// among the kW = 7 values of u from floor(m - 3 - a_r hinv) - 1 on.  One block
+// per coefficient row (m, j): g(m, .) into LDS, then the column combination.
 __global__ void __launch_bounds__(kB)
 k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val,
                 const double *__restrict__ acc, double *__restrict__ cb, int nR, int nU,
-                int64_t cs, int64_t nCoef, double Ulo, double Uhi, double start_v,
-                double hinv_v) {
-  const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (e >= nCoef) return;
+                int64_t cs, double Ulo, double Uhi, double start_v, double hinv_v) {
+  extern __shared__ double gl[];  // [nR]
+  const int m = blockIdx.x, j = blockIdx.y;
   const int S = nR + 2;
-  const int64_t j = e / cs;
-  const int rem = (int)(e - j * cs);
-  const int m = rem / S, k = rem - m * S;
-  const double *fj = fsb + j * (int64_t)nU * nR;
-  double out = 0.0;
-#pragma unroll
-  for (int dr = 0; dr < 3; dr++) {
-    const int r = k - dr;
-    if (r < 0 || r >= nR) continue;
+  const double *fj = fsb + (int64_t)j * nU * nR;
+  for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     const double a = acc[r];
     const double sh = a * hinv_v;
-    // n = u + a hinv up to rounding: rows n..n+3 include m for u in
-    // [m - 3 - a hinv, m - a hinv], widened by two on each side
-    const int ulo = (int)floor((double)m - 3.0 - sh) - 2;
-    constexpr int kW = 10;
+    const int ulo = (int)floor((double)m - 3.0 - sh) - 1;
+    constexpr int kW = 7;
     double lam[kW], uu[kW];
 #pragma unroll
     for (int q = 0; q < kW; q++) {
@@ -441,7 +437,7 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
 #pragma unroll
     for (int q = 0; q < kW; q++) {
       const int u = ulo + q;
-      if (u < 0 || u >= nU || lam[q] == 0.0) continue;
+      if (u < 0 || u >= nU) continue;
       const double newU = uu[q] + a;
       if ((newU <= Ulo) || (newU >= Uhi)) continue;
       double t;
@@ -449,30 +445,44 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
       split((newU - start_v) * hinv_v, n, t);
       const int k4 = m - n;
       if (k4 < 0 || k4 > 3) continue;
-      const double w = k4 == 0 ? BV2(1. - t) : k4 == 1 ? BV1(1. - t) : k4 == 2 ? BV1(t) : BV2(t);
-      g += lam[q] * w / 36.0;
+      g += lam[q] * bweight(k4, t) * (1.0 / 36.0);
     }
-    out += (dr == 1 ? 4.0 : 1.0) * g;
+    gl[r] = g;
   }
-  cb[e] = out;
+  __syncthreads();
+  double *row = cb + (int64_t)j * cs + (int64_t)m * S;
+  for (int k = threadIdx.x; k < S; k += blockDim.x) {
+    double out = k < nR ? gl[k] : 0.0;
+    if (k >= 1 && k - 1 < nR) out += 4.0 * gl[k - 1];
+    if (k >= 2) out += gl[k - 2];
+    row[k] = out;
+  }
 }
 
 // Line adjoints in registers for a line length known at compile time
 // (the reverse of k_line / k_line_rows); divisions become products with
-// the tabulated reciprocals.
+// the tabulated reciprocals.  Past row kHead the pivots are at their fixed
+// point (the table repeats uS and rdS there; the set-up checks K0 - 1 <=
+// kHead), so only the head is read from the table.  The adjoint is not held
+// to the forward's operation order, so its updates are fused multiply-adds.
+constexpr int kHead = 16;
 template <int M>
 __device__ __forceinline__ void thomas_adjoint_reg(const ThomasAxis &ax, double (&c)[M], int lz) {
-  c[1] -= ax.A01 * c[0];
-  c[2] -= ax.A02 * c[0];
+  const double uS = ax.uS + (double)lz, rdS = ax.rdS + (double)lz;
+  c[1] = fma(-ax.A01, c[0], c[1]);
+  c[2] = fma(-ax.A02, c[0], c[2]);
 #pragma unroll
-  for (int k = 1; k <= M - 2; k++) c[k + 1] -= ax.u[k + lz] * c[k];
+  for (int k = 1; k <= M - 2; k++) {
+    const double uv = k <= kHead ? ax.u[k + lz] : uS;
+    c[k + 1] = fma(-uv, c[k], c[k + 1]);
+  }
   const double tt = c[M - 1] * ax.rq;
-  c[M - 3] -= ax.hi2 * tt;
-  c[M - 2] -= ax.p * tt;
+  c[M - 3] = fma(-ax.hi2, tt, c[M - 3]);
+  c[M - 2] = fma(-ax.p, tt, c[M - 2]);
 #pragma unroll
   for (int k = M - 2; k >= 1; k--) {
-    c[k] = c[k] * ax.rd[k + lz];
-    if (k >= 2) c[k - 1] -= ax.c16 * c[k];
+    c[k] *= k <= kHead ? ax.rd[k + lz] : rdS;
+    if (k >= 2) c[k - 1] = fma(-ax.c16, c[k], c[k - 1]);
   }
 }
 
@@ -491,31 +501,96 @@ k_adj_v_reg(double *__restrict__ cb, int64_t cs, int lzmask, ThomasAxis ax) {
   for (int m = 1; m <= M - 2; m++) col[(int64_t)m * S] = c[m];
 }
 
+// r-row adjoint with coalesced memory access, the transpose of k_line_rows:
+// a wave takes 64 consecutive rows u0.. of slice j.  The padded coefficient
+// rows of the adjoint arrive in chunks of kC columns through a small LDS
+// transpose and feed the ascending sweep of the transposed elimination as
+// they come (the loads of the next chunk fly meanwhile); the descending
+// sweep releases the adjoint of f a chunk at a time, last chunk first.
+// Loads are clamped into the wave's rows; the extra lanes of a partial wave
+// solve a copy and store nothing.
 template <int M>
 __global__ void __launch_bounds__(64)
-k_adj_r_reg(const double *__restrict__ cb, double *__restrict__ out, int nU, int64_t cs,
-            int lzmask, ThomasAxis ax) {
-  constexpr int nR = M - 2, P = M | 1;  // odd pitch: the lanes' row reads spread over the banks
-  extern __shared__ double tile[];  // [64][P]
+k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int64_t cs,
+           int lzmask, ThomasAxis ax) {
+  constexpr int kC = 16, kP = kC + 1;
+  constexpr int nR = M - 2;
+  constexpr int nChIn = (M + kC - 1) / kC, nChOut = (nR + kC - 1) / kC;
+  __shared__ double chunk[64 * kP];
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
   const int j = blockIdx.y;
   const int u0 = blockIdx.x * 64;
   const int rows = min(64, nU - u0);
-  const int lz = threadIdx.x & lzmask;
-  const double *src = cb + j * cs + (int64_t)(u0 + 1) * M;
-  for (int e = threadIdx.x; e < rows * M; e += 64) tile[(e / M) * P + e % M] = src[e];
-  __syncthreads();
-  if (threadIdx.x < rows) {
-    double *row = tile + threadIdx.x * P;
-    double c[M];
+  const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC columns per pass
+  const double *src = cb + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  double *dst = out + ((size_t)j * nU + u0) * nR;
+  const double uS = ax.uS + (double)lz, rdS = ax.rdS + (double)lz;
+  double c[M];
+  double tmp[kC];
+  auto load_chunk = [&](int ch) {
+    const int e = min(ch * kC + ksub, M - 1);
 #pragma unroll
-    for (int m = 0; m < M; m++) c[m] = row[m];
-    thomas_adjoint_reg<M>(ax, c, lz);
+    for (int q = 0; q < kC; q++) tmp[q] = src[min(q * 4 + rsub, rows - 1) * M + e];
+  };
+  load_chunk(0);
 #pragma unroll
-    for (int m = 1; m <= M - 2; m++) row[m] = c[m];
+  for (int ch = 0; ch < nChIn; ch++) {
+#pragma unroll
+    for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[q];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < M) c[ch * kC + i] = chunk[lane * kP + i];
+    __syncthreads();
+    if (ch + 1 < nChIn) load_chunk(ch + 1);
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+      const int t = ch * kC + i;  // c[t] -= u[t-1] c[t-1], ascending
+      if (t == 2) {
+        c[1] = fma(-ax.A01, c[0], c[1]);
+        c[2] = fma(-ax.A02, c[0], c[2]);
+      }
+      if (t >= 2 && t <= M - 1) {
+        const double uv = t - 1 <= kHead ? ax.u[t - 1 + lz] : uS;
+        c[t] = fma(-uv, c[t - 1], c[t]);
+      }
+    }
   }
-  __syncthreads();
-  double *dst = out + ((int64_t)j * nU + u0) * nR;
-  for (int e = threadIdx.x; e < rows * nR; e += 64) dst[e] = tile[(e / nR) * P + e % nR + 1];
+  const double tt = c[M - 1] * ax.rq;
+  c[M - 3] = fma(-ax.hi2, tt, c[M - 3]);
+  c[M - 2] = fma(-ax.p, tt, c[M - 2]);
+#pragma unroll
+  for (int ch = nChOut - 1; ch >= 0; ch--) {
+#pragma unroll
+    for (int i = kC - 1; i >= 0; i--) {
+      const int k = ch * kC + i + 1;  // produces output column k - 1
+      if (k <= M - 2) {
+        c[k] *= k <= kHead ? ax.rd[k + lz] : rdS;
+        if (k >= 2) c[k - 1] = fma(-ax.c16, c[k], c[k - 1]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < nR) chunk[lane * kP + i] = c[ch * kC + i + 1];
+    __syncthreads();
+    // all LDS reads first, so that their latencies overlap, then the stores
+    const int e = ch * kC + ksub;
+    double sv[kC];
+#pragma unroll
+    for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
+    __syncthreads();
+    if (rows == 64 && (ch + 1) * kC <= nR) {
+#pragma unroll
+      for (int q = 0; q < kC; q++) dst[(q * 4 + rsub) * nR + e] = sv[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < kC; q++) {
+        const int row = q * 4 + rsub;
+        if (row < rows && e < nR) dst[row * nR + e] = sv[q];
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -532,7 +607,7 @@ struct AdjShape {
   AdjV v;
   AdjR r;
 };
-#define VLS_ADJ_SHAPE(R, U) {R, U, k_adj_v_reg<(U) + 2, (R) + 2>, k_adj_r_reg<(R) + 2>}
+#define VLS_ADJ_SHAPE(R, U) {R, U, k_adj_v_reg<(U) + 2, (R) + 2>, k_adj_rows<(R) + 2>}
 static const AdjShape kAdjShapes[] = {
     VLS_ADJ_SHAPE(17, 17),   VLS_ADJ_SHAPE(25, 13),   VLS_ADJ_SHAPE(33, 33),
     VLS_ADJ_SHAPE(49, 49),   VLS_ADJ_SHAPE(65, 65),   VLS_ADJ_SHAPE(129, 129),
@@ -548,9 +623,9 @@ static const AdjShape *adj_shape(const Run &r) {
 
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s) {
-  const int64_t nCoef = (int64_t)r.nJ * r.cs;
-  hipLaunchKernelGGL(k_adj_drift_gather, dim3(nblocks(nCoef)), dim3(kB), 0, s, fb, cb, r.xr, r.xu,
-                     r.adj.csrOff, r.adj.csrEnt, (int64_t)r.nU * r.nR, r.cs, nCoef);
+  hipLaunchKernelGGL(k_adj_drift_gather, dim3(nblocks(r.cs * kG), r.nJ), dim3(kB), 0, s, fb, cb,
+                     (const double2 *)r.adj.csrXY, r.adj.csrOff, r.adj.csrEnt,
+                     (int64_t)r.nU * r.nR, r.cs);
   if (r.nChains)
     hipLaunchKernelGGL(k_adj_chains, dim3(nblocks(r.nChains)), dim3(kB), 0, s, fb, cb, r.ch_point,
                        r.ch_off, r.ch_len, head0, stepOff, r.ch_xr, r.ch_xu, r.ch_out, ringb,
@@ -558,16 +633,15 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
 }
 
 void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
-  const size_t lds = 64 * (size_t)(r.nR + 3) * sizeof(double);
   if (const AdjShape *a = adj_shape(r)) {
     hipLaunchKernelGGL(a->v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.cs, 0, r.tv);
-    hipLaunchKernelGGL(a->r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), lds, s, cb, out, r.nU, r.cs, 0,
+    hipLaunchKernelGGL(a->r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nU, r.cs, 0,
                        r.tr);
     return;
   }
   hipLaunchKernelGGL(k_adj_v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.nR + 2, r.cs,
                      r.tv);
-  hipLaunchKernelGGL(k_adj_r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), lds, s, cb, out, r.nR, r.nU,
+  hipLaunchKernelGGL(k_adj_r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nR, r.nU,
                      r.cs, r.tr);
 }
 
@@ -577,9 +651,9 @@ void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *
                      accK, term, r.nR, r.nU, r.cs, r.nVal, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
   hipLaunchKernelGGL(k_adj_kick_acc, dim3(nblocks(r.nR), r.nJ), dim3(kB), 0, s, term, abp, r.nR,
                      r.nU);
-  const int64_t nCoef = (int64_t)r.nJ * r.cs;
-  hipLaunchKernelGGL(k_adj_kick_coef, dim3(nblocks(nCoef)), dim3(kB), 0, s, fsb, r.U_Val, accK, cb,
-                     r.nR, r.nU, r.cs, nCoef, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  const int bs = std::min(kB, (r.nR + 2 + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_adj_kick_coef, dim3(r.nU + 2, r.nJ), dim3(bs), r.nR * sizeof(double), s, fsb,
+                     r.U_Val, accK, cb, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
@@ -590,17 +664,6 @@ void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dM
                      r.dR, r.nR, r.nU, r.nVal, accumulate ? 1 : 0);
 }
 
-void configure_adjoint(int maxLdsPerBlock, int nR) {
-  const size_t lds = 64 * (size_t)(nR + 3) * sizeof(double);
-  if (lds > (size_t)maxLdsPerBlock)
-    throw Error(VLS_ERR_ARG, "grid too large for the adjoint's row tiles");
-  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_adj_r,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  for (const AdjShape &a : kAdjShapes)
-    if (a.nR == nR)
-      VLS_HIP_CHECK(hipFuncSetAttribute((const void *)a.r,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-}
 
 // The transposed foot table of the drift (built on the host from the folded
 // foot table): for every coefficient of the padded slices, the points whose

@@ -42,6 +42,19 @@ T *dalloc(vls::Run &r, size_t n) {
   return static_cast<T *>(p);
 }
 
+// Release an allocation of dalloc before the run ends (and null the pointer).
+template <class T>
+void dfree(vls::Run &r, T *&p) {
+  if (!p) return;
+  for (size_t i = 0; i < r.allocs.size(); i++)
+    if (r.allocs[i] == (void *)p) {
+      r.allocs.erase(r.allocs.begin() + i);
+      break;
+    }
+  VLS_HIP_CHECK(hipFree((void *)p));
+  p = nullptr;
+}
+
 size_t kCoefPad(const vls::Run &r) { return 2 * (size_t)(r.nR + 2) + 8; }
 
 // A foot can be evaluated when its integer parts lie in [0, dim-1] on both
@@ -716,7 +729,6 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.dMb = dalloc<double>(r, r.nR);
       A.Mbar = dalloc<double>(r, r.nR);
       A.endF = dalloc<double>(r, nv);
-      vls::configure_adjoint(ctx->c.maxLds, r.nR);
     }
     // the chain-sized arrays follow the current delay lines
     const bool chainsChanged = A.cells != (int64_t)nc || A.chains != (int64_t)nch;
@@ -742,10 +754,21 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       std::vector<int64_t> off;
       std::vector<uint32_t> ent;
       vls::build_drift_transpose(r, r.hxr, r.hxu, off, ent);
+      std::vector<double> xy(2 * nv);
+      for (size_t i = 0; i < nv; i++) {
+        xy[2 * i] = r.hxr[i];
+        xy[2 * i + 1] = r.hxu[i];
+      }
+      VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
+      dfree(r, A.csrOff);
+      dfree(r, A.csrEnt);
+      dfree(r, A.csrXY);
       A.csrOff = dalloc<int64_t>(r, off.size());
       A.csrEnt = dalloc<uint32_t>(r, ent.size());
+      A.csrXY = dalloc<double>(r, xy.size());
       h2d(A.csrOff, off.data(), off.size(), s);
       h2d(A.csrEnt, ent.data(), ent.size(), s);
+      h2d(A.csrXY, xy.data(), xy.size(), s);
       VLS_HIP_CHECK(hipStreamSynchronize(s));
       A.csrValid = true;
     }

@@ -100,6 +100,7 @@ struct Run {
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
     int64_t *csrOff = nullptr;  // transposed foot table of the drift
     uint32_t *csrEnt = nullptr;
+    double *csrXY = nullptr;    // the folded feet interleaved (xr, xu), for the gather
     bool csrValid = false;
     bool recorded = false;
   } adj;
@@ -140,7 +141,6 @@ void launch_stats(Run &r, hipStream_t s);
 void time_mark(Timing *t, hipStream_t s, const char *name);
 
 // adjoint launchers (vls_adjoint.hip)
-void configure_adjoint(int maxLdsPerBlock, int nR);
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s);
