@@ -180,12 +180,13 @@ int vls_mass_profile(vls_ctx *ctx, int run, double *M);
  * derivative in f_N; may be NULL).  The states between checkpoints are
  * recomputed segment by segment; the context ends in the state after the
  * recorded run.  grad_vals0 (may be NULL) has the ring layout of
- * vls_set_chains at the first recorded step.  The drift's transpose is a
- * gather in a fixed order; only the delay lines' back cells add into it with
- * floating-point atomics, so the last bits of the gradient may vary from call
- * to call where back cells share coefficients.  vls_record builds, on first
- * use after vls_set_drift, the transposed foot table on the host (16 entries
- * of 4 bytes per evaluated point) and keeps, per step of a checkpoint
+ * vls_set_chains at the first recorded step.  The drift's transpose adds
+ * the points' contributions into tiles of coefficients with LDS atomics and
+ * the delay lines' back cells add theirs with global floating-point atomics,
+ * so the last bits of the gradient may vary from call to call.  vls_record
+ * builds, on first use after vls_set_drift, the transposed foot table on the
+ * host (about 1.4 entries of 4 bytes per evaluated point, plus the feet
+ * interleaved, 16 bytes per point) and keeps, per step of a checkpoint
  * segment, the kick's coefficients and acceleration in HBM. */
 int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery);
 int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,

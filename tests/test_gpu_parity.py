@@ -122,3 +122,17 @@ def test_unfused_kick_bit_exact(monkeypatch):
     print("case udf_c1 with VLS_NO_FUSED_KICK", flush=True)
     monkeypatch.setenv("VLS_NO_FUSED_KICK", "1")
     _bit_exact_run(UDF_C1, 12, "unfused kick")
+
+
+@pytest.mark.parametrize("pres,ures", [(20, [18, 5]), (31, [33, 6])])
+def test_streaming_solvers_bit_exact(monkeypatch, pres, ures):
+    """The streaming line solvers (taken by lines too long for an LDS tile of
+    64 lines, such as the 515 padded nodes of the 513 x 513 grids), forced on
+    shapes without register solvers.  With nR = 32 the forward sweep ends on
+    a chunk boundary and the last two padded columns form a chunk of their own."""
+    from tests.configs import UDF_C1
+
+    cfg = dict(UDF_C1, **{"grid.Pres": pres, "grid.Ures": ures})
+    print(f"case udf {pres}x{ures} with VLS_SPLINE_STREAM", flush=True)
+    monkeypatch.setenv("VLS_SPLINE_STREAM", "1")
+    _bit_exact_run(cfg, 8, "streaming solvers")

@@ -4,13 +4,12 @@
 // (f, delay-line values) except through the kick's acceleration a, which is
 // linear in f (moment, exclusive radial scan).  Its adjoint, applied to
 // lambda_{n+1} = (fb, ringb), is, in reverse order of the forward phases:
-//   drift^T   each coefficient gathers fb * (B-spline weight) from the points
-//             whose 4x4 foot stencil covers it, through a transposed foot
-//             table built once (the feet are static), in point order, so the
-//             result does not depend on scheduling; a delay-line point moves
-//             its fb into the ring slot it read, and the slot written in this
-//             step passes its adjoint to the back cell's stencil (atomics,
-//             a few hundred cells);
+//   drift^T   per tile of coefficients, the points whose 4x4 foot stencil
+//             touches it (a table built once; the feet are static) add
+//             fb * (B-spline weight) into the tile in LDS, and the tile is
+//             stored; a delay-line point moves its fb into the ring slot it
+//             read, and the slot written in this step passes its adjoint to
+//             the back cell's stencil (global atomics, one chain per thread);
 //   spline^T  the transposed natural-spline solves, v lines then r rows, by
 //             reverse-mode differentiation of the Thomas elimination;
 //   kick^T    per point, the acceleration term fb* hinv/36 sum_k B_k'(t)
@@ -127,12 +126,13 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
 #pragma unroll
     for (int i = 0; i < kLC; i++) {
       const int t = m0 + i;
-      if (t >= M) break;
-      if (t == 0) c0 = v[i];
-      if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
-      if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-      if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
-      prev = v[i];
+      if (t < M) {
+        if (t == 0) c0 = v[i];
+        if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
+        if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
+        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+        prev = v[i];
+      }
     }
 #pragma unroll
     for (int i = 0; i < kLC; i++)
@@ -202,12 +202,13 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
 #pragma unroll
     for (int i = 0; i < kLC; i++) {
       const int t = m0 + i;
-      if (t >= M) break;
-      if (t == 0) c0 = v[i];
-      if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
-      if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-      if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
-      prev = v[i];
+      if (t < M) {
+        if (t == 0) c0 = v[i];
+        if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
+        if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
+        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+        prev = v[i];
+      }
     }
     scatter(src, M, m0);
   }
@@ -291,43 +292,59 @@ k_adj_moment(double *__restrict__ fb, const double *__restrict__ dMb,
   fb[i] = accumulate ? fb[i] + s : s;
 }
 
-// drift^T as a gather: coefficient e (index e of slice j's padded array,
-// m*S + k) takes lam(p) * wy[a] * wx[b] / 36 for each entry (p, a*4 + b) of
-// its list, p the point's index inside slice j; its entries are in
-// increasing point order.  A group of kG lanes takes one coefficient (most
-// lists hold 16 entries): the group's first two lanes read the list bounds
-// and pass them round, lane l reads entries l, l + kG, ..., so that a wave's
-// loads of the lists are contiguous and its gathers fall on the few rows of
-// f around the coefficient, and the group then sums its lanes in a fixed
-// tree.  The feet come interleaved (xr, xu), one 16-byte load per entry.
-constexpr int kG = 16;
-__global__ void __launch_bounds__(kB)
-k_adj_drift_gather(const double *__restrict__ fb, double *__restrict__ cb,
-                   const double2 *__restrict__ xy, const int64_t *__restrict__ off,
-                   const uint32_t *__restrict__ ent, int64_t nSlice, int64_t cs) {
-  const int64_t e = ((int64_t)blockIdx.x * kB + threadIdx.x) / kG;
-  const int l = threadIdx.x & (kG - 1);
-  const int64_t j = blockIdx.y;
-  const int64_t eg = j * cs + min(e, cs - 1);
-  const int64_t o = l < 2 ? off[eg + l] : 0;
-  const int64_t q0 = __shfl(o, 0, kG), q1 = __shfl(o, 1, kG);
+// drift^T over tiles of kTile x kTile coefficients of a padded slice: the
+// block (one wave) takes the points whose 4x4 foot stencil touches its tile
+// (a list built once on the host; the feet are static), adds each point's
+// lam/36 * wy[a] * wx[b] into the tile in LDS and stores the tile.  Every
+// coefficient belongs to one tile, so the global stores need no atomics,
+// and the table holds about 1.4 entries per point (16 for a list per
+// coefficient).  The points come in increasing order and one wave adds them,
+// a chunk of 64 at a time.
+constexpr int kTile = 16;
+__global__ void __launch_bounds__(64)
+k_adj_drift_tiles(const double *__restrict__ fb, double *__restrict__ cb,
+                  const double2 *__restrict__ xy, const int64_t *__restrict__ toff,
+                  const uint32_t *__restrict__ tpts, int64_t nSlice, int64_t cs, int nR, int nU,
+                  int ntk) {
+  __shared__ double acc[kTile * kTile];
+  const int tile = blockIdx.x, j = blockIdx.y;
+  const int tm = tile / ntk, tk = tile - tm * ntk;
+  const int m0 = tm * kTile, k0 = tk * kTile;
+  const int S = nR + 2, Mv = nU + 2;
+  for (int i = threadIdx.x; i < kTile * kTile; i += 64) acc[i] = 0.0;
+  __syncthreads();
+  const int64_t t = (int64_t)j * gridDim.x + tile;
+  const int64_t q1 = toff[t + 1];
   const double *fj = fb + j * nSlice;
   const double2 *xyj = xy + j * nSlice;
-  double acc = 0.0;
-  for (int64_t q = q0 + l; q < q1; q += kG) {
-    const uint32_t v = ent[q];
-    const int64_t p = v >> 4;
-    const double lam = fj[p];
+  for (int64_t q = toff[t] + threadIdx.x; q < q1; q += 64) {
+    const uint32_t p = tpts[q];
+    const double lam = fj[p] * (1.0 / 36.0);
     const double2 ft = xyj[p];
     int nx, ny;
     double x, y;
     split(ft.x, nx, x);
     split(ft.y, ny, y);
-    acc += lam * (1.0 / 36.0) * bweight((v >> 2) & 3, y) * bweight(v & 3, x);
-  }
+    const double wx[4] = {BV2(1. - x), BV1(1. - x), BV1(x), BV2(x)};
+    const double wy[4] = {BV2(1. - y), BV1(1. - y), BV1(y), BV2(y)};
 #pragma unroll
-  for (int o2 = kG / 2; o2 >= 1; o2 >>= 1) acc += __shfl_xor(acc, o2);
-  if (e < cs && l == 0) cb[eg] = acc;
+    for (int a = 0; a < 4; a++) {
+      const int m = ny + a - m0;
+      if (m < 0 || m >= kTile) continue;
+      const double la = lam * wy[a];
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int k = nx + b - k0;
+        if (k >= 0 && k < kTile) atomicAdd(&acc[m * kTile + k], la * wx[b]);
+      }
+    }
+  }
+  __syncthreads();
+  double *cj = cb + j * cs;
+  for (int i = threadIdx.x; i < kTile * kTile; i += 64) {
+    const int m = m0 + i / kTile, k = k0 + i % kTile;
+    if (m < Mv && k < S) cj[(int64_t)m * S + k] = acc[i];
+  }
 }
 
 // drift^T of the delay lines, one thread per chain (after the gather)
@@ -623,9 +640,10 @@ static const AdjShape *adj_shape(const Run &r) {
 
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_drift_gather, dim3(nblocks(r.cs * kG), r.nJ), dim3(kB), 0, s, fb, cb,
-                     (const double2 *)r.adj.csrXY, r.adj.csrOff, r.adj.csrEnt,
-                     (int64_t)r.nU * r.nR, r.cs);
+  const int ntm = (r.nU + 2 + kTile - 1) / kTile, ntk = (r.nR + 2 + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_adj_drift_tiles, dim3(ntm * ntk, r.nJ), dim3(64), 0, s, fb, cb,
+                     (const double2 *)r.adj.footXY, r.adj.tileOff, r.adj.tilePts,
+                     (int64_t)r.nU * r.nR, r.cs, r.nR, r.nU, ntk);
   if (r.nChains)
     hipLaunchKernelGGL(k_adj_chains, dim3(nblocks(r.nChains)), dim3(kB), 0, s, fb, cb, r.ch_point,
                        r.ch_off, r.ch_len, head0, stepOff, r.ch_xr, r.ch_xu, r.ch_out, ringb,
@@ -665,15 +683,18 @@ void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dM
 }
 
 
-// The transposed foot table of the drift (built on the host from the folded
-// foot table): for every coefficient of the padded slices, the points whose
-// 4x4 stencil covers it with a non-zero weight, in point order.
-void build_drift_transpose(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
-                           std::vector<int64_t> &off, std::vector<uint32_t> &ent) {
-  const int64_t nSlice = (int64_t)r.nU * r.nR, nCoef = (int64_t)r.nJ * r.cs;
+// The drift's transposed foot table (built on the host from the folded foot
+// table): for every kTile x kTile tile of the padded slices, in tile order
+// (j, tile row, tile column), the slice-local indices of the points whose
+// 4x4 stencil touches the tile with a non-zero weight, in increasing order.
+void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
+                       std::vector<int64_t> &off, std::vector<uint32_t> &pts) {
+  const int64_t nSlice = (int64_t)r.nU * r.nR;
   const int S = r.nR + 2, Mv = r.nU + 2;
-  if (nSlice >= (int64_t)1 << 28) throw Error(VLS_ERR_ARG, "slice too large for the adjoint's tables");
-  off.assign(nCoef + 1, 0);
+  const int ntm = (Mv + kTile - 1) / kTile, ntk = (S + kTile - 1) / kTile;
+  const int64_t nTiles = (int64_t)r.nJ * ntm * ntk;
+  if (nSlice >= (int64_t)1 << 32) throw Error(VLS_ERR_ARG, "slice too large for the adjoint's tables");
+  off.assign(nTiles + 1, 0);
   auto each = [&](auto &&fn) {
     for (int64_t j = 0; j < r.nJ; j++)
       for (int64_t p = 0; p < nSlice; p++) {
@@ -681,24 +702,20 @@ void build_drift_transpose(Run &r, const std::vector<double> &xr, const std::vec
         const double x = xr[i];
         if (x <= -1.0) continue;  // outside the grid or a delay line
         const double ix = std::trunc(x), iy = std::trunc(xu[i]);
-        const double fx = x - ix, fy = xu[i] - iy;
         const int nx = (int)ix, ny = (int)iy;
-        for (int a = 0; a < 4; a++) {
-          if (a == 3 && fy == 0.0) continue;  // weight BV2(0) = 0
-          for (int b = 0; b < 4; b++) {
-            if (b == 3 && fx == 0.0) continue;
-            const int row = ny + a, col = nx + b;
-            if (row >= Mv || col >= S) continue;
-            fn(j * r.cs + (int64_t)row * S + col, (uint32_t)((p << 4) | (a << 2) | b));
-          }
-        }
+        // last stencil row / column: weight BV2(0) = 0 at a fraction of 0
+        const int rowHi = std::min(ny + (xu[i] - iy == 0.0 ? 2 : 3), Mv - 1);
+        const int colHi = std::min(nx + (x - ix == 0.0 ? 2 : 3), S - 1);
+        for (int tm = ny / kTile; tm <= rowHi / kTile; tm++)
+          for (int tk = nx / kTile; tk <= colHi / kTile; tk++)
+            fn((j * ntm + tm) * ntk + tk, (uint32_t)p);
       }
   };
-  each([&](int64_t e, uint32_t) { off[e + 1]++; });
-  for (int64_t e = 0; e < nCoef; e++) off[e + 1] += off[e];
-  ent.assign(off[nCoef], 0);
+  each([&](int64_t t, uint32_t) { off[t + 1]++; });
+  for (int64_t t = 0; t < nTiles; t++) off[t + 1] += off[t];
+  pts.assign(off[nTiles], 0);
   std::vector<int64_t> cur(off.begin(), off.end() - 1);
-  each([&](int64_t e, uint32_t v) { ent[cur[e]++] = v; });
+  each([&](int64_t t, uint32_t p) { pts[cur[t]++] = p; });
 }
 
 }  // namespace vls

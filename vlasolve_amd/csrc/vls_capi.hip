@@ -507,7 +507,7 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.hxr.swap(xf);
     r.hxu.swap(uf);
-    r.adj.csrValid = false;  // the adjoint's transposed foot table follows the feet
+    r.adj.tilesValid = false;  // the adjoint's transposed foot table follows the feet
     vls::mark_chains(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
@@ -750,27 +750,27 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
       A.capSeg = seg;
     }
-    if (!A.csrValid) {
+    if (!A.tilesValid) {
       std::vector<int64_t> off;
-      std::vector<uint32_t> ent;
-      vls::build_drift_transpose(r, r.hxr, r.hxu, off, ent);
+      std::vector<uint32_t> pts;
+      vls::build_drift_tiles(r, r.hxr, r.hxu, off, pts);
       std::vector<double> xy(2 * nv);
       for (size_t i = 0; i < nv; i++) {
         xy[2 * i] = r.hxr[i];
         xy[2 * i + 1] = r.hxu[i];
       }
       VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
-      dfree(r, A.csrOff);
-      dfree(r, A.csrEnt);
-      dfree(r, A.csrXY);
-      A.csrOff = dalloc<int64_t>(r, off.size());
-      A.csrEnt = dalloc<uint32_t>(r, ent.size());
-      A.csrXY = dalloc<double>(r, xy.size());
-      h2d(A.csrOff, off.data(), off.size(), s);
-      h2d(A.csrEnt, ent.data(), ent.size(), s);
-      h2d(A.csrXY, xy.data(), xy.size(), s);
+      dfree(r, A.tileOff);
+      dfree(r, A.tilePts);
+      dfree(r, A.footXY);
+      A.tileOff = dalloc<int64_t>(r, off.size());
+      A.tilePts = dalloc<uint32_t>(r, pts.size());
+      A.footXY = dalloc<double>(r, xy.size());
+      h2d(A.tileOff, off.data(), off.size(), s);
+      h2d(A.tilePts, pts.data(), pts.size(), s);
+      h2d(A.footXY, xy.data(), xy.size(), s);
       VLS_HIP_CHECK(hipStreamSynchronize(s));
-      A.csrValid = true;
+      A.tilesValid = true;
     }
     for (int n = 0; n < nSteps; n++) {
       if (n % ckEvery == 0) {

@@ -56,6 +56,7 @@ struct Run {
   int stride = 0;       // padded row stride nR+2
   int TR = 32, LS = 0;  // r-sweep tile: rows per block, odd LDS row stride (>= TR)
   int TC = 32;          // v-sweep tile: columns per block
+  bool streamR = false, streamV = false;  // streaming solvers for long lines
   vls_solver_params sp{};
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
 
@@ -98,10 +99,10 @@ struct Run {
     double *segAcc = nullptr;   // [capSeg][nR] the kick accelerations
     double *fb = nullptr, *fsb = nullptr, *cb = nullptr, *gb = nullptr;
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
-    int64_t *csrOff = nullptr;  // transposed foot table of the drift
-    uint32_t *csrEnt = nullptr;
-    double *csrXY = nullptr;    // the folded feet interleaved (xr, xu), for the gather
-    bool csrValid = false;
+    int64_t *tileOff = nullptr;  // transposed foot table of the drift, per coefficient tile
+    uint32_t *tilePts = nullptr;
+    double *footXY = nullptr;    // the folded feet interleaved (xr, xu)
+    bool tilesValid = false;
     bool recorded = false;
   } adj;
   std::vector<double> hxr, hxu;  // host copies of the folded foot table
@@ -147,8 +148,8 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
 void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *term,
                      double *cb, double *abp, hipStream_t s);
-void build_drift_transpose(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
-                           std::vector<int64_t> &off, std::vector<uint32_t> &ent);
+void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
+                       std::vector<int64_t> &off, std::vector<uint32_t> &pts);
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
                      bool accumulate, hipStream_t s);
 

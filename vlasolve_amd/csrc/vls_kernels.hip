@@ -358,6 +358,128 @@ k_spline_v(double *__restrict__ c, int nR, int nU, int nJ, int64_t cs, int TC, i
       });
 }
 
+// Streaming line solvers for lines too long for an LDS tile of 64 lines
+// (M > ~100): the forward values y_k take a trip through memory (written
+// over the line, read back by the back substitution) instead of waiting on
+// chip, so that no LDS per line is needed and every line of the grid is in
+// flight at once; they move 4 doubles per coefficient instead of 2.  Same
+// operations in the same order as thomas_forward / thomas_backward.
+//
+// v direction: lane = padded column, 64 per wave, every access a coalesced
+// row segment, kPF rows ahead both ways; in place on c.
+__global__ void __launch_bounds__(64)
+k_spline_v_stream(double *__restrict__ c, int nR, int64_t cs, int lzmask, ThomasAxis ax) {
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;  // a zero the compiler treats as per-lane
+  const int S = nR + 2;
+  const int col = blockIdx.x * 64 + lane;
+  if (col >= S) return;  // no barriers below
+  double *tab = c + (size_t)blockIdx.y * cs + col;
+  double ym2, ym1;
+  thomas_forward<16>(
+      ax, lz, [&](int k) { return tab[(int64_t)k * S]; },
+      [&](int k, double y) { tab[(int64_t)k * S] = y; }, ym2, ym1);
+  thomas_backward<16>(
+      ax, lz, ym2, ym1, [&](int k) { return tab[(int64_t)k * S]; },
+      [&](int k, double v) { tab[(int64_t)k * S] = v; });
+}
+
+// r direction: lane = row, 64 consecutive rows of a slice per wave; chunks
+// of kC columns through a small LDS transpose (4 rows x kC columns per pass,
+// so that the global accesses are row segments).  The forward sweep reads
+// f and leaves y_k in the padded coefficient rows, the back substitution
+// reads them back a chunk at a time from the last column down.
+__global__ void __launch_bounds__(64)
+k_spline_r_stream(const double *__restrict__ f, double *__restrict__ c, int nR, int nU,
+                  int64_t cs, int lzmask, ThomasAxis ax) {
+  constexpr int kC = 16, kP = kC + 1;
+  __shared__ double chunk[64 * kP];
+  const int M = nR + 2;
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int j = blockIdx.y;
+  const int u0 = blockIdx.x * 64;
+  const int rows = min(64, nU - u0);
+  const int rsub = lane / kC, ksub = lane % kC;
+  const double *src = f + ((size_t)j * nU + u0) * nR;
+  double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  double v[kC];
+  // columns [e0, e0 + kC) of a (rows x width) array into v[] of the lane's row
+  auto gather = [&](const double *a, int width, int e0) {
+    const int e = e0 + ksub;
+#pragma unroll
+    for (int q = 0; q < kC; q++) {
+      const int row = min(q * 4 + rsub, rows - 1);
+      chunk[(q * 4 + rsub) * kP + ksub] = e < width ? a[(size_t)row * width + e] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kC; i++) v[i] = chunk[lane * kP + i];
+    __syncthreads();
+  };
+  auto scatter = [&](double *a, int width, int e0) {
+#pragma unroll
+    for (int i = 0; i < kC; i++) chunk[lane * kP + i] = v[i];
+    __syncthreads();
+    const int e = e0 + ksub;
+#pragma unroll
+    for (int q = 0; q < kC; q++) {
+      const int row = q * 4 + rsub;
+      if (row < rows && e < width) a[(size_t)row * width + e] = chunk[row * kP + ksub];
+    }
+    __syncthreads();
+  };
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double ym1 = ax.A03, ym2 = 0.0;
+  // forward elimination: rhs_k = f[k-1], y_k to padded column k
+  for (int e0 = 0; e0 < nR; e0 += kC) {
+    gather(src, nR, e0);
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+      const int k = e0 + i + 1;
+      if (k <= M - 2) {
+        double dv = dS, ev = rdS;
+        if (k <= kHead) {
+          dv = ax.d[k + lz];
+          ev = ax.rd[k + lz];
+        }
+        const double yk = div_known(v[i] - c16 * ym1, dv, ev);
+        v[i] = yk;
+        ym2 = ym1;
+        ym1 = yk;
+      }
+    }
+    scatter(dst, M, e0 + 1);
+  }
+  // last row, back substitution and first row, chunks of padded columns
+  double tt = 0.0 - ax.hi2 * ym2;
+  tt = tt - ax.p * ym1;
+  double cn = div_known(tt, ax.q, ax.rq);
+  const double uS = ax.uS + (double)lz;
+  double c1 = 0.0, c2 = cn;
+  for (int e0 = ((M - 1) / kC) * kC; e0 >= 0; e0 -= kC) {
+    gather(dst, M, e0);
+#pragma unroll
+    for (int i = kC - 1; i >= 0; i--) {
+      const int k = e0 + i;
+      if (k == M - 1) {
+        v[i] = cn;
+      } else if (k >= 1 && k <= M - 2) {
+        double uv = uS;
+        if (k <= kHead) uv = ax.u[k + lz];
+        const double ck = v[i] - uv * cn;
+        v[i] = ck;
+        c2 = cn;
+        cn = ck;
+        c1 = ck;
+      } else if (k == 0) {
+        v[i] = (ax.A03 - ax.A01 * c1) - ax.A02 * c2;
+      }
+    }
+    scatter(dst, M, e0);
+  }
+}
+
 // Whole-line solve in registers, for a line length M known at compile time.
 // Lane = line: all M-2 right-hand sides are loaded up front, the forward
 // elimination and the back substitution run on the register array y[] with
@@ -926,14 +1048,21 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
     int v = atoi(e);
     if (v >= 1 && v <= 64) r.TC = v;
   }
+  // lines too long for 64 of them in a tile: streaming solvers
+  // (VLS_SPLINE_STREAM=1 / 0 forces them on / off)
+  r.streamR = r.TR < 64;
+  r.streamV = r.TC < 64;
+  if (const char *e = getenv("VLS_SPLINE_STREAM")) r.streamR = r.streamV = atoi(e) != 0;
   const size_t ldsR = (size_t)r.LS * (r.nR + 2) * sizeof(double);
   const size_t ldsV = (size_t)r.TC * (r.nU + 2) * sizeof(double);
-  if (ldsR > (size_t)maxLdsPerBlock || ldsV > (size_t)maxLdsPerBlock)
+  if ((!r.streamR && ldsR > (size_t)maxLdsPerBlock) || (!r.streamV && ldsV > (size_t)maxLdsPerBlock))
     throw Error(VLS_ERR_ARG, "grid too large for the LDS of one workgroup");
-  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_r,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsR));
-  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_v,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsV));
+  if (!r.streamR)
+    VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_r,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsR));
+  if (!r.streamV)
+    VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_v,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsV));
 }
 
 // Row version of the register line solver with coalesced memory access.
@@ -1151,13 +1280,23 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
     time_mark(t, s, "spline_v");
     return;
   }
-  dim3 gr((r.nU + r.TR - 1) / r.TR, r.nJ);
-  hipLaunchKernelGGL(k_spline_r, gr, dim3(64), (size_t)r.LS * (r.nR + 2) * sizeof(double), s, src,
-                     r.c, r.nR, r.nU, r.nJ, r.cs, r.TR, r.LS, 0, r.tr);
+  if (r.streamR) {
+    hipLaunchKernelGGL(k_spline_r_stream, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c,
+                       r.nR, r.nU, r.cs, 0, r.tr);
+  } else {
+    dim3 gr((r.nU + r.TR - 1) / r.TR, r.nJ);
+    hipLaunchKernelGGL(k_spline_r, gr, dim3(64), (size_t)r.LS * (r.nR + 2) * sizeof(double), s,
+                       src, r.c, r.nR, r.nU, r.nJ, r.cs, r.TR, r.LS, 0, r.tr);
+  }
   time_mark(t, s, "spline_r");
-  dim3 gv((r.nR + 2 + r.TC - 1) / r.TC, r.nJ);
-  hipLaunchKernelGGL(k_spline_v, gv, dim3(64), (size_t)r.TC * (r.nU + 2) * sizeof(double), s,
-                     r.c, r.nR, r.nU, r.nJ, r.cs, r.TC, 0, r.tv);
+  if (r.streamV) {
+    hipLaunchKernelGGL(k_spline_v_stream, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, r.c,
+                       r.nR, r.cs, 0, r.tv);
+  } else {
+    dim3 gv((r.nR + 2 + r.TC - 1) / r.TC, r.nJ);
+    hipLaunchKernelGGL(k_spline_v, gv, dim3(64), (size_t)r.TC * (r.nU + 2) * sizeof(double), s,
+                       r.c, r.nR, r.nU, r.nJ, r.cs, r.TC, 0, r.tv);
+  }
   time_mark(t, s, "spline_v");
 }
 
