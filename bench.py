@@ -75,9 +75,10 @@ ADJ_STEPS, ADJ_CK = 2000, 50
 
 def adjoint_ratio(V):
     """wall(gradient of a mass-profile loss: recorded forward run with
-    checkpoints + reverse sweep recomputing each segment) / wall(forward)."""
-    import numpy as np
-
+    checkpoints + reverse sweep recomputing each segment) / wall(forward).
+    ratio: the first gradient, which also builds the drift's transposed foot
+    table on the host and allocates the checkpoints; ratio_steady: a second
+    one."""
     s = V.Solver(ADJ_WORKLOAD)
     ctx = s.context
     f0 = s.f()
@@ -90,18 +91,63 @@ def adjoint_ratio(V):
     ctx.step(ADJ_STEPS)
     ctx.synchronize()
     fwd = time.perf_counter() - t0
-    ctx.upload_f(0, f0)
-    ctx.put_chains(head0, vals0)
-    t0 = time.perf_counter()
-    ctx.record(ADJ_STEPS, ADJ_CK)
-    M = ctx.mass_profile()
-    g0, _ = ctx.adjoint(dL_dM=2 * (M - 1.01 * M))
-    grad = time.perf_counter() - t0
-    if not np.isfinite(g0).all():
-        raise RuntimeError("non-finite gradient")
+    grad = _gradient_wall(ctx, f0, head0, vals0, ADJ_STEPS, ADJ_CK)
+    grad2 = _gradient_wall(ctx, f0, head0, vals0, ADJ_STEPS, ADJ_CK)
     return {"workload": "PLDF n=1 eta=0.1, 129 log-r x 129 v x 64 quadratic-j, dt 0.005",
             "steps": ADJ_STEPS, "ck_every": ADJ_CK, "forward_s": round(fwd, 4),
             "gradient_s": round(grad, 4), "ratio": round(grad / fwd, 3),
+            "gradient_s_steady": round(grad2, 4), "ratio_steady": round(grad2 / fwd, 3),
+            "loss": "sum_r (M(<r) - M_target)^2 of the final state"}
+
+
+# The large adjoint workload: the Hernquist sphere (M=1, R0=1) on its default
+# ranges, 513 log-r x 513 v x 256 log-j nodes (67,371,264 points), dt=0.0025,
+# 4000 steps with checkpoints every 63 steps (about sqrt(4000)).  Opt-in
+# (--hernquist-adjoint): about a minute of GPU time and 80 GB of HBM.
+HQ_WORKLOAD = {"init.type": "Hernquist", "init.M": 1, "init.R0": 1, "grid.Pres": 512,
+               "grid.Ures": [512, 256], "solver.dt": 0.0025}
+HQ_STEPS, HQ_CK = 4000, 63
+
+
+def _gradient_wall(ctx, f0, head0, vals0, steps, ck):
+    import numpy as np
+
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, vals0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.record(steps, ck)
+    M = ctx.mass_profile()
+    g0, _ = ctx.adjoint(dL_dM=2 * (M - 1.01 * M))
+    wall = time.perf_counter() - t0
+    if not np.isfinite(g0).all():
+        raise RuntimeError("non-finite gradient")
+    return wall
+
+
+def adjoint_ratio_hernquist(V, steps=HQ_STEPS, ck=HQ_CK):
+    """The adjoint/forward wall ratio on the Hernquist workload, first and
+    steady gradient as in adjoint_ratio."""
+    s = V.Solver(HQ_WORKLOAD)
+    ctx = s.context
+    f0 = s.f()
+    head0, vals0 = ctx.get_chains()
+    ctx.step(3)
+    ctx.upload_f(0, f0)
+    ctx.put_chains(head0, vals0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    fwd = time.perf_counter() - t0
+    first = _gradient_wall(ctx, f0, head0, vals0, steps, ck)
+    second = _gradient_wall(ctx, f0, head0, vals0, steps, ck)
+    npts = s.nR * s.nU * s.nJ
+    return {"workload": "Hernquist M=1 R0=1, 513 log-r x 513 v x 256 log-j, dt 0.0025",
+            "points": npts, "steps": steps, "ck_every": ck, "forward_s": round(fwd, 3),
+            "forward_point_updates_per_s": npts * steps / fwd,
+            "gradient_s": round(first, 3), "ratio": round(first / fwd, 3),
+            "gradient_s_steady": round(second, 3), "ratio_steady": round(second / fwd, 3),
             "loss": "sum_r (M(<r) - M_target)^2 of the final state"}
 
 
@@ -216,6 +262,8 @@ def main():
                     help="steps of the CPU baseline sample (default 150, ~10-20 s of CPU work)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-run ensemble")
+    ap.add_argument("--hernquist-adjoint", action="store_true",
+                    help="also the adjoint ratio on the 513x513x256 Hernquist grid (~1 min)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -272,6 +320,10 @@ def main():
         except Exception as e:  # reported, not fatal for the forward figure
             ens = {"speedup": None, "error": str(e)[:200]}
 
+    hq = None
+    if args.hernquist_adjoint:
+        hq = adjoint_ratio_hernquist(V)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -292,6 +344,8 @@ def main():
         "adjoint": adj,
         "ensemble": ens,
     }
+    if hq is not None:
+        line["adjoint_hernquist"] = hq
     print(json.dumps(line))
 
 

@@ -88,6 +88,45 @@ __device__ __forceinline__ void eval_c_adj(double *cb, int st, double xr, double
   }
 }
 
+// The transposed moment quadrature factors over the cell pairs around a
+// point: 1/4 (cu(u-1) + cu(u)) (w(r-1) + w(r)), with cu(u) = Ct1[j] dU[u] and
+// w(r) = dR[r] dMb[r] for the cells that exist (u <= nU-2, r <= nR-2) and 0
+// otherwise.  mom_cu is a row's factor, mom_w a column's.
+__device__ __forceinline__ double mom_cu(int u, int nU, double ct, const double *__restrict__ dU) {
+  const double a = (u >= 1 && u - 1 <= nU - 2) ? ct * dU[u - 1] : 0.0;
+  const double b = (u >= 0 && u <= nU - 2) ? ct * dU[u] : 0.0;
+  return 0.25 * (a + b);
+}
+__device__ __forceinline__ double mom_w(int r, int nR, const double *__restrict__ dR,
+                                        const double *__restrict__ dMb) {
+  const double a = (r >= 1 && r - 1 <= nR - 2) ? dR[r - 1] * dMb[r - 1] : 0.0;
+  const double b = (r >= 0 && r <= nR - 2) ? dR[r] * dMb[r] : 0.0;
+  return a + b;
+}
+
+// Transposed moment quadrature at point (u, r) of a slice (the reverse of
+// the cell averages of k_moment_partial): a quarter of each adjacent cell's
+// weight Ct1[j] dU dR times that cell's dM adjoint.
+__device__ __forceinline__ double moment_adj(int u, int r, int nU, int nR, double ct,
+                                             const double *__restrict__ dU,
+                                             const double *__restrict__ dR,
+                                             const double *__restrict__ dMb) {
+  double s = 0.0;
+#pragma unroll
+  for (int du = -1; du <= 0; du++) {
+    const int uu = u + du;
+    if (uu < 0 || uu > nU - 2) continue;
+    const double cu = ct * dU[uu];
+#pragma unroll
+    for (int dr = -1; dr <= 0; dr++) {
+      const int rr = r + dr;
+      if (rr < 0 || rr > nR - 2) continue;
+      s += 0.25 * ((cu * dR[rr]) * dMb[rr]);
+    }
+  }
+  return s;
+}
+
 // Line adjoints for any line length M (the reverse of the LDS-tiled forward
 // solvers): the reverse-mode derivative of the Thomas elimination, run as two
 // streaming sweeps over the padded line, a chunk of kLC elements at a time.
@@ -159,7 +198,8 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
 // cb (a work array here), the descending one into out.
 __global__ void __launch_bounds__(64)
 k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64_t cs,
-        ThomasAxis ax) {
+        ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
+        const double *__restrict__ dU, const double *__restrict__ dR) {
   constexpr int kP = kLC + 1;
   __shared__ double chunk[64 * kP];
   const int M = nR + 2;
@@ -196,6 +236,7 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
     }
     __syncthreads();
   };
+  const double cu = dMb ? mom_cu(u0 + lane, nU, Ct1[j], dU) : 0.0;
   double c0 = 0.0, prev = 0.0;
   for (int m0 = 0; m0 < M; m0 += kLC) {
     gather(src, M, m0);
@@ -220,6 +261,11 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
 #pragma unroll
     for (int i = kLC - 1; i >= 0; i--)
       if (e0 + i < nR) v[i] = adj_down(ax, e0 + i + 1, v[i], tt, carry);
+    if (dMb) {  // plus the transposed moment quadrature of the lane's row
+#pragma unroll
+      for (int i = 0; i < kLC; i++)
+        if (e0 + i < nR) v[i] += cu * mom_w(e0 + i, nR, dR, dMb);
+    }
     scatter(dst, nR, e0);
   }
 }
@@ -275,20 +321,7 @@ k_adj_moment(double *__restrict__ fb, const double *__restrict__ dMb,
   const int j = (int)(i / nSlice);
   const int il = (int)(i - j * nSlice);
   const int u = il / nR, r = il - u * nR;
-  const double ct = Ct1[j];
-  double s = 0.0;
-#pragma unroll
-  for (int du = -1; du <= 0; du++) {
-    const int uu = u + du;
-    if (uu < 0 || uu > nU - 2) continue;
-    const double cu = ct * dU[uu];
-#pragma unroll
-    for (int dr = -1; dr <= 0; dr++) {
-      const int rr = r + dr;
-      if (rr < 0 || rr > nR - 2) continue;
-      s += 0.25 * ((cu * dR[rr]) * dMb[rr]);
-    }
-  }
+  const double s = moment_adj(u, r, nU, nR, Ct1[j], dU, dR, dMb);
   fb[i] = accumulate ? fb[i] + s : s;
 }
 
@@ -368,68 +401,66 @@ k_adj_chains(const double *__restrict__ fb, double *__restrict__ cb,
   if (!bout[k] && xb != 0.0) eval_c_adj(cb + (pt / nSlice) * cs, st, bxr[k], bxu[k], xb);
 }
 
-// kick^T, acceleration part: term[i] = fb*[i] hinv/36 sum_k B_k'(t) g_{n+k}
-__global__ void __launch_bounds__(kB)
-k_adj_kick_terms(const double *__restrict__ fsb, const double *__restrict__ c,
+// kick^T, acceleration part: abp[j][r] = hinv/36 sum_u fb*[j][u][r]
+// sum_k B_k'(t) g_{n+k}, g the column combination of the kick's coefficients.
+// A block takes 64 columns; its kRG waves take every kRG-th row u and add up
+// their rows, and the kRG partial sums are combined through LDS.
+constexpr int kRG = 16;
+__global__ void __launch_bounds__(64 * kRG)
+k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
                  const double *__restrict__ U_Val, const double *__restrict__ acc,
-                 double *__restrict__ term, int nR, int nU, int64_t cs, int64_t nVal, double Ulo,
-                 double Uhi, double start_v, double hinv_v) {
-  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (i >= nVal) return;
-  const int64_t nSlice = (int64_t)nU * nR;
-  const int64_t j = i / nSlice;
-  const int il = (int)(i - j * nSlice);
-  const int u = il / nR, r = il - u * nR;
-  const double lam = fsb[i];
-  double out = 0.0;
-  const double newU = U_Val[u] + acc[r];
-  if (lam != 0.0 && !(newU <= Ulo) && !(newU >= Uhi)) {
-    double t;
-    int n;
-    split((newU - start_v) * hinv_v, n, t);
-    const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
-    const int S = nR + 2;
-    const double *cj = c + j * cs;
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (n + k > nU + 1) break;  // past the padded column: weight 0
-      const double *rowc = cj + (int64_t)(n + k) * S + r;
-      s += dw[k] * ((rowc[0] + rowc[1] * 4) + rowc[2]);
-    }
-    out = lam * (hinv_v / 36.0) * s;
-  }
-  term[i] = out;
-}
-
-// abp[j][r] = sum over u of term[j][u][r]
-__global__ void __launch_bounds__(kB)
-k_adj_kick_acc(const double *__restrict__ term, double *__restrict__ abp, int nR, int nU) {
-  const int r = blockIdx.x * kB + threadIdx.x;
+                 double *__restrict__ abp, int nR, int nU, int64_t cs, double Ulo, double Uhi,
+                 double start_v, double hinv_v) {
+  __shared__ double part[kRG][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
   const int j = blockIdx.y;
-  if (r >= nR) return;
-  const double *t = term + (int64_t)j * nU * nR + r;
-  constexpr int kU = 16;
+  const int S = nR + 2;
   double s = 0.0;
-  int u = 0;
-  for (; u + kU <= nU; u += kU) {
-    double v[kU];
+  if (r < nR) {
+    const double a = acc[r];
+    const double *fj = fsb + (int64_t)j * nU * nR + r;
+    const double *cj = c + j * cs + r;
+    // straight-line body (points outside the v range read row 0 and count
+    // 0), two rows per trip so that their loads overlap
+#pragma unroll 2
+    for (int u = g; u < nU; u += kRG) {
+      const double lam = fj[(int64_t)u * nR];
+      const double newU = U_Val[u] + a;
+      const bool in = !(newU <= Ulo) && !(newU >= Uhi);
+      double t;
+      int n;
+      split(in ? (newU - start_v) * hinv_v : 0.0, n, t);
+      const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
+      double sk = 0.0;
 #pragma unroll
-    for (int q = 0; q < kU; q++) v[q] = t[(int64_t)(u + q) * nR];
-#pragma unroll
-    for (int q = 0; q < kU; q++) s += v[q];
+      for (int k = 0; k < 4; k++) {
+        // past the padded column the weight is 0: read the last row instead
+        const double *rowc = cj + (int64_t)min(n + k, nU + 1) * S;
+        const double gk = (rowc[0] + rowc[1] * 4) + rowc[2];
+        sk += n + k <= nU + 1 ? dw[k] * gk : 0.0;
+      }
+      s += in ? lam * sk : 0.0;
+    }
   }
-  for (; u < nU; u++) s += t[(int64_t)u * nR];
-  abp[(int64_t)j * nR + r] = s;
+  part[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && r < nR) {
+    double t = part[0][lane];
+#pragma unroll
+    for (int q = 1; q < kRG; q++) t += part[q][lane];
+    abp[(int64_t)j * nR + r] = t * (hinv_v / 36.0);
+  }
 }
 
 // kick^T, coefficient part: cb[j][m][k] = g(m, k) + 4 g(m, k-1) + g(m, k-2),
 // g(m, r) = sum over the points u of column r whose rows n..n+3 include m of
-// fb*[u] B_{m-n}(t) / 36.  The target row of point u is n = u + a_r hinv, up
-// to rounding and to the v nodes' offsets from the knots (at most a quarter
-// of a cell, checked when the run is recorded), so the points of row m are
-// among the kW = 7 values of u from floor(m - 3 - a_r hinv) - 1 on.  One block
-// per coefficient row (m, j): g(m, .) into LDS, then the column combination.
+// fb*[u] B_{m-n}(t) / 36.  The target row of point u is n = floor(u + a_r hinv
+// + d_u), d_u the v node's offset from its knot plus rounding, below 0.01 of
+// a cell (checked when the run is recorded), so the points of row m satisfy
+// m - 3 - a_r hinv - 0.01 < u < m + 1 - a_r hinv + 0.01: the kW = 5 values
+// of u from floor(m - 3 - a_r hinv - 0.01) + 1 on.  One block per
+// coefficient row (m, j): g(m, .) into LDS, then the column combination.
 __global__ void __launch_bounds__(kB)
 k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val,
                 const double *__restrict__ acc, double *__restrict__ cb, int nR, int nU,
@@ -441,8 +472,8 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     const double a = acc[r];
     const double sh = a * hinv_v;
-    const int ulo = (int)floor((double)m - 3.0 - sh) - 1;
-    constexpr int kW = 7;
+    const int ulo = (int)floor((double)m - 3.0 - sh - 0.01) + 1;
+    constexpr int kW = 5;
     double lam[kW], uu[kW];
 #pragma unroll
     for (int q = 0; q < kW; q++) {
@@ -529,7 +560,9 @@ k_adj_v_reg(double *__restrict__ cb, int64_t cs, int lzmask, ThomasAxis ax) {
 template <int M>
 __global__ void __launch_bounds__(64)
 k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int64_t cs,
-           int lzmask, ThomasAxis ax) {
+           int lzmask, ThomasAxis ax, const double *__restrict__ dMb,
+           const double *__restrict__ Ct1, const double *__restrict__ dU,
+           const double *__restrict__ dR) {
   constexpr int kC = 16, kP = kC + 1;
   constexpr int nR = M - 2;
   constexpr int nChIn = (M + kC - 1) / kC, nChOut = (nR + kC - 1) / kC;
@@ -543,6 +576,7 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
   const double *src = cb + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double *dst = out + ((size_t)j * nU + u0) * nR;
   const double uS = ax.uS + (double)lz, rdS = ax.rdS + (double)lz;
+  const double cu = dMb ? mom_cu(u0 + lane, nU, Ct1[j], dU) : 0.0;
   double c[M];
   double tmp[kC];
   auto load_chunk = [&](int ch) {
@@ -587,6 +621,11 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
         if (k >= 2) c[k - 1] = fma(-ax.c16, c[k], c[k - 1]);
       }
     }
+    if (dMb) {  // plus the transposed moment quadrature of the lane's row
+#pragma unroll
+      for (int i = 0; i < kC; i++)
+        if (ch * kC + i < nR) c[ch * kC + i + 1] += cu * mom_w(ch * kC + i, nR, dR, dMb);
+    }
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nR) chunk[lane * kP + i] = c[ch * kC + i + 1];
@@ -618,7 +657,8 @@ void launch_fill(double *p, int64_t n, double v, hipStream_t s) {
 
 // register line adjoints for the shapes of the forward's register solvers
 using AdjV = void (*)(double *, int64_t, int, ThomasAxis);
-using AdjR = void (*)(const double *, double *, int, int64_t, int, ThomasAxis);
+using AdjR = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, const double *,
+                      const double *, const double *, const double *);
 struct AdjShape {
   int nR, nU;
   AdjV v;
@@ -650,34 +690,38 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
                        r.nChains, (int64_t)r.nU * r.nR, r.cs, r.nR + 2);
 }
 
-void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
+void launch_adj_spline(Run &r, double *cb, double *out, const double *dMb, hipStream_t s) {
   if (const AdjShape *a = adj_shape(r)) {
     hipLaunchKernelGGL(a->v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.cs, 0, r.tv);
     hipLaunchKernelGGL(a->r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nU, r.cs, 0,
-                       r.tr);
+                       r.tr, dMb, r.Ct1, r.dU, r.dR);
     return;
   }
   hipLaunchKernelGGL(k_adj_v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.nR + 2, r.cs,
                      r.tv);
   hipLaunchKernelGGL(k_adj_r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nR, r.nU,
-                     r.cs, r.tr);
+                     r.cs, r.tr, dMb, r.Ct1, r.dU, r.dR);
 }
 
-void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *term,
-                     double *cb, double *abp, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_kick_terms, dim3(nblocks(r.nVal)), dim3(kB), 0, s, fsb, cK, r.U_Val,
-                     accK, term, r.nR, r.nU, r.cs, r.nVal, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
-  hipLaunchKernelGGL(k_adj_kick_acc, dim3(nblocks(r.nR), r.nJ), dim3(kB), 0, s, term, abp, r.nR,
-                     r.nU);
-  const int bs = std::min(kB, (r.nR + 2 + 63) / 64 * 64);
+void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
+                     double *abp, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64, r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
+                     r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  // the fewest passes of at most kB threads over the row, split evenly
+  const int passes = (r.nR + 2 + kB - 1) / kB;
+  const int bs = ((r.nR + 2 + passes - 1) / passes + 63) / 64 * 64;
   hipLaunchKernelGGL(k_adj_kick_coef, dim3(r.nU + 2, r.nJ), dim3(bs), r.nR * sizeof(double), s, fsb,
                      r.U_Val, accK, cb, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
-void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
-                     bool accumulate, hipStream_t s) {
+void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_scan, dim3(1), dim3(256), r.nR * sizeof(double), s, abp, MbarIn,
                      r.gfac, dMb, r.nR, r.nJ);
+}
+
+void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
+                     bool accumulate, hipStream_t s) {
+  launch_adj_scan(r, abp, MbarIn, dMb, s);
   hipLaunchKernelGGL(k_adj_moment, dim3(nblocks(r.nVal)), dim3(kB), 0, s, fb, dMb, r.Ct1, r.dU,
                      r.dR, r.nR, r.nU, r.nVal, accumulate ? 1 : 0);
 }

@@ -710,9 +710,10 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
     if (r.sp.no_linking)
       throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
     // the kick's transpose finds the points of a coefficient row from
-    // n ~ u + a hinv, which needs v nodes on the uniform spline knots
+    // n ~ u + a hinv, which needs v nodes on the uniform spline knots (to
+    // 0.01 of a cell; a linear v scale is there to rounding)
     for (int u = 0; u < r.nU; u++)
-      if (std::fabs((r.hU[u] - r.start_v) * r.hinv_v - u) > 0.25)
+      if (std::fabs((r.hU[u] - r.start_v) * r.hinv_v - u) > 0.01)
         throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint needs a uniform v grid");
     hipStream_t s = ctx->c.stream;
     auto &A = r.adj;
@@ -724,7 +725,6 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.fb = dalloc<double>(r, nv);
       A.fsb = dalloc<double>(r, nv);
       A.cb = dalloc<double>(r, coef);
-      A.gb = dalloc<double>(r, coef);
       A.abp = dalloc<double>(r, (size_t)r.nJ * r.nR);
       A.dMb = dalloc<double>(r, r.nR);
       A.Mbar = dalloc<double>(r, r.nR);
@@ -826,10 +826,10 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
       for (int n = n1 - 1; n >= n0; n--) {
         const double *cK = A.segC + coef * (n - n0), *accK = A.segAcc + (size_t)r.nR * (n - n0);
         vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
-        vls::launch_adj_spline(r, A.cb, A.fsb, s);
-        vls::launch_adj_kick(r, A.fsb, cK, accK, A.gb, A.cb, A.abp, s);
-        vls::launch_adj_spline(r, A.cb, A.fb, s);
-        vls::launch_adj_mass(r, A.abp, nullptr, A.dMb, A.fb, true, s);
+        vls::launch_adj_spline(r, A.cb, A.fsb, nullptr, s);
+        vls::launch_adj_kick(r, A.fsb, cK, accK, A.cb, A.abp, s);
+        vls::launch_adj_scan(r, A.abp, nullptr, A.dMb, s);
+        vls::launch_adj_spline(r, A.cb, A.fb, A.dMb, s);  // plus the moment's transpose
       }
     }
     VLS_HIP_CHECK(hipGetLastError());

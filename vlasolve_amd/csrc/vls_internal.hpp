@@ -97,7 +97,7 @@ struct Run {
     int32_t *endHead = nullptr;
     double *segC = nullptr;     // [capSeg][coef] the kick coefficients of each state
     double *segAcc = nullptr;   // [capSeg][nR] the kick accelerations
-    double *fb = nullptr, *fsb = nullptr, *cb = nullptr, *gb = nullptr;
+    double *fb = nullptr, *fsb = nullptr, *cb = nullptr;
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
     int64_t *tileOff = nullptr;  // transposed foot table of the drift, per coefficient tile
     uint32_t *tilePts = nullptr;
@@ -145,9 +145,12 @@ void time_mark(Timing *t, hipStream_t s, const char *name);
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s);
-void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
-void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *term,
-                     double *cb, double *abp, hipStream_t s);
+// transposed spline solves, cb (destroyed) -> out; with dMb, the transposed
+// moment quadrature of dMb is added to out
+void launch_adj_spline(Run &r, double *cb, double *out, const double *dMb, hipStream_t s);
+void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
+                     double *abp, hipStream_t s);
+void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s);
 void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
                        std::vector<int64_t> &off, std::vector<uint32_t> &pts);
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
