@@ -13,13 +13,14 @@
 //   spline^T  the transposed natural-spline solves, v lines then r rows, by
 //             reverse-mode differentiation of the Thomas elimination;
 //   kick^T    per point, the acceleration term fb* hinv/36 sum_k B_k'(t)
-//             g_{n+k}, summed over u per (j, r); per coefficient, the points
-//             whose four target rows include it (a window of u), through
-//             the column combination (1, 4, 1);
-//   spline^T  again, giving the direct part of f_n's adjoint;
+//             g_{n+k}, summed over u per (j, r); the direct part through the
+//             kick's equivalent 1D v-spline of each column (see
+//             k_adj_kick_coef): per row of the column, the points whose four
+//             target rows include it (a window of u), then the transposed
+//             v-solve, giving the direct part of f_n's adjoint;
 //   scan^T    a-bar * G/R^2 dt summed over j, then the suffix sums of the
 //             exclusive scan, then the transposed moment quadrature, added to
-//             f_n's adjoint.
+//             f_n's adjoint as the column solve stores it.
 // None of this has a counterpart in the reference, which has no derivative;
 // the forward phases it differentiates are those of
 // vlasovSolver21_spline::makeOneStep (src/vlasovSolver21_spline.hxx:680-801).
@@ -27,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "vls_internal.hpp"
@@ -62,6 +64,13 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
   double ip = trunc(x);
   t = x - ip;
   n = (int)ip;
+}
+
+__global__ void __launch_bounds__(kB)
+k_interleave(const double *__restrict__ a, const double *__restrict__ b, double2 *__restrict__ ab,
+             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n) ab[i] = make_double2(a[i], b[i]);
 }
 
 __global__ void __launch_bounds__(kB) k_fill(double *__restrict__ p, int64_t n, double v) {
@@ -192,14 +201,65 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
   }
 }
 
+// The kick's transposed v-solve for any column length (see k_adj_gcol):
+// the two streaming sweeps of k_adj_v over g, the descending one writing to
+// fb instead of in place.
+__global__ void __launch_bounds__(64)
+k_adj_gcol_stream(double *__restrict__ g, double *__restrict__ fb, int nR, int nU, int64_t cs,
+                  ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
+                  const double *__restrict__ dU, const double *__restrict__ dR) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (r >= nR) return;
+  const int M = ax.M;
+  double *col = g + j * cs + r;
+  double v[kLC];
+  double c0 = 0.0, prev = 0.0;
+  for (int m0 = 0; m0 < M; m0 += kLC) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m0 + i < M) v[i] = col[(int64_t)(m0 + i) * nR];
+#pragma unroll
+    for (int i = 0; i < kLC; i++) {
+      const int t = m0 + i;
+      if (t < M) {
+        if (t == 0) c0 = v[i];
+        if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
+        if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
+        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+        prev = v[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m0 + i < M) col[(int64_t)(m0 + i) * nR] = v[i];
+  }
+  const double tt = prev * ax.rq;
+  const double ct = dMb ? Ct1[j] : 0.0, w = dMb ? mom_w(r, nR, dR, dMb) : 0.0;
+  double *out = fb + (int64_t)j * nU * nR + r;
+  double carry = 0.0;
+  for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m1 - i >= 1) v[i] = col[(int64_t)(m1 - i) * nR];
+#pragma unroll
+    for (int i = 0; i < kLC; i++) {
+      const int m = m1 - i;
+      if (m >= 1) {
+        v[i] = adj_down(ax, m, v[i], tt, carry);
+        out[(int64_t)(m - 1) * nR] = dMb ? v[i] + mom_cu(m - 1, nU, ct, dU) * w : v[i];
+      }
+    }
+  }
+}
+
 // r rows (lane = padded row u+1 of 64 consecutive rows), rhs-bar of row u+1
 // to out[j][u][.]: chunks of kLC columns through an LDS transpose, so that
 // the global accesses are row segments; the ascending sweep writes back into
 // cb (a work array here), the descending one into out.
 __global__ void __launch_bounds__(64)
 k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64_t cs,
-        ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
-        const double *__restrict__ dU, const double *__restrict__ dR) {
+        ThomasAxis ax) {
   constexpr int kP = kLC + 1;
   __shared__ double chunk[64 * kP];
   const int M = nR + 2;
@@ -236,7 +296,6 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
     }
     __syncthreads();
   };
-  const double cu = dMb ? mom_cu(u0 + lane, nU, Ct1[j], dU) : 0.0;
   double c0 = 0.0, prev = 0.0;
   for (int m0 = 0; m0 < M; m0 += kLC) {
     gather(src, M, m0);
@@ -261,11 +320,6 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
 #pragma unroll
     for (int i = kLC - 1; i >= 0; i--)
       if (e0 + i < nR) v[i] = adj_down(ax, e0 + i + 1, v[i], tt, carry);
-    if (dMb) {  // plus the transposed moment quadrature of the lane's row
-#pragma unroll
-      for (int i = 0; i < kLC; i++)
-        if (e0 + i < nR) v[i] += cu * mom_w(e0 + i, nR, dR, dMb);
-    }
     scatter(dst, nR, e0);
   }
 }
@@ -453,22 +507,30 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
   }
 }
 
-// kick^T, coefficient part: cb[j][m][k] = g(m, k) + 4 g(m, k-1) + g(m, k-2),
-// g(m, r) = sum over the points u of column r whose rows n..n+3 include m of
-// fb*[u] B_{m-n}(t) / 36.  The target row of point u is n = floor(u + a_r hinv
-// + d_u), d_u the v node's offset from its knot plus rounding, below 0.01 of
-// a cell (checked when the run is recorded), so the points of row m satisfy
-// m - 3 - a_r hinv - 0.01 < u < m + 1 - a_r hinv + 0.01: the kW = 5 values
-// of u from floor(m - 3 - a_r hinv - 0.01) + 1 on.  One block per
-// coefficient row (m, j): g(m, .) into LDS, then the column combination.
+// kick^T, coefficient part.  The kick reads the 2D coefficients c only
+// through the column combination Q c = (c[.][r] + 4 c[.][r+1] + c[.][r+2]) / 6,
+// which is exactly the interior rows of the r-direction spline system, so
+// Q R^-1 = identity at the nodes and, the r and v solves commuting, Q c =
+// V^-1 f: the kick is the 1D natural v-spline of each column of f, evaluated
+// with the weights B_k(t) / 6 (the forward computes it through the 2D
+// spline, as the reference does; the two agree to rounding).  Its transpose
+// therefore needs no r-direction adjoint: g(m, r) = sum over the points u of
+// column r whose rows n..n+3 include m of fb*[u] B_{m-n}(t) / 6, then the
+// transposed v-solve of each column (k_adj_gcol).
+//
+// g goes to the coefficient buffer as [j][m][r] (slice stride cs, row
+// stride nR).  The target row of point u is n = floor(u + a_r hinv + d_u),
+// d_u the v node's offset from its knot plus rounding, below 0.01 of a cell
+// (checked when the run is recorded), so the points of row m satisfy
+// m - 3 - a_r hinv - 0.01 < u < m + 1 - a_r hinv + 0.01: the kW = 5 values of
+// u from floor(m - 3 - a_r hinv - 0.01) + 1 on.  One block per row (m, j).
 __global__ void __launch_bounds__(kB)
 k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val,
-                const double *__restrict__ acc, double *__restrict__ cb, int nR, int nU,
+                const double *__restrict__ acc, double *__restrict__ g, int nR, int nU,
                 int64_t cs, double Ulo, double Uhi, double start_v, double hinv_v) {
-  extern __shared__ double gl[];  // [nR]
   const int m = blockIdx.x, j = blockIdx.y;
-  const int S = nR + 2;
   const double *fj = fsb + (int64_t)j * nU * nR;
+  double *gm = g + (int64_t)j * cs + (int64_t)m * nR;
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     const double a = acc[r];
     const double sh = a * hinv_v;
@@ -481,7 +543,7 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
       lam[q] = fj[(int64_t)u * nR + r];
       uu[q] = U_Val[u];
     }
-    double g = 0.0;
+    double acc6 = 0.0;
 #pragma unroll
     for (int q = 0; q < kW; q++) {
       const int u = ulo + q;
@@ -493,17 +555,9 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
       split((newU - start_v) * hinv_v, n, t);
       const int k4 = m - n;
       if (k4 < 0 || k4 > 3) continue;
-      g += lam[q] * bweight(k4, t) * (1.0 / 36.0);
+      acc6 += lam[q] * bweight(k4, t);
     }
-    gl[r] = g;
-  }
-  __syncthreads();
-  double *row = cb + (int64_t)j * cs + (int64_t)m * S;
-  for (int k = threadIdx.x; k < S; k += blockDim.x) {
-    double out = k < nR ? gl[k] : 0.0;
-    if (k >= 1 && k - 1 < nR) out += 4.0 * gl[k - 1];
-    if (k >= 2) out += gl[k - 2];
-    row[k] = out;
+    gm[r] = acc6 * (1.0 / 6.0);
   }
 }
 
@@ -549,6 +603,35 @@ k_adj_v_reg(double *__restrict__ cb, int64_t cs, int lzmask, ThomasAxis ax) {
   for (int m = 1; m <= M - 2; m++) col[(int64_t)m * S] = c[m];
 }
 
+// The kick's transposed v-solve: the columns r of g ([j][m][r], M = nU + 2
+// rows) in registers, their rows 1..nU to fb[j][u][r] plus, with dMb, the
+// transposed moment quadrature.
+template <int M>
+__global__ void __launch_bounds__(64)
+k_adj_gcol(const double *__restrict__ g, double *__restrict__ fb, int nR, int64_t cs, int lzmask,
+           ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
+           const double *__restrict__ dU, const double *__restrict__ dR) {
+  constexpr int nU = M - 2;
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  const int lz = threadIdx.x & lzmask;
+  if (r >= nR) return;
+  const double *col = g + j * cs + r;
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) c[m] = col[(int64_t)m * nR];
+  thomas_adjoint_reg<M>(ax, c, lz);
+  double *out = fb + (int64_t)j * nU * nR + r;
+  if (dMb) {
+    const double ct = Ct1[j], w = mom_w(r, nR, dR, dMb);
+#pragma unroll
+    for (int u = 0; u < nU; u++) out[(int64_t)u * nR] = c[u + 1] + mom_cu(u, nU, ct, dU) * w;
+  } else {
+#pragma unroll
+    for (int u = 0; u < nU; u++) out[(int64_t)u * nR] = c[u + 1];
+  }
+}
+
 // r-row adjoint with coalesced memory access, the transpose of k_line_rows:
 // a wave takes 64 consecutive rows u0.. of slice j.  The padded coefficient
 // rows of the adjoint arrive in chunks of kC columns through a small LDS
@@ -560,9 +643,7 @@ k_adj_v_reg(double *__restrict__ cb, int64_t cs, int lzmask, ThomasAxis ax) {
 template <int M>
 __global__ void __launch_bounds__(64)
 k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int64_t cs,
-           int lzmask, ThomasAxis ax, const double *__restrict__ dMb,
-           const double *__restrict__ Ct1, const double *__restrict__ dU,
-           const double *__restrict__ dR) {
+           int lzmask, ThomasAxis ax) {
   constexpr int kC = 16, kP = kC + 1;
   constexpr int nR = M - 2;
   constexpr int nChIn = (M + kC - 1) / kC, nChOut = (nR + kC - 1) / kC;
@@ -576,7 +657,6 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
   const double *src = cb + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double *dst = out + ((size_t)j * nU + u0) * nR;
   const double uS = ax.uS + (double)lz, rdS = ax.rdS + (double)lz;
-  const double cu = dMb ? mom_cu(u0 + lane, nU, Ct1[j], dU) : 0.0;
   double c[M];
   double tmp[kC];
   auto load_chunk = [&](int ch) {
@@ -621,11 +701,6 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
         if (k >= 2) c[k - 1] = fma(-ax.c16, c[k], c[k - 1]);
       }
     }
-    if (dMb) {  // plus the transposed moment quadrature of the lane's row
-#pragma unroll
-      for (int i = 0; i < kC; i++)
-        if (ch * kC + i < nR) c[ch * kC + i + 1] += cu * mom_w(ch * kC + i, nR, dR, dMb);
-    }
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nR) chunk[lane * kP + i] = c[ch * kC + i + 1];
@@ -651,20 +726,28 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
 
 }  // namespace
 
+void launch_interleave(const double *a, const double *b, double *ab, int64_t n, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_interleave, dim3(nblocks(n)), dim3(kB), 0, s, a, b, (double2 *)ab, n);
+}
+
 void launch_fill(double *p, int64_t n, double v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kB), 0, s, p, n, v);
 }
 
 // register line adjoints for the shapes of the forward's register solvers
 using AdjV = void (*)(double *, int64_t, int, ThomasAxis);
-using AdjR = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, const double *,
+using AdjR = void (*)(const double *, double *, int, int64_t, int, ThomasAxis);
+using AdjG = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, const double *,
                       const double *, const double *, const double *);
 struct AdjShape {
   int nR, nU;
   AdjV v;
   AdjR r;
+  AdjG gcol;
 };
-#define VLS_ADJ_SHAPE(R, U) {R, U, k_adj_v_reg<(U) + 2, (R) + 2>, k_adj_rows<(R) + 2>}
+#define VLS_ADJ_SHAPE(R, U) \
+  {R, U, k_adj_v_reg<(U) + 2, (R) + 2>, k_adj_rows<(R) + 2>, k_adj_gcol<(U) + 2>}
 static const AdjShape kAdjShapes[] = {
     VLS_ADJ_SHAPE(17, 17),   VLS_ADJ_SHAPE(25, 13),   VLS_ADJ_SHAPE(33, 33),
     VLS_ADJ_SHAPE(49, 49),   VLS_ADJ_SHAPE(65, 65),   VLS_ADJ_SHAPE(129, 129),
@@ -690,17 +773,17 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
                        r.nChains, (int64_t)r.nU * r.nR, r.cs, r.nR + 2);
 }
 
-void launch_adj_spline(Run &r, double *cb, double *out, const double *dMb, hipStream_t s) {
+void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
   if (const AdjShape *a = adj_shape(r)) {
     hipLaunchKernelGGL(a->v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.cs, 0, r.tv);
     hipLaunchKernelGGL(a->r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nU, r.cs, 0,
-                       r.tr, dMb, r.Ct1, r.dU, r.dR);
+                       r.tr);
     return;
   }
   hipLaunchKernelGGL(k_adj_v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.nR + 2, r.cs,
                      r.tv);
   hipLaunchKernelGGL(k_adj_r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nR, r.nU,
-                     r.cs, r.tr, dMb, r.Ct1, r.dU, r.dR);
+                     r.cs, r.tr);
 }
 
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
@@ -708,10 +791,21 @@ void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *
   hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64, r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
                      r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
   // the fewest passes of at most kB threads over the row, split evenly
-  const int passes = (r.nR + 2 + kB - 1) / kB;
-  const int bs = ((r.nR + 2 + passes - 1) / passes + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_adj_kick_coef, dim3(r.nU + 2, r.nJ), dim3(bs), r.nR * sizeof(double), s, fsb,
-                     r.U_Val, accK, cb, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  const int passes = (r.nR + kB - 1) / kB;
+  const int bs = ((r.nR + passes - 1) / passes + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_adj_kick_coef, dim3(r.nU + 2, r.nJ), dim3(bs), 0, s, fsb, r.U_Val, accK, cb,
+                     r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+}
+
+void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipStream_t s) {
+  const dim3 grid((r.nR + 63) / 64, r.nJ);
+  if (const AdjShape *a = adj_shape(r)) {
+    hipLaunchKernelGGL(a->gcol, grid, dim3(64), 0, s, g, fb, r.nR, r.cs, 0, r.tv, dMb, r.Ct1,
+                       r.dU, r.dR);
+    return;
+  }
+  hipLaunchKernelGGL(k_adj_gcol_stream, grid, dim3(64), 0, s, g, fb, r.nR, r.nU, r.cs, r.tv, dMb,
+                     r.Ct1, r.dU, r.dR);
 }
 
 void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s) {
@@ -736,30 +830,40 @@ void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<
   const int64_t nSlice = (int64_t)r.nU * r.nR;
   const int S = r.nR + 2, Mv = r.nU + 2;
   const int ntm = (Mv + kTile - 1) / kTile, ntk = (S + kTile - 1) / kTile;
-  const int64_t nTiles = (int64_t)r.nJ * ntm * ntk;
+  const int64_t perSlice = (int64_t)ntm * ntk, nTiles = r.nJ * perSlice;
   if (nSlice >= (int64_t)1 << 32) throw Error(VLS_ERR_ARG, "slice too large for the adjoint's tables");
   off.assign(nTiles + 1, 0);
-  auto each = [&](auto &&fn) {
-    for (int64_t j = 0; j < r.nJ; j++)
-      for (int64_t p = 0; p < nSlice; p++) {
-        const int64_t i = j * nSlice + p;
-        const double x = xr[i];
-        if (x <= -1.0) continue;  // outside the grid or a delay line
-        const double ix = std::trunc(x), iy = std::trunc(xu[i]);
-        const int nx = (int)ix, ny = (int)iy;
-        // last stencil row / column: weight BV2(0) = 0 at a fraction of 0
-        const int rowHi = std::min(ny + (xu[i] - iy == 0.0 ? 2 : 3), Mv - 1);
-        const int colHi = std::min(nx + (x - ix == 0.0 ? 2 : 3), S - 1);
-        for (int tm = ny / kTile; tm <= rowHi / kTile; tm++)
-          for (int tk = nx / kTile; tk <= colHi / kTile; tk++)
-            fn((j * ntm + tm) * ntk + tk, (uint32_t)p);
-      }
+  // the tiles of slice j that point p of the slice touches
+  auto each = [&](int64_t j, auto &&fn) {
+    for (int64_t p = 0; p < nSlice; p++) {
+      const int64_t i = j * nSlice + p;
+      const double x = xr[i];
+      if (x <= -1.0) continue;  // outside the grid or a delay line
+      const double ix = std::trunc(x), iy = std::trunc(xu[i]);
+      const int nx = (int)ix, ny = (int)iy;
+      // last stencil row / column: weight BV2(0) = 0 at a fraction of 0
+      const int rowHi = std::min(ny + (xu[i] - iy == 0.0 ? 2 : 3), Mv - 1);
+      const int colHi = std::min(nx + (x - ix == 0.0 ? 2 : 3), S - 1);
+      for (int tm = ny / kTile; tm <= rowHi / kTile; tm++)
+        for (int tk = nx / kTile; tk <= colHi / kTile; tk++)
+          fn((j * ntm + tm) * ntk + tk, (uint32_t)p);
+    }
   };
-  each([&](int64_t t, uint32_t) { off[t + 1]++; });
+  // slices are independent: both passes run over the host's threads
+  auto over_slices = [&](auto &&body) {
+    const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 32));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+      th.emplace_back([&, t] {
+        for (int64_t j = t; j < r.nJ; j += nt) body(j);
+      });
+    for (auto &x : th) x.join();
+  };
+  over_slices([&](int64_t j) { each(j, [&](int64_t t, uint32_t) { off[t + 1]++; }); });
   for (int64_t t = 0; t < nTiles; t++) off[t + 1] += off[t];
   pts.assign(off[nTiles], 0);
   std::vector<int64_t> cur(off.begin(), off.end() - 1);
-  each([&](int64_t t, uint32_t p) { pts[cur[t]++] = p; });
+  over_slices([&](int64_t j) { each(j, [&](int64_t t, uint32_t p) { pts[cur[t]++] = p; }); });
 }
 
 }  // namespace vls

@@ -754,21 +754,16 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       std::vector<int64_t> off;
       std::vector<uint32_t> pts;
       vls::build_drift_tiles(r, r.hxr, r.hxu, off, pts);
-      std::vector<double> xy(2 * nv);
-      for (size_t i = 0; i < nv; i++) {
-        xy[2 * i] = r.hxr[i];
-        xy[2 * i + 1] = r.hxu[i];
-      }
       VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
       dfree(r, A.tileOff);
       dfree(r, A.tilePts);
       dfree(r, A.footXY);
       A.tileOff = dalloc<int64_t>(r, off.size());
       A.tilePts = dalloc<uint32_t>(r, pts.size());
-      A.footXY = dalloc<double>(r, xy.size());
+      A.footXY = dalloc<double>(r, 2 * nv);
       h2d(A.tileOff, off.data(), off.size(), s);
       h2d(A.tilePts, pts.data(), pts.size(), s);
-      h2d(A.footXY, xy.data(), xy.size(), s);
+      vls::launch_interleave(r.xr, r.xu, A.footXY, (int64_t)nv, s);
       VLS_HIP_CHECK(hipStreamSynchronize(s));
       A.tilesValid = true;
     }
@@ -826,10 +821,10 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
       for (int n = n1 - 1; n >= n0; n--) {
         const double *cK = A.segC + coef * (n - n0), *accK = A.segAcc + (size_t)r.nR * (n - n0);
         vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
-        vls::launch_adj_spline(r, A.cb, A.fsb, nullptr, s);
+        vls::launch_adj_spline(r, A.cb, A.fsb, s);
         vls::launch_adj_kick(r, A.fsb, cK, accK, A.cb, A.abp, s);
         vls::launch_adj_scan(r, A.abp, nullptr, A.dMb, s);
-        vls::launch_adj_spline(r, A.cb, A.fb, A.dMb, s);  // plus the moment's transpose
+        vls::launch_adj_kick_cols(r, A.cb, A.fb, A.dMb, s);  // plus the moment's transpose
       }
     }
     VLS_HIP_CHECK(hipGetLastError());

@@ -143,14 +143,18 @@ void time_mark(Timing *t, hipStream_t s, const char *name);
 
 // adjoint launchers (vls_adjoint.hip)
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
+// ab[2i] = a[i], ab[2i+1] = b[i]
+void launch_interleave(const double *a, const double *b, double *ab, int64_t n, hipStream_t s);
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s);
-// transposed spline solves, cb (destroyed) -> out; with dMb, the transposed
-// moment quadrature of dMb is added to out
-void launch_adj_spline(Run &r, double *cb, double *out, const double *dMb, hipStream_t s);
+// transposed spline solves, cb (destroyed) -> out
+void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
                      double *abp, hipStream_t s);
 void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s);
+// the kick's transposed v-solve of g (left in cb by launch_adj_kick, destroyed)
+// into fb, plus the transposed moment quadrature of dMb when given
+void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipStream_t s);
 void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
                        std::vector<int64_t> &off, std::vector<uint32_t> &pts);
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
