@@ -25,7 +25,7 @@ LIB_PATH = HERE / "liboracle.so"
 REF_EXE = HERE / "_ref" / "vlasolve"
 
 SCALES = {"linear": 0, "logarithmic": 1, "quadratic": 2, "cubic": 3, "quartic": 4}
-INITS = {"UDF": 0, "PLDF": 1, "Hernquist": 2}
+INITS = {"UDF": 0, "PLDF": 1, "Hernquist": 2, "Plummer": 3}
 KERNELS = {"delayed": 0, "reflective": 1}
 
 
@@ -131,9 +131,9 @@ def resolve(cfg: dict, nthreads: int = 1) -> Params:
     if itype not in INITS:
         raise ValueError(f"unknown initial conditions type {itype}")
     p.initType = INITS[itype]
-    # defaultSolverParams (UDF :104-112, PLDF :127-135, Hernquist :77-85)
+    # defaultSolverParams (UDF :104-112, PLDF :127-135, Hernquist :77-85, Plummer :72-80)
     p.G = _get(cfg, "solver.G", 1.0)
-    p.dt = _get(cfg, "solver.dt", 0.025 if itype == "Hernquist" else 0.005)
+    p.dt = _get(cfg, "solver.dt", {"Hernquist": 0.025, "Plummer": 0.1}.get(itype, 0.005))
     p.noLinking = int(_get(cfg, "solver.noLinking", False, conv=_bool))
     p.kernelType = KERNELS[_get(cfg, "solver.kernelType", "delayed", conv=str)]
     p.kernelBufferSize = _get(cfg, "solver.kernelBufferSize", 0, conv=int)
@@ -162,6 +162,11 @@ def resolve(cfg: dict, nthreads: int = 1) -> Params:
         grid = dict(Pmin=0.01, Pmax=200.0, Umin=[-16 * math.sqrt(sigma2), 0.0],
                     Umax=[16 * math.sqrt(sigma2), 1.6], Pres=200, Ures=[200, 200], Pscale=1,
                     Uscale=[0, 2])
+    elif itype == "Plummer":  # initGen_Plummer.hxx:37-70
+        p.R0 = _get(cfg, "init.R0", 1.0)
+        p.M = _get(cfg, "init.M", 1.0)
+        grid = dict(Pmin=0.01, Pmax=100.0, Umin=[-1.45, 0.0], Umax=[1.45, 3.24], Pres=180,
+                    Ures=[80, 80], Pscale=1, Uscale=[0, 2])
     else:  # initGen_Hernquist.hxx:38-75
         p.R0 = _get(cfg, "init.R0", 1.0)
         p.M = _get(cfg, "init.M", 1.0)

@@ -207,12 +207,26 @@ static double hq_F(const orc_state *s, double phi, double r, double u, double j)
   return result;
 }
 
-/* initGen::valueAt for each generator (UDF :175-179, PLDF :191-195, Hernquist :156-161) */
+/* Plummer, initGen_Plummer.hxx:89-92 (Phi), :126-131 (F) */
+static double pl_phi(const orc_state *s, double R)
+{
+  return -(s->p.G * s->p.M) / sqrt(R * R + s->p.R0 * s->p.R0);
+}
+static double pl_F(double phi, double r, double u, double j)
+{
+  double eps = phi + (u * u + (j * j) / (r * r)) * 0.5;
+  if (eps >= 0) return 0;
+  return pow(-2. * eps, 3.5) * (3. / (M_PI * M_PI * M_PI * 7.));
+}
+
+/* initGen::valueAt for each generator (UDF :175-179, PLDF :191-195, Hernquist :156-161,
+ * Plummer initGen_Plummer.hxx:133-138) */
 static double value_at(const orc_state *s, double r, double u, double j)
 {
   switch (s->p.initType) {
   case ORC_UDF: return udf_F(s, r, u, j);
   case ORC_PLDF: return pldf_F(s, r, u, j);
+  case ORC_PLUMMER: return pl_F(pl_phi(s, r), r, u, j);
   default: return hq_F(s, hq_phi(s, r), r, u, j);
   }
 }
@@ -856,7 +870,8 @@ orc_state *orc_create(const orc_params *p)
   if (p->initType == ORC_UDF) udf_setup(s);
   else if (p->initType == ORC_PLDF) pldf_setup(s);
   else s->Vg = sqrt((p->G * p->M) / (p->R0));
-  /* generate (UDF :182-208, PLDF :197-224, Hernquist :163-204) */
+  /* generate (UDF :182-208, PLDF :197-224, Hernquist :163-204, Plummer
+   * initGen_Plummer.hxx:141-162: Phi at the r nodes, as valueAt) */
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < s->nJ; j++) {
     double J = s->J_Val[j];

@@ -23,7 +23,7 @@ extern "C" {
 /* scale types, same codes as the reference (scale.hxx:17-19) */
 enum { ORC_LINEAR = 0, ORC_LOG = 1, ORC_QUADRATIC = 2, ORC_CUBIC = 3, ORC_QUARTIC = 4 };
 /* initial-condition generators (initGen_all.hxx:19-25) */
-enum { ORC_UDF = 0, ORC_PLDF = 1, ORC_HERNQUIST = 2 };
+enum { ORC_UDF = 0, ORC_PLDF = 1, ORC_HERNQUIST = 2, ORC_PLUMMER = 3 };
 /* kernel types (vlasovSolver21.hxx:66-68) */
 enum { ORC_DELAYED = 0, ORC_REFLECTIVE = 1 };
 

@@ -53,6 +53,12 @@ SMALL_CASES["udf_odd_shape"] = (UDF_ODD, 30)
 HERNQUIST_OM = dict(HERNQUIST_SMALL, **{"init.Ra": 1.5, "grid.Pres": 32, "grid.Ures": [32, 16]})
 SMALL_CASES["hernquist_om"] = (HERNQUIST_OM, 40)
 
+# Plummer sphere (the isotropic Plummer distribution function) on its default
+# ranges (r 0.01..100 log, v +-1.45, j 0..3.24 quadratic)
+PLUMMER_SMALL = {"init.type": "Plummer", "init.M": 1, "init.R0": 1, "grid.Pres": 48,
+                 "grid.Ures": [40, 20], "solver.dt": 0.05}
+SMALL_CASES["plummer"] = (PLUMMER_SMALL, 40)
+
 # Strong kick: accelerations that move f by tens of v cells in one step, so
 # that the target rows of the kick spread widely across neighbouring r nodes
 # (physically extreme, numerically valid).

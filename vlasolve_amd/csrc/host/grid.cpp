@@ -295,6 +295,50 @@ class Hernquist : public InitGen {
   double mass() const override { return M; }
 };
 
+// Plummer sphere, isotropic (initGen_Plummer.hxx)
+class Plummer : public InitGen {
+  double M = 1, R0 = 1, G = 1;
+
+  double phi(double R) const { return -(G * M) / std::sqrt(R * R + R0 * R0); }  // :91-93
+  static double F(double ph, double r, double u, double j) {                    // :125-130
+    double eps = ph + (u * u + (j * j) / (r * r)) * 0.5;
+    if (eps >= 0) return 0;
+    return std::pow(-2. * eps, 3.5) * (3. / (M_PI * M_PI * M_PI * 7.));
+  }
+
+ public:
+  std::string type() const override { return "Plummer"; }
+  void setup(const Params &prm, const SolverParams &sp) override {  // :37-44
+    G = sp.G;
+    R0 = prm.get<double>("R0", "init", 1.0);
+    M = prm.get<double>("M", "init", 1.0);
+  }
+  GridParams defaultGrid() const override {  // :50-70
+    GridParams g;
+    g.Pmin = 0.01; g.Pmax = 25 * 4;
+    g.Umin[0] = -1.45; g.Umax[0] = 1.45; g.Umin[1] = 0.00; g.Umax[1] = 3.24;
+    g.Pres = 45 * 4; g.Ures[0] = 40 * 2; g.Ures[1] = 20 * 4;
+    g.Pscale = LOGARITHMIC; g.Uscale[0] = LINEAR; g.Uscale[1] = QUADRATIC;
+    return g;
+  }
+  SolverParams defaultSolver() const override {  // :72-80
+    SolverParams s;
+    s.G = 1; s.T0 = 0; s.Tmax = 62; s.dt = 0.1; s.nSpecies = 1;
+    return s;
+  }
+  double valueAt(double r, double u, double j) const override { return F(phi(r), r, u, j); }
+  void generate(const Grid21 &g, double *f) const override {  // :138-160
+    std::vector<double> Phi(g.nR);
+    for (int r = 0; r < g.nR; r++) Phi[r] = phi(g.R_Val[r]);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < g.nJ; j++)
+      for (int u = 0; u < g.nU; u++)
+        for (int r = 0; r < g.nR; r++)
+          f[g.index(j, u, r)] = F(Phi[r], g.R_Val[r], g.U_Val[u], g.J_Val[j]);
+  }
+  double mass() const override { return M; }
+};
+
 }  // namespace
 
 std::unique_ptr<InitGen> makeInitGen(const std::string &type) {
@@ -302,6 +346,7 @@ std::unique_ptr<InitGen> makeInitGen(const std::string &type) {
   if (type == "UDF") return std::unique_ptr<InitGen>(new UDF());
   if (type == "PLDF") return std::unique_ptr<InitGen>(new PLDF());
   if (type == "Hernquist") return std::unique_ptr<InitGen>(new Hernquist());
+  if (type == "Plummer") return std::unique_ptr<InitGen>(new Plummer());
   throw Failure("ERROR: unknown initial conditions type " + type + "\n.");
 }
 
