@@ -70,6 +70,8 @@ typedef struct vls_solver_params {
   double dt;              /* solver.dt */
   int32_t no_linking;     /* solver.noLinking: D(dt/2) K(dt) D(dt/2) */
   int32_t kernel_type;    /* 0 delayed, 1 reflective (solver.kernelType) */
+  int32_t n_species;      /* solver.nSpecies, 1..5 (0 means 1): the fields of
+                             gridBase (nFields, src/grid_base.hxx:78) */
 } vls_solver_params;
 
 /* ---------------------------------------------------------------- layer 1 */
@@ -82,9 +84,10 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
 void vls_destroy(vls_ctx *ctx);
 const char *vls_last_error(void);
 
-/* Distribution function of one run, host layout [nJ][nU][nR].
- * Replaces the grid21 data the reference's step reads and writes
- * (gridBase::getDataPtr, src/grid_base.hxx:383). */
+/* Distribution function of one run, host layout [nJ][nU][nR], with several
+ * species [nJ][nU][nR][nSpecies] (species fastest, as the reference
+ * interleaves them).  Replaces the grid21 data the reference's step reads
+ * and writes (gridBase::getDataPtr, src/grid_base.hxx:383-385). */
 int vls_upload_f(vls_ctx *ctx, int run, const double *f);
 int vls_download_f(vls_ctx *ctx, int run, double *f);
 
@@ -104,7 +107,9 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu,
  * whose front is at ring index head[c].  back_xr/back_xu are the spline-unit
  * coordinates of the chain's last cell and back_out[c] is 1 when that cell's
  * v lies outside (U_min, U_max), where the reference stores 0
- * (src/vlasovSolver21_spline.hxx:776-786). */
+ * (src/vlasovSolver21_spline.hxx:776-786).  With several species every cell
+ * holds nSpecies values, vals[cell * nSpecies + s]; the species share the
+ * ring, which turns once per step. */
 int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
                    const int32_t *len, const int32_t *head, const double *back_xr,
                    const double *back_xu, const uint8_t *back_out, const double *vals);
@@ -130,8 +135,15 @@ int vls_set_graph(vls_ctx *ctx, int on);
 int vls_moments(vls_ctx *ctx, int run, double *dM, double *accel, double *mTot);
 
 /* Statistics of the current f (vlasovSolver21::statistics,
- * src/vlasovSolver21.hxx:627-689): out = {Mass, E, Ek, Ep, S, L1, L2}. */
+ * src/vlasovSolver21.hxx:627-711): out = {Mass, E, Ek, Ep, S, L1, L2}, the
+ * line of statistics.txt.  With several species this is the line of the
+ * files statistics_S%d.txt, the same in each as the reference computes it:
+ * Mass and Ep of all species, Ek, S, L1 and L2 of species 0. */
 int vls_stats(vls_ctx *ctx, int run, double out[7]);
+/* Both lines: the species files' (line) and statistics.txt's (total: the
+ * species line added once per species, with S and L2 of the summed species;
+ * equal to line with one species).  Either pointer may be NULL. */
+int vls_stats_total(vls_ctx *ctx, int run, double line[7], double total[7]);
 
 /* Coefficients of the last drift spline of slice j, (nU+2)*(nR+2) doubles,
  * the padded array of localSpline<2> (src/localSpline.hxx:60-150). */
@@ -156,8 +168,9 @@ int vls_copy_probe(size_t bytes);
 int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
                        int64_t *mismatches);
 
-/* Sizes of one run: {nR, nU, nJ, number of delay lines, delay-line cells}. */
-int vls_run_info(vls_ctx *ctx, int run, int64_t out[5]);
+/* Sizes of one run: {nR, nU, nJ, number of delay lines, delay-line cells,
+ * species}. */
+int vls_run_info(vls_ctx *ctx, int run, int64_t out[6]);
 /* Overwrite the delay-line state (the inverse of vls_get_chains): ring heads
  * and ring values in the layout of vls_set_chains.  Either may be NULL. */
 int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *vals);
@@ -204,9 +217,11 @@ int vlsd_create(int argc, const char *const *argv, vlsd_solver **out);
 void vlsd_destroy(vlsd_solver *s);
 /* geometry: {nR, nU, nJ}; time, step count */
 int vlsd_info(vlsd_solver *s, int64_t dims[3], double *time, int64_t *stepCount);
+int vlsd_species(vlsd_solver *s);                   /* solver.nSpecies */
 int vlsd_step(vlsd_solver *s, int nSteps);          /* makeOneStep x n, time += dt */
-int vlsd_get_f(vlsd_solver *s, double *f);
-int vlsd_stats(vlsd_solver *s, double out[7]);
+int vlsd_get_f(vlsd_solver *s, double *f);          /* layout of vls_download_f */
+int vlsd_stats(vlsd_solver *s, double out[7]);      /* line of vls_stats */
+int vlsd_stats_total(vlsd_solver *s, double out[7]); /* statistics.txt line (vls_stats_total) */
 int vlsd_moments(vlsd_solver *s, double *dM, double *accel, double *mTot);
 int vlsd_run(vlsd_solver *s);                       /* vlasovSolver::run, src/vlasovSolver.hxx:134-190 */
 int vlsd_write_restart(vlsd_solver *s, const char *fname_no_ext);   /* .vls */
@@ -229,13 +244,15 @@ vls_ctx *vlsd_ensemble_context(vlsd_ensemble *e);
  * static feet and delay lines as vlsd_create, for inspection and for
  * machines without a GPU.  dims = {nR, nU, nJ}; feet are the free-streaming
  * foot positions and VLS_FOOT_* flags of every point; chains as in
- * vls_set_chains, cells front first with their positions and values. */
+ * vls_set_chains, cells front first with their positions and values
+ * (nSpecies values per cell). */
 typedef struct vlsd_host vlsd_host;
 int vlsd_host_create(int argc, const char *const *argv, vlsd_host **out);
 void vlsd_host_destroy(vlsd_host *h);
 int vlsd_host_info(vlsd_host *h, int64_t dims[3], int64_t *nChains, int64_t *nCells,
                    double *kernelRadius);
-int vlsd_host_f(vlsd_host *h, double *f);
+int vlsd_host_species(vlsd_host *h);
+int vlsd_host_f(vlsd_host *h, double *f);  /* layout of vls_upload_f */
 int vlsd_host_feet(vlsd_host *h, double *driftR, double *driftU, uint8_t *flag);
 int vlsd_host_chains(vlsd_host *h, int64_t *point, int32_t *len, double *cellR, double *cellU,
                      double *val);

@@ -27,6 +27,7 @@ REF_EXE = HERE / "_ref" / "vlasolve"
 SCALES = {"linear": 0, "logarithmic": 1, "quadratic": 2, "cubic": 3, "quartic": 4}
 INITS = {"UDF": 0, "PLDF": 1, "Hernquist": 2, "Plummer": 3}
 KERNELS = {"delayed": 0, "reflective": 1}
+MAX_SPECIES = 5  # initGen::MAX_SPECIES, initGen.hxx:21
 
 
 class Params(C.Structure):
@@ -40,6 +41,11 @@ class Params(C.Structure):
         ("M", C.c_double), ("R0", C.c_double), ("alpha", C.c_double),
         ("smoothingRadius", C.c_double), ("gap0", C.c_double), ("gap1", C.c_double),
         ("eta", C.c_double), ("n", C.c_double), ("Ra", C.c_double),
+        ("nSpecies", C.c_int),
+        ("spM", C.c_double * MAX_SPECIES), ("spR0", C.c_double * MAX_SPECIES),
+        ("spAlpha", C.c_double * MAX_SPECIES), ("spSR", C.c_double * MAX_SPECIES),
+        ("spGap0", C.c_double * MAX_SPECIES), ("spGap1", C.c_double * MAX_SPECIES),
+        ("spEta", C.c_double * MAX_SPECIES), ("spN", C.c_double * MAX_SPECIES),
         ("nthreads", C.c_int),
     ]
 
@@ -62,6 +68,8 @@ def lib():
         L.orc_dims.argtypes = [C.c_void_p, P(C.c_long)]
         L.orc_nval.restype = C.c_long
         L.orc_nval.argtypes = [C.c_void_p]
+        L.orc_nspecies.restype = C.c_int
+        L.orc_nspecies.argtypes = [C.c_void_p]
         for name in ("orc_get_f", "orc_set_f"):
             getattr(L, name).argtypes = [C.c_void_p, P(d)]
         L.orc_get_axes.argtypes = [C.c_void_p, P(d), P(d), P(d), P(d)]
@@ -71,6 +79,7 @@ def lib():
         L.orc_last_moments.argtypes = [C.c_void_p, P(d), P(d), P(d)]
         L.orc_moments.argtypes = [C.c_void_p, P(d), P(d), P(d)]
         L.orc_stats.argtypes = [C.c_void_p, P(d)]
+        L.orc_stats_total.argtypes = [C.c_void_p, P(d)]
         L.orc_get_feet.argtypes = [C.c_void_p, P(d), P(d), P(C.c_uint8)]
         L.orc_num_chains.restype = C.c_long
         L.orc_num_chains.argtypes = [C.c_void_p]
@@ -137,28 +146,37 @@ def resolve(cfg: dict, nthreads: int = 1) -> Params:
     p.noLinking = int(_get(cfg, "solver.noLinking", False, conv=_bool))
     p.kernelType = KERNELS[_get(cfg, "solver.kernelType", "delayed", conv=str)]
     p.kernelBufferSize = _get(cfg, "solver.kernelBufferSize", 0, conv=int)
+    p.nSpecies = _get(cfg, "solver.nSpecies", 1, conv=int)  # vlasovSolver.hxx:72
     p.nthreads = nthreads
     # generator parameters
-    if itype == "UDF":  # initGen_UniformDensitySphere.hxx:49-62
-        p.R0 = _get(cfg, "init.R0", 2.0)
-        p.M = _get(cfg, "init.M", 1.0)
-        p.alpha = _get(cfg, "init.alpha", 0.5)
-        p.smoothingRadius = _get(cfg, "init.smoothingRadius", 0.5)
-        p.gap0 = _get(cfg, "init.gap", 0.0, 0)
-        p.gap1 = _get(cfg, "init.gap", 0.0, 1)
+    if itype == "UDF":  # initGen_UniformDensitySphere.hxx:49-62, value i for species i
+        for i in range(min(p.nSpecies, MAX_SPECIES)):
+            p.spR0[i] = _get(cfg, "init.R0", 2.0, i)
+            p.spM[i] = _get(cfg, "init.M", 1.0, i)
+            p.spAlpha[i] = _get(cfg, "init.alpha", 0.5, i)
+            p.spSR[i] = _get(cfg, "init.smoothingRadius", 0.5, i)
+            p.spGap0[i] = _get(cfg, "init.gap", 0.0, 2 * i)
+            p.spGap1[i] = _get(cfg, "init.gap", 0.0, 2 * i + 1)
+        p.R0, p.M, p.alpha = p.spR0[0], p.spM[0], p.spAlpha[0]
+        p.smoothingRadius, p.gap0, p.gap1 = p.spSR[0], p.spGap0[0], p.spGap1[0]
         grid = dict(Pmin=0.01, Pmax=25.0, Umin=[-2.0, 0.0], Umax=[2.0, 1.6], Pres=200,
                     Ures=[200, 200], Pscale=1, Uscale=[0, 2])
-    elif itype == "PLDF":  # initGen_PowerLawDensitySphere.hxx:50-80
-        p.M = _get(cfg, "init.M", 1.0)
-        p.alpha = _get(cfg, "init.alpha", 2.0)
-        p.eta = _get(cfg, "init.eta", 0.1)
-        p.n = _get(cfg, "init.n", 1.0)
-        p.smoothingRadius = _get(cfg, "init.smoothingRadius", 0.5)
-        p.R0 = _get(cfg, "init.R0", 1.0 / p.eta)
-        n = p.n
-        sigO2 = (3.0 - n) / (5.0 - 2 * n) * (p.M * p.eta) / (p.R0 * (p.alpha * 0.5 + 1))
-        sigR2 = sigO2 * 0.5 * p.alpha
-        sigma2 = max(0.0, sigO2 + 2 * sigR2)
+    elif itype == "PLDF":  # initGen_PowerLawDensitySphere.hxx:50-80, value i for species i
+        sigma2 = 0.0
+        for i in range(min(p.nSpecies, MAX_SPECIES)):
+            p.spM[i] = _get(cfg, "init.M", 1.0, i)
+            p.spAlpha[i] = _get(cfg, "init.alpha", 2.0, i)
+            p.spEta[i] = _get(cfg, "init.eta", 0.1, i)
+            p.spN[i] = _get(cfg, "init.n", 1.0, i)
+            p.spSR[i] = _get(cfg, "init.smoothingRadius", 0.5, i)
+            p.spR0[i] = _get(cfg, "init.R0", 1.0 / p.spEta[i], i)
+            n = p.spN[i]
+            sigO2 = ((3.0 - n) / (5.0 - 2 * n) * (p.spM[i] * p.spEta[i]) /
+                     (p.spR0[i] * (p.spAlpha[i] * 0.5 + 1)))
+            sigR2 = sigO2 * 0.5 * p.spAlpha[i]
+            sigma2 = max(sigma2, sigO2 + 2 * sigR2)  # defaultGridParams :95-100
+        p.M, p.alpha, p.eta, p.n = p.spM[0], p.spAlpha[0], p.spEta[0], p.spN[0]
+        p.smoothingRadius, p.R0 = p.spSR[0], p.spR0[0]
         grid = dict(Pmin=0.01, Pmax=200.0, Umin=[-16 * math.sqrt(sigma2), 0.0],
                     Umax=[16 * math.sqrt(sigma2), 1.6], Pres=200, Ures=[200, 200], Pscale=1,
                     Uscale=[0, 2])
@@ -211,7 +229,9 @@ class Oracle:
         dims = (C.c_long * 3)()
         L.orc_dims(self._h, dims)
         self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
-        self.shape = (self.nJ, self.nU, self.nR)
+        self.nS = L.orc_nspecies(self._h)
+        # the reference's layout: species fastest when there are several
+        self.shape = (self.nJ, self.nU, self.nR) + ((self.nS,) if self.nS > 1 else ())
 
     def close(self):
         if self._h:
@@ -234,7 +254,7 @@ class Oracle:
 
     def set_f(self, f):
         a = np.ascontiguousarray(f, np.float64)
-        assert a.size == self.nR * self.nU * self.nJ
+        assert a.size == self.nR * self.nU * self.nJ * self.nS
         lib().orc_set_f(self._h, _ptr(a))
 
     def axes(self):
@@ -263,8 +283,15 @@ class Oracle:
         return dM, a, t.value
 
     def stats(self):
+        """the statistics line of the species files (statistics.txt with one species)"""
         out = np.empty(7)
         lib().orc_stats(self._h, _ptr(out))
+        return dict(zip(("M", "E", "Ek", "Ep", "S", "L1", "L2"), out))
+
+    def stats_total(self):
+        """the statistics.txt line"""
+        out = np.empty(7)
+        lib().orc_stats_total(self._h, _ptr(out))
         return dict(zip(("M", "E", "Ek", "Ep", "S", "L1", "L2"), out))
 
     def feet(self):
@@ -273,7 +300,8 @@ class Oracle:
         U = np.empty(n)
         fl = np.empty(n, np.uint8)
         lib().orc_get_feet(self._h, _ptr(R), _ptr(U), _ptr(fl, C.c_uint8))
-        return R.reshape(self.shape), U.reshape(self.shape), fl.reshape(self.shape)
+        pts = (self.nJ, self.nU, self.nR)  # one foot per point, whatever the species
+        return R.reshape(pts), U.reshape(pts), fl.reshape(pts)
 
     def chains(self):
         L = lib()
@@ -284,9 +312,11 @@ class Oracle:
         ln = np.empty(nc, np.int32)
         R = np.empty(ncell)
         U = np.empty(ncell)
-        V = np.empty(ncell)
+        V = np.empty(ncell * self.nS)
         L.orc_get_chains(self._h, _ptr(point, C.c_int64), _ptr(off, C.c_int64),
                          _ptr(ln, C.c_int32), _ptr(R), _ptr(U), _ptr(V))
+        if self.nS > 1:
+            V = V.reshape(ncell, self.nS)
         return dict(point=point, off=off, len=ln, R=R, U=U, val=V)
 
     # ---- tangent-linear model (checks the adjoint of the HIP path)

@@ -93,7 +93,7 @@ typedef struct {
   long point; /* grid index i the chain feeds (map key, sphericalKernel.hxx:174-187) */
   long j;
   long n, cap;
-  double *R, *U, *val;
+  double *R, *U, *val; /* val[k * nS + s]: nS species values per cell */
   double *tval; /* tangent of val (tangent-linear model), or NULL */
 } chain_t;
 
@@ -101,9 +101,11 @@ struct orc_state {
   orc_params p;
   long nR, nU, nJ, nVal;
   double *R_Val, *U_Val, *J_Val, *R_V, *U_V, *J_V;
-  double *f;
-  /* initial-condition constants */
-  double C0, C1, C2, rho0, R0, Vg;
+  double *f;   /* nS planes of nVal points, species s at f + s * nVal */
+  int nS;      /* species */
+  /* initial-condition constants, per species */
+  double C0[ORC_MAX_SPECIES], C1[ORC_MAX_SPECIES], C2[ORC_MAX_SPECIES], rho0[ORC_MAX_SPECIES];
+  double R0, Vg;
   /* spline (localSpline.hxx:60-93) */
   int sdim[2];
   double sstart[2], sstep_inv[2];
@@ -135,24 +137,24 @@ struct orc_state {
 /* initial conditions */
 
 /* UniformDensitySphere::setupParameters, initGen_UniformDensitySphere.hxx:65-74 */
-static void udf_setup(orc_state *s)
+static void udf_setup(orc_state *s, int k)
 {
   const orc_params *p = &s->p;
-  s->rho0 = (3 * p->M) / (4 * M_PI * powi_ref(p->R0, 3));
-  double sig0 = sqrt((p->alpha) / (5 * p->R0));
-  s->C1 = s->rho0 * pow(2 * M_PI * sig0 * sig0, -1.5);
-  s->C2 = 2 * sig0 * sig0;
+  s->rho0[k] = (3 * p->spM[k]) / (4 * M_PI * powi_ref(p->spR0[k], 3));
+  double sig0 = sqrt((p->spAlpha[k]) / (5 * p->spR0[k]));
+  s->C1[k] = s->rho0[k] * pow(2 * M_PI * sig0 * sig0, -1.5);
+  s->C2[k] = 2 * sig0 * sig0;
 }
 
 /* UniformDensitySphere::computeF, initGen_UniformDensitySphere.hxx:158-173 */
-static double udf_F(const orc_state *s, double r, double u, double j)
+static double udf_F(const orc_state *s, int k, double r, double u, double j)
 {
   const orc_params *p = &s->p;
-  double sm = 0.5 * (1. + erf((p->R0 - r) / p->smoothingRadius));
-  double res = sm * s->C1 * exp(-(u * u + (j * j) / (r * r)) / s->C2);
-  if (p->gap1 > 0) {
-    double sm0 = 0.5 * (1. + erf((p->gap0 - r) / 0.15));
-    double sm1 = 0.5 * (1. + erf(-(p->gap1 - r) / 0.15));
+  double sm = 0.5 * (1. + erf((p->spR0[k] - r) / p->spSR[k]));
+  double res = sm * s->C1[k] * exp(-(u * u + (j * j) / (r * r)) / s->C2[k]);
+  if (p->spGap1[k] > 0) {
+    double sm0 = 0.5 * (1. + erf((p->spGap0[k] - r) / 0.15));
+    double sm1 = 0.5 * (1. + erf(-(p->spGap1[k] - r) / 0.15));
     res *= (sm0 + sm1);
   }
   if (res < 1.E-200) res = 1.E-200;
@@ -161,24 +163,25 @@ static double udf_F(const orc_state *s, double r, double u, double j)
 
 /* PowerLawDensitySphere::setupParameters, initGen_PowerLawDensitySphere.hxx:62-80
  * (R0 resolved by the caller: 1/eta unless given, :63-64) */
-static void pldf_setup(orc_state *s)
+static void pldf_setup(orc_state *s, int k)
 {
   const orc_params *p = &s->p;
-  double n = p->n;
-  s->rho0 = ((3.0 - n) * p->M) / (4 * M_PI * pow(p->R0, 3.0 - n));
-  double sigO2 = (3.0 - n) / (5.0 - 2 * n) * (p->M * p->eta) / (p->R0 * (p->alpha * 0.5 + 1));
-  double sigR2 = sigO2 * 0.5 * p->alpha;
-  s->C0 = s->rho0 / (sqrt(2 * sigR2 * M_PI * M_PI * M_PI) * sigO2);
-  s->C1 = 1.0 / (2 * sigR2);
-  s->C2 = 1.0 / (sigO2);
+  double n = p->spN[k], M = p->spM[k], R0 = p->spR0[k], alpha = p->spAlpha[k];
+  s->rho0[k] = ((3.0 - n) * M) / (4 * M_PI * pow(R0, 3.0 - n));
+  double sigO2 = (3.0 - n) / (5.0 - 2 * n) * (M * p->spEta[k]) / (R0 * (alpha * 0.5 + 1));
+  double sigR2 = sigO2 * 0.5 * alpha;
+  s->C0[k] = s->rho0[k] / (sqrt(2 * sigR2 * M_PI * M_PI * M_PI) * sigO2);
+  s->C1[k] = 1.0 / (2 * sigR2);
+  s->C2[k] = 1.0 / (sigO2);
 }
 
 /* PowerLawDensitySphere::computeF, initGen_PowerLawDensitySphere.hxx:181-189 */
-static double pldf_F(const orc_state *s, double r, double u, double j)
+static double pldf_F(const orc_state *s, int k, double r, double u, double j)
 {
   const orc_params *p = &s->p;
-  double sm = 0.5 * (1. + erf((p->R0 - r) / p->smoothingRadius));
-  double res = sm * s->C0 * exp(-u * u * s->C1 - ((j * j) / (r * r)) * s->C2) * pow(r, -p->n);
+  double sm = 0.5 * (1. + erf((p->spR0[k] - r) / p->spSR[k]));
+  double res = sm * s->C0[k] * exp(-u * u * s->C1[k] - ((j * j) / (r * r)) * s->C2[k]) *
+               pow(r, -p->spN[k]);
   if (res < 1.E-200) res = 1.E-200;
   return res;
 }
@@ -221,11 +224,11 @@ static double pl_F(double phi, double r, double u, double j)
 
 /* initGen::valueAt for each generator (UDF :175-179, PLDF :191-195, Hernquist :156-161,
  * Plummer initGen_Plummer.hxx:133-138) */
-static double value_at(const orc_state *s, double r, double u, double j)
+static double value_at(const orc_state *s, int k, double r, double u, double j)
 {
   switch (s->p.initType) {
-  case ORC_UDF: return udf_F(s, r, u, j);
-  case ORC_PLDF: return pldf_F(s, r, u, j);
+  case ORC_UDF: return udf_F(s, k, r, u, j);
+  case ORC_PLDF: return pldf_F(s, k, r, u, j);
   case ORC_PLUMMER: return pl_F(pl_phi(s, r), r, u, j);
   default: return hq_F(s, hq_phi(s, r), r, u, j);
   }
@@ -465,6 +468,17 @@ static inline double avg4(const orc_state *s, const double *f, long i)
   return ((f[i]) + f[i + 1] + f[i + s->nR] + f[i + s->nR + 1]) * 0.25;
 }
 
+/* the integrand of int21::f(0, ns) (integrand21.hxx:21-39): get_avg() of the
+ * one field, or with several fields 0 + get_avg(0) + get_avg(1) + ...; the
+ * planes of f are nVal apart */
+static inline double avgS(const orc_state *s, const double *f, long i, int ns)
+{
+  if (ns <= 1) return avg4(s, f, i);
+  double v = 0;
+  for (int k = 0; k < ns; k++) v += avg4(s, f + (size_t)k * s->nVal, i);
+  return v;
+}
+
 /* static schedule chunk of iteration range [0,n) for thread t of T
  * (contiguous, balanced; identical to every OpenMP runtime when T | n) */
 static void chunk(long n, int T, int t, long *lo, long *hi)
@@ -476,17 +490,17 @@ static void chunk(long n, int T, int t, long *lo, long *hi)
 
 /* per-point moment term of integrate1D (quadrature21.hxx:111-119) */
 static inline double moment_term(const orc_state *s, const double *f, long j, long u, long r,
-                                 double Ct1)
+                                 double Ct1, int ns)
 {
   double dU = get_dU(s, u);
   if (dU <= 0) return NAN;
   double dR = get_dR(s, r);
   if (dR <= 0) return NAN;
-  return Ct1 * dU * dR * avg4(s, f, (j * s->nU + u) * s->nR + r);
+  return Ct1 * dU * dR * avgS(s, f, (j * s->nU + u) * s->nR + r, ns);
 }
 
-/* quadrature21::integrate1D with int21::f (quadrature21.hxx:88-142) */
-static void integrate1D(orc_state *s, const double *f, double *Mrv)
+/* quadrature21::integrate1D with int21::f(0, ns) (quadrature21.hxx:88-142) */
+static void integrate1D(orc_state *s, const double *f, double *Mrv, int ns)
 {
   const long nR = s->nR, nU = s->nU, nJ = s->nJ;
   long r;
@@ -506,7 +520,7 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv)
             if (dU <= 0) continue;
             double dR = get_dR(s, rr);
             if (dR <= 0) continue;
-            Mr[rr] += Ct1 * dU * dR * avg4(s, f, (j * nU + u) * nR + rr);
+            Mr[rr] += Ct1 * dU * dR * avgS(s, f, (j * nU + u) * nR + rr, ns);
           }
       }
     }
@@ -535,7 +549,7 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv)
           for (long t = 0; t < 16; t++) {
             long u = 64 * w + 16 * q + t;
             if (u >= nU - 1) continue;
-            double tval = moment_term(s, f, j, u, rr, Ct1);
+            double tval = moment_term(s, f, j, u, rr, Ct1, ns);
             if (tval == tval) acc += tval;
           }
           G[q] = acc;
@@ -569,8 +583,9 @@ static inline double integrand(int kind, double r, double u, double j, double v)
   }
 }
 
-/* quadrature21::integrate (quadrature21.hxx:57-85) */
-static double integrate(orc_state *s, const double *f, int kind)
+/* quadrature21::integrate (quadrature21.hxx:57-85), integrand over the sum
+ * of the first ns fields */
+static double integrate(orc_state *s, const double *f, int kind, int ns)
 {
   const long nR = s->nR, nU = s->nU, nJ = s->nJ;
   int T = s->p.nthreads;
@@ -591,7 +606,7 @@ static double integrate(orc_state *s, const double *f, int kind)
             if (dU <= 0) continue;
             double dR = get_dR(s, r);
             if (dR <= 0) continue;
-            double v = avg4(s, f, (j * nU + u) * nR + r);
+            double v = avgS(s, f, (j * nU + u) * nR + r, ns);
             acc += Ct1 * dU * dR * integrand(kind, s->R_Val[r] + 0.5 * dR, s->U_Val[u] + 0.5 * dU, J, v);
           }
       }
@@ -615,7 +630,7 @@ static double integrate(orc_state *s, const double *f, int kind)
       double acc = 0;
       for (long u = 0; u < nU - 1; u++) {
         double dU = get_dU(s, u);
-        double v = avg4(s, f, (j * nU + u) * nR + r);
+        double v = avgS(s, f, (j * nU + u) * nR + r, ns);
         acc += Ct1 * dU * dR * integrand(kind, s->R_Val[r] + 0.5 * dR, s->U_Val[u] + 0.5 * dU, J, v);
       }
       qj += acc;
@@ -641,31 +656,35 @@ static double accumulate(const double *dM, double *Mr, long n)
 }
 
 /* ------------------------------------------------------------------ */
-static void chain_push(chain_t *c, double R, double U, double v)
+/* append a cell with its nS species values (v NULL: zeros) */
+static void chain_push(chain_t *c, double R, double U, const double *v, int nS)
 {
   if (c->n == c->cap) {
     c->cap = c->cap ? 2 * c->cap : 4;
     c->R = realloc(c->R, c->cap * sizeof(double));
     c->U = realloc(c->U, c->cap * sizeof(double));
-    c->val = realloc(c->val, c->cap * sizeof(double));
+    c->val = realloc(c->val, c->cap * nS * sizeof(double));
   }
   c->R[c->n] = R;
   c->U[c->n] = U;
-  c->val[c->n] = v;
+  for (int k = 0; k < nS; k++) c->val[c->n * nS + k] = v ? v[k] : 0;
   c->n++;
 }
 
-/* cellChain::readRoll + roll (cellChain.hxx:49-63): read the front value, then
- * successive swaps move every value one place towards the front and the old
- * front value to the back */
-static double chain_read_roll(chain_t *c)
+/* sphericalKernel::readAt (sphericalKernel.hxx:48-54) with cellChain::roll
+ * (cellChain.hxx:49-56): read species k of the front cell; with roll,
+ * successive swaps of the cells' value vectors move every cell's values one
+ * place towards the front and the old front's to the back */
+static double chain_read(chain_t *c, int k, int nS, int roll)
 {
-  double result = c->val[0];
-  for (long k = 0; k + 1 < c->n; k++) {
-    double t = c->val[k];
-    c->val[k] = c->val[k + 1];
-    c->val[k + 1] = t;
-  }
+  double result = c->val[k];
+  if (roll)
+    for (long q = 0; q + 1 < c->n; q++)
+      for (int m = 0; m < nS; m++) {
+        double t = c->val[q * nS + m];
+        c->val[q * nS + m] = c->val[(q + 1) * nS + m];
+        c->val[(q + 1) * nS + m] = t;
+      }
   return result;
 }
 
@@ -718,31 +737,39 @@ static int setup_chains(orc_state *s, double dt)
         double deltaT = dth, nr, nu;
         if (advance(s->R_Val[r], s->U_Val[u], J, deltaT, -1, &nr, &nu)) return -1;
         while (nr < s->kernelRadius) {
-          chain_push(c, nr, nu, value_at(s, nr, nu, J));
+          double v[ORC_MAX_SPECIES];
+          for (int k = 0; k < s->nS; k++) v[k] = value_at(s, k, nr, nu, J);
+          chain_push(c, nr, nu, v, s->nS);
           deltaT += dth;
           if (advance(s->R_Val[r], s->U_Val[u], J, deltaT, -1, &nr, &nu)) return -1;
         }
-        chain_push(c, nr, nu, 0);
+        /* the back cell starts at 0 for every species (the reference makes it
+         * with one value, dummyCell21(R,U,1), vlasovSolver21.hxx:322: with
+         * several species its later writes and reads of species >= 1 fall
+         * outside that cell's vector) */
+        chain_push(c, nr, nu, NULL, s->nS);
       }
   }
   s->chainStart[s->nJ] = s->nChains;
   return 0;
 }
 
-/* vlasovSolver21::renormalize (vlasovSolver21.hxx:468-524), kernelMass = 0 */
-static void renormalize(orc_state *s, double norm)
+/* vlasovSolver21::renormalize(norm, which) (vlasovSolver21.hxx:468-524) for
+ * species k, kernelMass = 0 */
+static void renormalize(orc_state *s, double norm, int k)
 {
   if (norm <= 0) return;
   /* set-up runs on the host in the reference's order; in the device order
    * mode the serial (one-thread) order is used */
   int T = s->p.nthreads;
   if (T < 1) s->p.nthreads = 1;
-  double mass = integrate(s, s->f, IG_F) + 0.0;
+  double *fk = s->f + (size_t)k * s->nVal;
+  double mass = integrate(s, fk, IG_F, 1) + 0.0;
   s->p.nthreads = T;
   double fac = norm / mass;
-  for (long i = 0; i < s->nVal; i++) s->f[i] *= fac;
+  for (long i = 0; i < s->nVal; i++) fk[i] *= fac;
   for (long c = 0; c < s->nChains; c++)
-    for (long k = 0; k < s->chains[c].n; k++) s->chains[c].val[k] *= fac;
+    for (long q = 0; q < s->chains[c].n; q++) s->chains[c].val[q * s->nS + k] *= fac;
 }
 
 /* ------------------------------------------------------------------ */
@@ -763,16 +790,17 @@ static int thread_id(void)
 #endif
 }
 
-static void build_all(orc_state *s)
+/* the splines of every slice of the plane f (setupSplines for one species) */
+static void build_all(orc_state *s, const double *f)
 {
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < s->nJ; j++)
-    build_slice(s, s->tmpArr + (size_t)thread_id() * s->tmpStride, s->f + j * s->nU * s->nR,
+    build_slice(s, s->tmpArr + (size_t)thread_id() * s->tmpStride, f + j * s->nU * s->nR,
                 s->coef + (size_t)j * s->cstride);
 }
 
-/* kick loop, vlasovSolver21_spline.hxx:750-765 */
-static void kick(orc_state *s)
+/* kick loop of one species (plane f), vlasovSolver21_spline.hxx:750-765 */
+static void kick(orc_state *s, double *f)
 {
   const double Ulo = s->U_Val[0], Uhi = s->U_Val[s->nU - 1];
 #pragma omp parallel for schedule(static)
@@ -782,57 +810,67 @@ static void kick(orc_state *s)
       for (long r = 0; r < s->nR; r++) {
         long i = (j * s->nU + u) * s->nR + r;
         double newU = s->U_Val[u] + s->Mr[r];
-        if ((newU <= Ulo) || (newU >= Uhi)) s->f[i] = 0;
-        else s->f[i] = evalNxPy(s, c, (int)r, newU);
+        if ((newU <= Ulo) || (newU >= Uhi)) f[i] = 0;
+        else f[i] = evalNxPy(s, c, (int)r, newU);
       }
   }
 }
 
-/* drift loop, vlasovSolver21_spline.hxx:769-798 (delayed-kernel back values
- * :776-786, then per point :788-796) */
-static void drift(orc_state *s)
+/* drift loop of species k, vlasovSolver21_spline.hxx:769-798 (delayed-kernel
+ * back values :776-786, then per point :788-796); the delay lines turn on the
+ * last species' reads (:793) */
+static void drift(orc_state *s, int k)
 {
   const double Ulo = s->U_Val[0], Uhi = s->U_Val[s->nU - 1];
+  const int nS = s->nS, roll = k == nS - 1;
+  double *f = s->f + (size_t)k * s->nVal;
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < s->nJ; j++) {
     const double *c = s->coef + (size_t)j * s->cstride;
     if (s->p.kernelType == ORC_DELAYED)
-      for (long k = s->chainStart[j]; k < s->chainStart[j + 1]; k++) {
-        chain_t *ch = &s->chains[k];
+      for (long q = s->chainStart[j]; q < s->chainStart[j + 1]; q++) {
+        chain_t *ch = &s->chains[q];
         double R = ch->R[ch->n - 1], U = ch->U[ch->n - 1];
-        if ((U <= Ulo) || (U >= Uhi)) ch->val[ch->n - 1] = 0;
-        else ch->val[ch->n - 1] = eval_xy(s, c, R, U);
+        double *bv = &ch->val[(ch->n - 1) * nS + k];
+        if ((U <= Ulo) || (U >= Uhi)) *bv = 0;
+        else *bv = eval_xy(s, c, R, U);
       }
     for (long u = 0; u < s->nU; u++)
       for (long r = 0; r < s->nR; r++) {
         long i = (j * s->nU + u) * s->nR + r;
-        if (s->driftR[i] < s->kernelRadius) s->f[i] = chain_read_roll(&s->chains[s->chainOfPoint[i]]);
-        else if (s->driftOut[i]) s->f[i] = 0;
-        else s->f[i] = evalC(s->stride1, c, s->driftR_I[i], s->driftR_F[i], s->driftU_I[i], s->driftU_F[i]);
+        if (s->driftR[i] < s->kernelRadius) f[i] = chain_read(&s->chains[s->chainOfPoint[i]], k, nS, roll);
+        else if (s->driftOut[i]) f[i] = 0;
+        else f[i] = evalC(s->stride1, c, s->driftR_I[i], s->driftR_F[i], s->driftU_I[i], s->driftU_F[i]);
       }
   }
 }
 
 static void moment_and_accel(orc_state *s)
 {
-  integrate1D(s, s->f, s->dM);
+  integrate1D(s, s->f, s->dM, s->nS);
   s->mTot = accumulate(s->dM, s->Mr, s->nR);
   for (long i = 0; i < s->nR; i++) s->Mr[i] *= s->p.G / (s->R_Val[i] * s->R_Val[i]) * s->p.dt;
 }
 
-/* vlasovSolver21_spline::makeOneStep, vlasovSolver21_spline.hxx:680-801 */
+/* vlasovSolver21_spline::makeOneStep, vlasovSolver21_spline.hxx:680-801: every
+ * phase species by species (each species' splines are independent) */
 void orc_step(orc_state *s, int nsteps)
 {
   for (int n = 0; n < nsteps; n++) {
-    if (s->p.noLinking) {
-      build_all(s);
-      drift(s);
-    }
+    if (s->p.noLinking)
+      for (int k = 0; k < s->nS; k++) {
+        build_all(s, s->f + (size_t)k * s->nVal);
+        drift(s, k);
+      }
     moment_and_accel(s);
-    build_all(s);
-    kick(s);
-    build_all(s);
-    drift(s);
+    for (int k = 0; k < s->nS; k++) {
+      build_all(s, s->f + (size_t)k * s->nVal);
+      kick(s, s->f + (size_t)k * s->nVal);
+    }
+    for (int k = 0; k < s->nS; k++) {
+      build_all(s, s->f + (size_t)k * s->nVal);
+      drift(s, k);
+    }
     s->time += s->p.dt;
     s->stepCount++;
   }
@@ -866,21 +904,33 @@ orc_state *orc_create(const orc_params *p)
   orc_gen_scale(p->Umin[1], p->Umax[1], p->Ures[1], p->Uscale[1], 1, s->J_Val);
   orc_gen_scale(p->Umin[1], p->Umax[1], p->Ures[1], p->Uscale[1], 0, s->J_V);
 
-  s->f = calloc((size_t)s->nVal, sizeof(double));
-  if (p->initType == ORC_UDF) udf_setup(s);
-  else if (p->initType == ORC_PLDF) pldf_setup(s);
-  else s->Vg = sqrt((p->G * p->M) / (p->R0));
+  s->nS = p->nSpecies < 1 ? 1 : p->nSpecies;
+  if (s->nS > ORC_MAX_SPECIES ||
+      (s->nS > 1 && p->initType != ORC_UDF && p->initType != ORC_PLDF)) {
+    set_err("several species need UDF or PLDF initial conditions (at most 5 species)");
+    free(s);
+    return NULL;
+  }
+  s->f = calloc((size_t)s->nVal * s->nS, sizeof(double));
+  for (int k = 0; k < s->nS; k++) {
+    if (p->initType == ORC_UDF) udf_setup(s, k);
+    else if (p->initType == ORC_PLDF) pldf_setup(s, k);
+  }
+  if (p->initType != ORC_UDF && p->initType != ORC_PLDF) s->Vg = sqrt((p->G * p->M) / (p->R0));
   /* generate (UDF :182-208, PLDF :197-224, Hernquist :163-204, Plummer
    * initGen_Plummer.hxx:141-162: Phi at the r nodes, as valueAt) */
+  for (int k = 0; k < s->nS; k++) {
+    double *fk = s->f + (size_t)k * s->nVal;
 #pragma omp parallel for schedule(static)
-  for (long j = 0; j < s->nJ; j++) {
-    double J = s->J_Val[j];
-    for (long u = 0; u < s->nU; u++)
-      for (long r = 0; r < s->nR; r++) {
-        long i = (j * s->nU + u) * s->nR + r;
-        if (p->initType == ORC_HERNQUIST) s->f[i] = hq_F(s, hq_phi(s, s->R_Val[r]), s->R_Val[r], s->U_Val[u], J);
-        else s->f[i] = value_at(s, s->R_Val[r], s->U_Val[u], J);
-      }
+    for (long j = 0; j < s->nJ; j++) {
+      double J = s->J_Val[j];
+      for (long u = 0; u < s->nU; u++)
+        for (long r = 0; r < s->nR; r++) {
+          long i = (j * s->nU + u) * s->nR + r;
+          if (p->initType == ORC_HERNQUIST) fk[i] = hq_F(s, hq_phi(s, s->R_Val[r]), s->R_Val[r], s->U_Val[u], J);
+          else fk[i] = value_at(s, k, s->R_Val[r], s->U_Val[u], J);
+        }
+    }
   }
 
   /* spline set-up, vlasovSolver21_spline.hxx:296-314 + localSpline::init :60-93 */
@@ -939,8 +989,9 @@ orc_state *orc_create(const orc_params *p)
     pos2coord(s, s->driftR[i], 0, &s->driftR_I[i], &s->driftR_F[i]);
     pos2coord(s, s->driftU[i], 1, &s->driftU_I[i], &s->driftU_F[i]);
   }
-  /* vlasovSolver::init renormalizes to the generator's mass (:94-95) */
-  renormalize(s, p->M);
+  /* vlasovSolver::init renormalizes each species to its mass (:94-95) */
+  for (int k = 0; k < s->nS; k++)
+    renormalize(s, p->initType == ORC_UDF || p->initType == ORC_PLDF ? p->spM[k] : p->M, k);
   s->dM = calloc(s->nR, sizeof(double));
   s->Mr = calloc(s->nR, sizeof(double));
   return s;
@@ -975,19 +1026,20 @@ static double evalNxPy_dU(const orc_state *s, const double *data, int Nx, double
 }
 
 /* overwrite the delay-line values, cells front first in chain order */
+/* cell values in the layout of orc_get_chains (nS per cell) */
 void orc_set_chain_vals(orc_state *s, const double *vals)
 {
   long off = 0;
   for (long c = 0; c < s->nChains; c++) {
-    memcpy(s->chains[c].val, vals + off, s->chains[c].n * sizeof(double));
+    memcpy(s->chains[c].val, vals + off * s->nS, s->chains[c].n * s->nS * sizeof(double));
     off += s->chains[c].n;
   }
 }
 
 int orc_tl_init(orc_state *s, const double *df, const double *dvals)
 {
-  if (s->p.noLinking || s->p.Uscale[0] != ORC_LINEAR) {
-    set_err("tangent-linear model: linking steps and a linear v scale only");
+  if (s->p.noLinking || s->p.Uscale[0] != ORC_LINEAR || s->nS != 1) {
+    set_err("tangent-linear model: linking steps, a linear v scale and one species only");
     return -1;
   }
   if (!s->tf) {
@@ -1013,9 +1065,9 @@ void orc_tl_get(const orc_state *s, double *df) { memcpy(df, s->tf, s->nVal * si
 void orc_tl_mass(orc_state *s, double *M, double *dM)
 {
   double *tmp = malloc(s->nR * sizeof(double));
-  integrate1D(s, s->f, tmp);
+  integrate1D(s, s->f, tmp, 1);
   accumulate(tmp, M, s->nR);
-  integrate1D(s, s->tf, tmp);
+  integrate1D(s, s->tf, tmp, 1);
   accumulate(tmp, dM, s->nR);
   free(tmp);
 }
@@ -1034,12 +1086,12 @@ static void tl_step1(orc_state *s)
   /* moment and acceleration, and their tangents */
   moment_and_accel(s);
   double *tdM = malloc(nR * sizeof(double));
-  integrate1D(s, s->tf, tdM);
+  integrate1D(s, s->tf, tdM, 1);
   accumulate(tdM, s->tMr, nR);
   for (long i = 0; i < nR; i++) s->tMr[i] *= s->p.G / (s->R_Val[i] * s->R_Val[i]) * s->p.dt;
   free(tdM);
   /* kick */
-  build_all(s);
+  build_all(s, s->f);
   tl_build(s);
   const double Ulo = s->U_Val[0], Uhi = s->U_Val[nU - 1];
 #pragma omp parallel for schedule(static)
@@ -1060,7 +1112,7 @@ static void tl_step1(orc_state *s)
       }
   }
   /* drift with the delay lines */
-  build_all(s);
+  build_all(s, s->f);
   tl_build(s);
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < nJ; j++) {
@@ -1083,7 +1135,7 @@ static void tl_step1(orc_state *s)
           for (long k = 0; k + 1 < ch->n; k++) ch->tval[k] = ch->tval[k + 1];
           ch->tval[ch->n - 1] = t0;
           s->tf[i] = t0;
-          s->f[i] = chain_read_roll(ch);
+          s->f[i] = chain_read(ch, 0, 1, 1);
         } else if (s->driftOut[i]) {
           s->f[i] = 0;
           s->tf[i] = 0;
@@ -1133,8 +1185,19 @@ void orc_dims(const orc_state *s, long dims[3])
   dims[2] = s->nJ;
 }
 long orc_nval(const orc_state *s) { return s->nVal; }
-void orc_get_f(const orc_state *s, double *out) { memcpy(out, s->f, s->nVal * sizeof(double)); }
-void orc_set_f(orc_state *s, const double *in) { memcpy(s->f, in, s->nVal * sizeof(double)); }
+int orc_nspecies(const orc_state *s) { return s->nS; }
+/* the planes <-> the reference's interleaved fields (gridBase::getDataPtr,
+ * grid_base.hxx:385: arr[i * nFields + s]) */
+void orc_get_f(const orc_state *s, double *out)
+{
+  for (int k = 0; k < s->nS; k++)
+    for (long i = 0; i < s->nVal; i++) out[i * s->nS + k] = s->f[(size_t)k * s->nVal + i];
+}
+void orc_set_f(orc_state *s, const double *in)
+{
+  for (int k = 0; k < s->nS; k++)
+    for (long i = 0; i < s->nVal; i++) s->f[(size_t)k * s->nVal + i] = in[i * s->nS + k];
+}
 double orc_kernel_radius(const orc_state *s) { return s->kernelRadius; }
 
 void orc_get_axes(const orc_state *s, double *R_Val, double *U_Val, double *J_Val, double *J_V)
@@ -1156,7 +1219,7 @@ void orc_moments(orc_state *s, double *dM, double *Mexcl, double *mTot)
 {
   double *d = malloc(s->nR * sizeof(double));
   double *m = malloc(s->nR * sizeof(double));
-  integrate1D(s, s->f, d);
+  integrate1D(s, s->f, d, s->nS);
   double t = accumulate(d, m, s->nR);
   if (dM) memcpy(dM, d, s->nR * sizeof(double));
   if (Mexcl) memcpy(Mexcl, m, s->nR * sizeof(double));
@@ -1174,7 +1237,7 @@ void orc_moments(orc_state *s, double *dM, double *Mexcl, double *mTot)
 void orc_stats(orc_state *s, double out[7])
 {
   double *dM = malloc(s->nR * sizeof(double));
-  integrate1D(s, s->f, dM);
+  integrate1D(s, s->f, dM, s->nS); /* int21::f(0, nSpecies) */
   double M = 0.0 + 0.0, Ep = 0; /* Mr[0] + kernelMass */
   for (long i = 1; i < s->nR; i++) {
     double r = 0.5 * (s->R_Val[i - 1] + s->R_Val[i]);
@@ -1183,12 +1246,39 @@ void orc_stats(orc_state *s, double out[7])
   }
   M += dM[s->nR - 1];
   free(dM);
-  double Ek = integrate(s, s->f, IG_EK);
+  /* the default integrands read the first field only (get_avg(),
+   * slice21Iterator.hxx:90-94), whatever the number of species */
+  double Ek = integrate(s, s->f, IG_EK, 1);
   double E = Ek + Ep;
-  double S = integrate(s, s->f, IG_S);
-  double L1 = integrate(s, s->f, IG_L1);
-  double L2 = integrate(s, s->f, IG_L2);
+  double S = integrate(s, s->f, IG_S, 1);
+  double L1 = integrate(s, s->f, IG_L1, 1);
+  double L2 = integrate(s, s->f, IG_L2, 1);
   out[0] = M; out[1] = E; out[2] = Ek; out[3] = Ep; out[4] = S; out[5] = L1; out[6] = L2;
+}
+
+/* the statistics.txt line with several species (vlasovSolver21.hxx:656-710):
+ * the species loop adds the species line nSpecies times, and S and L2 are
+ * those of the summed fields (int21::entropy(0,nsp), L2Norm(0,nsp)) */
+void orc_stats_total(orc_state *s, double out[7])
+{
+  double line[7];
+  orc_stats(s, line);
+  if (s->nS == 1) {
+    memcpy(out, line, sizeof line);
+    return;
+  }
+  double Ep_tot = 0, M_tot = 0, Ek_tot = 0, E_tot = 0, L1_tot = 0;
+  for (int k = 0; k < s->nS; k++) {
+    Ep_tot += line[3];
+    M_tot += line[0];
+    Ek_tot += line[2];
+    E_tot += line[1];
+    L1_tot += line[5];
+  }
+  out[0] = M_tot; out[1] = E_tot; out[2] = Ek_tot; out[3] = Ep_tot;
+  out[4] = integrate(s, s->f, IG_S, s->nS);
+  out[5] = L1_tot;
+  out[6] = integrate(s, s->f, IG_L2, s->nS);
 }
 
 void orc_get_feet(const orc_state *s, double *driftR, double *driftU, uint8_t *flag)
@@ -1220,7 +1310,8 @@ void orc_get_chains(const orc_state *s, int64_t *point, int64_t *off, int32_t *l
     for (long k = 0; k < ch->n; k++) {
       if (cellR) cellR[o + k] = ch->R[k];
       if (cellU) cellU[o + k] = ch->U[k];
-      if (cellVal) cellVal[o + k] = ch->val[k];
+      if (cellVal)
+        for (int m = 0; m < s->nS; m++) cellVal[(o + k) * s->nS + m] = ch->val[k * s->nS + m];
     }
     o += ch->n;
   }

@@ -26,6 +26,8 @@ enum { ORC_LINEAR = 0, ORC_LOG = 1, ORC_QUADRATIC = 2, ORC_CUBIC = 3, ORC_QUARTI
 enum { ORC_UDF = 0, ORC_PLDF = 1, ORC_HERNQUIST = 2, ORC_PLUMMER = 3 };
 /* kernel types (vlasovSolver21.hxx:66-68) */
 enum { ORC_DELAYED = 0, ORC_REFLECTIVE = 1 };
+/* initGen::MAX_SPECIES (initGen.hxx:21) */
+enum { ORC_MAX_SPECIES = 5 };
 
 /* Fully resolved run parameters (defaults already applied by the caller). */
 typedef struct orc_params {
@@ -44,6 +46,13 @@ typedef struct orc_params {
   double M, R0, alpha, smoothingRadius, gap0, gap1; /* UDF */
   double eta, n;                                    /* PLDF (R0, alpha, M, smoothingRadius shared) */
   double Ra;                                        /* Hernquist */
+  /* species (solver.nSpecies, vlasovSolver.hxx:72): UDF and PLDF take one
+   * value of each parameter per species (species 0's equal the scalars
+   * above); Hernquist and Plummer have one species */
+  int nSpecies;
+  double spM[ORC_MAX_SPECIES], spR0[ORC_MAX_SPECIES], spAlpha[ORC_MAX_SPECIES];
+  double spSR[ORC_MAX_SPECIES], spGap0[ORC_MAX_SPECIES], spGap1[ORC_MAX_SPECIES];
+  double spEta[ORC_MAX_SPECIES], spN[ORC_MAX_SPECIES];
   /* Reduction order.  nthreads >= 1: the reference's order with that many
    * OpenMP threads under a static schedule (quadrature21.hxx:96-139).
    * nthreads == 0: the device order, which the HIP kernels use: per slice
@@ -59,7 +68,9 @@ void orc_destroy(orc_state *s);
 const char *orc_last_error(void);
 
 void orc_dims(const orc_state *s, long dims[3]);      /* nR, nU, nJ */
-long orc_nval(const orc_state *s);
+long orc_nval(const orc_state *s);                    /* points (per species) */
+int orc_nspecies(const orc_state *s);
+/* f in the reference's layout [nJ][nU][nR][nSpecies] (species fastest) */
 void orc_get_f(const orc_state *s, double *out);
 void orc_set_f(orc_state *s, const double *in);
 void orc_get_axes(const orc_state *s, double *R_Val, double *U_Val, double *J_Val,
@@ -72,15 +83,19 @@ void orc_step(orc_state *s, int nsteps);
 void orc_last_moments(const orc_state *s, double *dM, double *accel, double *mTot);
 /* moment of the current f without stepping: dM[nR], Mexcl[nR], mTot */
 void orc_moments(orc_state *s, double *dM, double *Mexcl, double *mTot);
-/* statistics (vlasovSolver21.hxx:644-711): M, E, Ek, Ep, S, L1, L2 */
+/* statistics (vlasovSolver21.hxx:644-711): M, E, Ek, Ep, S, L1, L2 as the
+ * species files statistics_S%d.txt hold them (with several species M and
+ * Ep count every species, Ek, S, L1 and L2 species 0 only, in every file);
+ * total: the line of statistics.txt (the same line with one species) */
 void orc_stats(orc_state *s, double out[7]);
+void orc_stats_total(orc_state *s, double out[7]);
 
 /* feet / chain introspection (for fixtures and device-table checks) */
 void orc_get_feet(const orc_state *s, double *driftR, double *driftU, uint8_t *flag);
 long orc_num_chains(const orc_state *s);
 long orc_chain_cells(const orc_state *s);
-/* per chain: point index, slice j, offset into cells, length; per cell R, U, val
- * (list order, front first) */
+/* per chain: point index, slice j, offset into cells, length; per cell R, U, and
+ * nSpecies values (list order, front first; cellVal[cell * nSpecies + s]) */
 void orc_get_chains(const orc_state *s, int64_t *point, int64_t *off, int32_t *len,
                     double *cellR, double *cellU, double *cellVal);
 
@@ -104,6 +119,7 @@ long orc_gen_scale(double start, double stop, long N, int sct, int cell, double 
  * and its tangent together; orc_tl_mass returns the exclusive-scan mass
  * profile M(<R_r) of the current f and its tangent. */
 int orc_tl_init(orc_state *s, const double *df, const double *dvals);
+/* the delay-line cell values, in the layout of orc_get_chains */
 void orc_set_chain_vals(orc_state *s, const double *vals);
 int orc_tl_step(orc_state *s, int nsteps);
 void orc_tl_get(const orc_state *s, double *df);

@@ -59,6 +59,19 @@ PLUMMER_SMALL = {"init.type": "Plummer", "init.M": 1, "init.R0": 1, "grid.Pres":
                  "grid.Ures": [40, 20], "solver.dt": 0.05}
 SMALL_CASES["plummer"] = (PLUMMER_SMALL, 40)
 
+# Two species (solver.nSpecies 2, one value of each UDF parameter per
+# species) with the reflective kernel.  With the delayed kernel the
+# reference's back cells hold one value (dummyCell21(R,U,1),
+# vlasovSolver21.hxx:322) and its species-1 writes and reads fall outside
+# them, so the reference is pinned on this case and the delayed one
+# (UDF_2SPECIES_DELAYED) is checked against the oracle's well-defined
+# semantics (every cell holds each species, back cells start at 0).
+UDF_2SPECIES = dict(UDF_C1, **{"grid.Pres": 24, "grid.Ures": [20, 12], "solver.nSpecies": 2,
+                               "init.M": [1, 0.5], "init.R0": [2, 1.5],
+                               "init.alpha": [0.5, 0.8], "solver.kernelType": "reflective"})
+SMALL_CASES["udf_2species_reflective"] = (UDF_2SPECIES, 30)
+UDF_2SPECIES_DELAYED = dict(UDF_2SPECIES, **{"solver.kernelType": "delayed"})
+
 # Strong kick: accelerations that move f by tens of v cells in one step, so
 # that the target rows of the kick spread widely across neighbouring r nodes
 # (physically extreme, numerically valid).

@@ -43,7 +43,11 @@ def test_trajectory_bit_exact(name):
         dMg, ag, mg = gpu.moments()
         dMo, ao, mo = orc.last_moments()
         assert same(dMg, dMo) and same(ag, ao) and mg == mo, f"step {k}: moments differ"
-    sg, so = gpu.stats(), orc.stats()
+    _stats_match(gpu.stats(), orc.stats())
+    _stats_match(gpu.stats_total(), orc.stats_total())
+
+
+def _stats_match(sg, so):
     for key in ("M", "E", "Ek", "Ep", "L1", "L2"):
         assert sg[key] == so[key], (key, sg[key], so[key])
     # the entropy uses the device log, which may differ from glibc by an ulp
@@ -136,3 +140,25 @@ def test_streaming_solvers_bit_exact(monkeypatch, pres, ures):
     print(f"case udf {pres}x{ures} with VLS_SPLINE_STREAM", flush=True)
     monkeypatch.setenv("VLS_SPLINE_STREAM", "1")
     _bit_exact_run(cfg, 8, "streaming solvers")
+
+
+def test_two_species_delayed_bit_exact():
+    """Two species sharing the delay lines' rings (each cell holds both
+    species, the rings turn once per step): f of both species, the moment of
+    their sum and both statistics lines, bit for bit."""
+    from tests.configs import UDF_2SPECIES_DELAYED
+
+    print("case udf 2 species delayed", flush=True)
+    gpu = V.Solver(UDF_2SPECIES_DELAYED)
+    orc = O.Oracle(UDF_2SPECIES_DELAYED, nthreads=0)
+    assert gpu.f().shape[-1] == 2 and same(gpu.f(), orc.f())
+    for k in (1, 12):
+        gpu.step(k)
+        orc.step(k)
+        fg, fo = gpu.f(), orc.f()
+        assert np.count_nonzero(fg != fo) == 0
+    dMg, ag, mg = gpu.moments()
+    dMo, ao, mo = orc.last_moments()
+    assert same(dMg, dMo) and same(ag, ao) and mg == mo
+    _stats_match(gpu.stats(), orc.stats())
+    _stats_match(gpu.stats_total(), orc.stats_total())

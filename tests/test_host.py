@@ -60,7 +60,7 @@ def test_command_line_wins_over_file():
     (["-a.b.c", "1"], "ill formated keyword"),
     (["positional", "extra"], "ERROR parsing command line"),
     (["-solver.kernelType", "sticky"], "unknown kernel type"),
-    (["-solver.nSpecies", "2"], "nSpecies"),
+    (["-solver.nSpecies", "6"], "nSpecies"),
     (["-grid.Uscale", "quadratic", "quadratic"], "linear scale"),
 ])
 def test_errors(argv, msg):
@@ -74,3 +74,26 @@ def test_negative_numbers_are_values():
     o = O.Oracle({"grid.Umin": [-1.5, 0], "grid.Umax": [1.5, 1.6], "grid.Pres": 8,
                   "grid.Ures": [8, 4]}, nthreads=1)
     assert np.array_equal(h.f(), o.f())
+
+
+def test_host_two_species_delayed_bit_exact():
+    """Two species with delay lines: every cell carries both species' values
+    (the oracle's semantics where the reference's back cells hold one)."""
+    from tests.configs import UDF_2SPECIES_DELAYED
+
+    h = V.HostSetup(UDF_2SPECIES_DELAYED)
+    o = O.Oracle(UDF_2SPECIES_DELAYED, nthreads=1)
+    assert h.shape[-1] == 2 and np.array_equal(h.f(), o.f())
+    ch, co = h.chains(), o.chains()
+    assert len(ch["point"]) > 0
+    for k in ("point", "len", "R", "U", "val"):
+        assert np.array_equal(ch[k], co[k]), k
+    # the back cell of every chain starts at 0 for both species
+    back = np.cumsum(ch["len"]) - 1
+    assert np.all(ch["val"][back] == 0)
+
+
+def test_species_limited_to_udf_and_pldf():
+    with pytest.raises(V.VlsError, match="nSpecies"):
+        V.HostSetup({"init.type": "Hernquist", "solver.nSpecies": 2, "grid.Pres": 16,
+                     "grid.Ures": [16, 8]})

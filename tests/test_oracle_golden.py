@@ -32,11 +32,12 @@ def test_trajectory_bytes(name):
     o = O.Oracle(cfg, nthreads=1)
     assert sha(o.f()) == fx["sha_f0"], "initial conditions differ"
     assert len(o.chains()["len"]) == fx["n_chains"]
-    # statistics are written before the step of their row (vlasovSolver.hxx:170-174)
+    # statistics are written before the step of their row (vlasovSolver.hxx:170-174);
+    # statistics.txt holds the line of all species
     rows = {int(r[1]): r for r in fx["stats"]}
     for k in range(nsteps + 1):
         if k in rows:
-            s = o.stats()
+            s = o.stats_total()
             ref = rows[k]
             for i, key in enumerate(("M", "E", "Ek", "Ep", "S", "L1", "L2")):
                 assert s[key] == pytest.approx(ref[2 + i], rel=2e-9, abs=1e-300), (k, key)

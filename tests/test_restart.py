@@ -42,6 +42,38 @@ def test_host_reads_reference_restart(tmp_path):
     assert np.array_equal(ch["val"], _front_first(state))
 
 
+@needs_ref
+def test_host_reads_reference_two_species_restart(tmp_path):
+    """A restart the reference writes with two species (nFields = 2, reflective
+    kernel): f interleaved per point."""
+    from tests.configs import UDF_2SPECIES
+
+    path = _ref_final(UDF_2SPECIES, 6, tmp_path)
+    state = vlsfile.read(path)
+    assert state["grid"]["nFields"] == 2 and state["f"].shape[-1] == 2
+    h = V.HostSetup({"reload": str(path.with_suffix(""))})
+    assert h.n_species == 2 and np.array_equal(h.f(), state["f"])
+
+
+@pytest.mark.gpu
+def test_gpu_two_species_restart_round_trip(tmp_path):
+    """Two species with delay lines: f and both species' ring values survive
+    write -> reload, and the continuation is bit-identical."""
+    from tests.configs import UDF_2SPECIES_DELAYED
+
+    print("case udf 2 species restart round trip", flush=True)
+    a = V.Solver(UDF_2SPECIES_DELAYED)
+    a.step(6)
+    a.write_restart(tmp_path / "mid")
+    state = vlsfile.read(tmp_path / "mid.vls")
+    assert np.array_equal(state["f"], a.f())
+    b = V.Solver({"reload": str(tmp_path / "mid")})
+    assert np.array_equal(b.f(), a.f())
+    a.step(5)
+    b.step(5)
+    assert np.array_equal(a.f(), b.f())
+
+
 @pytest.mark.gpu
 def test_gpu_restart_round_trip(tmp_path):
     print("case udf_tiny restart round trip", flush=True)

@@ -30,7 +30,8 @@ EXPORTS = (
     "vls_copy_probe", "vlsd_host_create", "vlsd_host_destroy", "vlsd_host_info", "vlsd_host_f",
     "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
     "vls_record", "vls_adjoint", "vls_set_graph", "vlsd_ensemble_create", "vlsd_ensemble_destroy",
-    "vlsd_ensemble_context",
+    "vlsd_ensemble_context", "vls_stats_total", "vlsd_species", "vlsd_stats_total",
+    "vlsd_host_species",
 )
 
 
@@ -50,7 +51,7 @@ class Geometry(C.Structure):
 
 class SolverParams(C.Structure):
     _fields_ = [("G", C.c_double), ("dt", C.c_double), ("no_linking", C.c_int32),
-                ("kernel_type", C.c_int32)]
+                ("kernel_type", C.c_int32), ("n_species", C.c_int32)]
 
 
 _lib = None
@@ -79,6 +80,7 @@ def lib():
         L.vls_synchronize.argtypes = [vp]
         L.vls_moments.argtypes = [vp, C.c_int, P(d), P(d), P(d)]
         L.vls_stats.argtypes = [vp, C.c_int, P(d)]
+        L.vls_stats_total.argtypes = [vp, C.c_int, P(d), P(d)]
         L.vls_download_coef.argtypes = [vp, C.c_int, C.c_int, P(d)]
         L.vls_set_timing.argtypes = [vp, C.c_int]
         L.vls_kernel_times.argtypes = [vp, P(C.c_char_p), P(d), P(C.c_int)]
@@ -89,6 +91,9 @@ def lib():
         L.vlsd_step.argtypes = [vp, C.c_int]
         L.vlsd_get_f.argtypes = [vp, P(d)]
         L.vlsd_stats.argtypes = [vp, P(d)]
+        L.vlsd_stats_total.argtypes = [vp, P(d)]
+        L.vlsd_species.argtypes = [vp]
+        L.vlsd_host_species.argtypes = [vp]
         L.vlsd_moments.argtypes = [vp, P(d), P(d), P(d)]
         L.vlsd_run.argtypes = [vp]
         L.vlsd_write_restart.argtypes = [vp, C.c_char_p]
@@ -159,7 +164,9 @@ class Solver:
         dims = (C.c_int64 * 3)()
         _check(lib().vlsd_info(self._h, dims, None, None), driver=True)
         self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
-        self.shape = (self.nJ, self.nU, self.nR)
+        self.n_species = lib().vlsd_species(self._h)
+        # the reference's layout: species fastest when there are several
+        self.shape = (self.nJ, self.nU, self.nR) + ((self.n_species,) if self.n_species > 1 else ())
 
     def close(self):
         if getattr(self, "_h", None):
@@ -193,8 +200,15 @@ class Solver:
         return out
 
     def stats(self):
+        """the statistics line (of the species files with several species)"""
         out = np.empty(7)
         _check(lib().vlsd_stats(self._h, _ptr(out)), driver=True)
+        return dict(zip(STAT_NAMES, out))
+
+    def stats_total(self):
+        """the statistics.txt line"""
+        out = np.empty(7)
+        _check(lib().vlsd_stats_total(self._h, _ptr(out)), driver=True)
         return dict(zip(STAT_NAMES, out))
 
     def moments(self):
@@ -275,16 +289,16 @@ class Context:
         _check(lib().vls_set_timing(self._h, int(bool(on))))
 
     def info(self, run=0):
-        """{nR, nU, nJ, chains, cells} of a run"""
-        out = (C.c_int64 * 5)()
+        """{nR, nU, nJ, chains, cells, species} of a run"""
+        out = (C.c_int64 * 6)()
         _check(lib().vls_run_info(self._h, run, out))
-        return dict(zip(("nR", "nU", "nJ", "chains", "cells"), list(out)))
+        return dict(zip(("nR", "nU", "nJ", "chains", "cells", "species"), list(out)))
 
     def get_chains(self, run=0):
-        """(ring heads, ring values) of the delay lines"""
+        """(ring heads, ring values: species fastest) of the delay lines"""
         inf = self.info(run)
         head = np.zeros(inf["chains"], np.int32)
-        vals = np.zeros(inf["cells"])
+        vals = np.zeros(inf["cells"] * inf["species"])
         _check(lib().vls_get_chains(self._h, run, _ptr(head, C.c_int32), _ptr(vals)))
         return head, vals
 
@@ -393,7 +407,9 @@ class HostSetup:
         nc, ncell, kr = i64(), i64(), d()
         _check(L.vlsd_host_info(h, dims, C.byref(nc), C.byref(ncell), C.byref(kr)), driver=True)
         self.nR, self.nU, self.nJ = dims[0], dims[1], dims[2]
-        self.shape = (self.nJ, self.nU, self.nR)
+        self.n_species = L.vlsd_host_species(h)
+        self.points = (self.nJ, self.nU, self.nR)
+        self.shape = self.points + ((self.n_species,) if self.n_species > 1 else ())
         self.n_chains, self.n_cells, self.kernel_radius = nc.value, ncell.value, kr.value
 
     def close(self):
@@ -413,9 +429,9 @@ class HostSetup:
         return out
 
     def feet(self):
-        R = np.empty(self.shape)
-        U = np.empty(self.shape)
-        fl = np.empty(self.shape, np.uint8)
+        R = np.empty(self.points)
+        U = np.empty(self.points)
+        fl = np.empty(self.points, np.uint8)
         _check(lib().vlsd_host_feet(self._h, _ptr(R), _ptr(U), _ptr(fl, C.c_uint8)), driver=True)
         return R, U, fl
 
@@ -424,7 +440,9 @@ class HostSetup:
         ln = np.empty(self.n_chains, np.int32)
         R = np.empty(self.n_cells)
         U = np.empty(self.n_cells)
-        V = np.empty(self.n_cells)
+        V = np.empty(self.n_cells * self.n_species)
         _check(lib().vlsd_host_chains(self._h, _ptr(point, C.c_int64), _ptr(ln, C.c_int32),
                                       _ptr(R), _ptr(U), _ptr(V)), driver=True)
+        if self.n_species > 1:
+            V = V.reshape(self.n_cells, self.n_species)
         return dict(point=point, len=ln, R=R, U=U, val=V)

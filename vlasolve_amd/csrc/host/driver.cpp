@@ -68,6 +68,8 @@ int vlsd_info(vlsd_solver *s, int64_t dims[3], double *time, int64_t *stepCount)
   });
 }
 
+int vlsd_species(vlsd_solver *s) { return s ? s->solver.nSpecies() : 0; }
+
 int vlsd_step(vlsd_solver *s, int nSteps) {
   return guarded([&] {
     if (!s || nSteps < 0) throw vlh::Failure("bad arguments");
@@ -86,6 +88,13 @@ int vlsd_stats(vlsd_solver *s, double out[7]) {
   return guarded([&] {
     if (!s || !out) throw vlh::Failure("bad arguments");
     s->solver.stats(out);
+  });
+}
+
+int vlsd_stats_total(vlsd_solver *s, double out[7]) {
+  return guarded([&] {
+    if (!s || !out) throw vlh::Failure("bad arguments");
+    s->solver.statsTotal(out);
   });
 }
 
@@ -203,10 +212,12 @@ int vlsd_host_info(vlsd_host *h, int64_t dims[3], int64_t *nChains, int64_t *nCe
       dims[2] = g.nJ;
     }
     if (nChains) *nChains = (int64_t)h->solver.chains().point.size();
-    if (nCells) *nCells = (int64_t)h->solver.chains().val.size();
+    if (nCells) *nCells = (int64_t)h->solver.chains().cellR.size();
     if (kernelRadius) *kernelRadius = h->solver.kernelRadius();
   });
 }
+
+int vlsd_host_species(vlsd_host *h) { return h ? h->solver.nSpecies() : 0; }
 
 int vlsd_host_f(vlsd_host *h, double *f) {
   return guarded([&] {

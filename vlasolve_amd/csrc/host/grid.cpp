@@ -133,34 +133,42 @@ vls_geometry Grid21::geometry() const {
 
 // ------------------------------------------------------------ generators
 
-void InitGen::generate(const Grid21 &g, double *f) const {
+void InitGen::generate(const Grid21 &g, double *f, int nS) const {
+  // species s of point i at f[i * nS + s] (gridBase::getDataPtr, src/grid_base.hxx:385)
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < g.nJ; j++)
     for (int u = 0; u < g.nU; u++)
-      for (int r = 0; r < g.nR; r++) f[g.index(j, u, r)] = valueAt(g.R_Val[r], g.U_Val[u], g.J_Val[j]);
+      for (int r = 0; r < g.nR; r++)
+        for (int s = 0; s < nS; s++)
+          f[g.index(j, u, r) * nS + s] = valueAt(g.R_Val[r], g.U_Val[u], g.J_Val[j], s);
 }
 
 namespace {
 
 // Uniform-density sphere with erf edge (initGen_UniformDensitySphere.hxx)
 class UDF : public InitGen {
-  double M = 1, R0 = 2, alpha = 0.5, sr = 0.5, gap[2] = {0, 0};
-  double rho0 = 0, C1 = 0, C2 = 0;
+  // one value per species (initGen::MAX_SPECIES = 5, src/initGen.hxx:21)
+  double M[5] = {}, R0[5] = {}, alpha[5] = {}, sr[5] = {}, gap0[5] = {}, gap1[5] = {};
+  double rho0[5] = {}, C1[5] = {}, C2[5] = {};
 
  public:
   std::string type() const override { return "UDF"; }
-  void setup(const Params &prm, const SolverParams &) override {
-    // :51-74
-    R0 = prm.get<double>("R0", "init", 2.0, 0);
-    M = prm.get<double>("M", "init", 1.0, 0);
-    alpha = prm.get<double>("alpha", "init", 0.5, 0);
-    sr = prm.get<double>("smoothingRadius", "init", 0.5, 0);
-    gap[0] = prm.get<double>("gap", "init", 0.0, 0);
-    gap[1] = prm.get<double>("gap", "init", 0.0, 1);
-    rho0 = (3 * M) / (4 * M_PI * powiRef(R0, 3));
-    double sig0 = std::sqrt((alpha) / (5 * R0));
-    C1 = rho0 * std::pow(2 * M_PI * sig0 * sig0, -1.5);
-    C2 = 2 * sig0 * sig0;
+  int maxSpecies() const override { return 5; }
+  void setup(const Params &prm, const SolverParams &sp) override {
+    // :43-74: value i of each parameter for species i (the gap's two values
+    // at 2i, 2i+1)
+    for (int i = 0; i < std::min(std::max(sp.nSpecies, 1), 5); i++) {
+      R0[i] = prm.get<double>("R0", "init", 2.0, i);
+      M[i] = prm.get<double>("M", "init", 1.0, i);
+      alpha[i] = prm.get<double>("alpha", "init", 0.5, i);
+      sr[i] = prm.get<double>("smoothingRadius", "init", 0.5, i);
+      gap0[i] = prm.get<double>("gap", "init", 0.0, 2 * i);
+      gap1[i] = prm.get<double>("gap", "init", 0.0, 2 * i + 1);
+      rho0[i] = (3 * M[i]) / (4 * M_PI * powiRef(R0[i], 3));
+      double sig0 = std::sqrt((alpha[i]) / (5 * R0[i]));
+      C1[i] = rho0[i] * std::pow(2 * M_PI * sig0 * sig0, -1.5);
+      C2[i] = 2 * sig0 * sig0;
+    }
   }
   GridParams defaultGrid() const override {  // :82-102
     GridParams g;
@@ -175,48 +183,55 @@ class UDF : public InitGen {
     s.G = 1; s.T0 = 0; s.dt = 0.005; s.Tmax = 60.; s.nSpecies = 1;
     return s;
   }
-  double valueAt(double r, double u, double j) const override {  // computeF :158-173
-    double sm = 0.5 * (1. + std::erf((R0 - r) / sr));
-    double res = sm * C1 * std::exp(-(u * u + (j * j) / (r * r)) / C2);
-    if (gap[1] > 0) {
-      double sm0 = 0.5 * (1. + std::erf((gap[0] - r) / 0.15));
-      double sm1 = 0.5 * (1. + std::erf(-(gap[1] - r) / 0.15));
+  double valueAt(double r, double u, double j, int s) const override {  // computeF :158-173
+    double sm = 0.5 * (1. + std::erf((R0[s] - r) / sr[s]));
+    double res = sm * C1[s] * std::exp(-(u * u + (j * j) / (r * r)) / C2[s]);
+    if (gap1[s] > 0) {
+      double sm0 = 0.5 * (1. + std::erf((gap0[s] - r) / 0.15));
+      double sm1 = 0.5 * (1. + std::erf(-(gap1[s] - r) / 0.15));
       res *= (sm0 + sm1);
     }
     if (res < 1.E-200) res = 1.E-200;
     return res;
   }
-  double mass() const override { return M; }
+  double mass(int s) const override { return M[s]; }
 };
 
 // Power-law density sphere (initGen_PowerLawDensitySphere.hxx)
 class PLDF : public InitGen {
-  double M = 1, alpha = 2, eta = 0.1, n = 1, sr = 0.5, R0 = 10;
-  double rho0 = 0, sigO2 = 0, sigR2 = 0, C0 = 0, C1 = 0, C2 = 0;
+  int nS = 1;
+  double M[5] = {}, alpha[5] = {}, eta[5] = {}, n[5] = {}, sr[5] = {}, R0[5] = {};
+  double rho0[5] = {}, sigO2[5] = {}, sigR2[5] = {}, C0[5] = {}, C1[5] = {}, C2[5] = {};
 
  public:
   std::string type() const override { return "PLDF"; }
-  void setup(const Params &prm, const SolverParams &) override {
-    // :50-80
-    M = prm.get<double>("M", "init", 1.0, 0);
-    alpha = prm.get<double>("alpha", "init", 2.0, 0);
-    eta = prm.get<double>("eta", "init", 0.1, 0);
-    n = prm.get<double>("n", "init", 1.0, 0);
-    sr = prm.get<double>("smoothingRadius", "init", 0.5, 0);
-    R0 = 1. / eta;
-    R0 = prm.get<double>("R0", "init", R0, 0);
-    rho0 = ((3.0 - n) * M) / (4 * M_PI * std::pow(R0, 3.0 - n));
-    sigO2 = (3.0 - n) / (5.0 - 2 * n) * (M * eta) / (R0 * (alpha * 0.5 + 1));
-    sigR2 = sigO2 * 0.5 * alpha;
-    C0 = rho0 / (std::sqrt(2 * sigR2 * M_PI * M_PI * M_PI) * sigO2);
-    C1 = 1.0 / (2 * sigR2);
-    C2 = 1.0 / (sigO2);
+  int maxSpecies() const override { return 5; }
+  void setup(const Params &prm, const SolverParams &sp) override {
+    // :44-80: value i of each parameter for species i
+    nS = std::min(std::max(sp.nSpecies, 1), 5);
+    for (int i = 0; i < nS; i++) {
+      M[i] = prm.get<double>("M", "init", 1.0, i);
+      alpha[i] = prm.get<double>("alpha", "init", 2.0, i);
+      eta[i] = prm.get<double>("eta", "init", 0.1, i);
+      n[i] = prm.get<double>("n", "init", 1.0, i);
+      sr[i] = prm.get<double>("smoothingRadius", "init", 0.5, i);
+      R0[i] = 1. / eta[i];
+      R0[i] = prm.get<double>("R0", "init", R0[i], i);
+      rho0[i] = ((3.0 - n[i]) * M[i]) / (4 * M_PI * std::pow(R0[i], 3.0 - n[i]));
+      sigO2[i] = (3.0 - n[i]) / (5.0 - 2 * n[i]) * (M[i] * eta[i]) / (R0[i] * (alpha[i] * 0.5 + 1));
+      sigR2[i] = sigO2[i] * 0.5 * alpha[i];
+      C0[i] = rho0[i] / (std::sqrt(2 * sigR2[i] * M_PI * M_PI * M_PI) * sigO2[i]);
+      C1[i] = 1.0 / (2 * sigR2[i]);
+      C2[i] = 1.0 / (sigO2[i]);
+    }
   }
   GridParams defaultGrid() const override {  // :88-121
     GridParams g;
     double sigma2 = 0;
-    double sig2 = sigO2 + 2 * sigR2;
-    if (sigma2 < sig2) sigma2 = sig2;
+    for (int i = 0; i < nS; i++) {
+      double sig2 = sigO2[i] + 2 * sigR2[i];
+      if (sigma2 < sig2) sigma2 = sig2;
+    }
     g.Pmin = 0.01; g.Pmax = 200;
     g.Umin[0] = -16 * std::sqrt(sigma2); g.Umax[0] = 16 * std::sqrt(sigma2);
     g.Umin[1] = 0; g.Umax[1] = 1.6;
@@ -229,13 +244,14 @@ class PLDF : public InitGen {
     s.G = 1; s.T0 = 0; s.dt = 0.005; s.Tmax = 60.; s.nSpecies = 1;
     return s;
   }
-  double valueAt(double r, double u, double j) const override {  // computeF :181-189
-    double sm = 0.5 * (1. + std::erf((R0 - r) / sr));
-    double res = sm * C0 * std::exp(-u * u * C1 - ((j * j) / (r * r)) * C2) * std::pow(r, -n);
+  double valueAt(double r, double u, double j, int s) const override {  // computeF :181-189
+    double sm = 0.5 * (1. + std::erf((R0[s] - r) / sr[s]));
+    double res =
+        sm * C0[s] * std::exp(-u * u * C1[s] - ((j * j) / (r * r)) * C2[s]) * std::pow(r, -n[s]);
     if (res < 1.E-200) res = 1.E-200;
     return res;
   }
-  double mass() const override { return M; }
+  double mass(int s) const override { return M[s]; }
 };
 
 // Hernquist sphere, isotropic or Osipkov-Merritt (initGen_Hernquist.hxx)
@@ -282,8 +298,8 @@ class Hernquist : public InitGen {
     s.G = 1; s.T0 = 0; s.Tmax = 62; s.dt = 0.025; s.nSpecies = 1;
     return s;
   }
-  double valueAt(double r, double u, double j) const override { return F(phi(r), r, u, j); }
-  void generate(const Grid21 &g, double *f) const override {  // :163-204
+  double valueAt(double r, double u, double j, int) const override { return F(phi(r), r, u, j); }
+  void generate(const Grid21 &g, double *f, int) const override {  // :163-204
     std::vector<double> Phi(g.nR);
     for (int r = 0; r < g.nR; r++) Phi[r] = phi(g.R_Val[r]);
 #pragma omp parallel for schedule(static)
@@ -292,7 +308,7 @@ class Hernquist : public InitGen {
         for (int r = 0; r < g.nR; r++)
           f[g.index(j, u, r)] = F(Phi[r], g.R_Val[r], g.U_Val[u], g.J_Val[j]);
   }
-  double mass() const override { return M; }
+  double mass(int) const override { return M; }
 };
 
 // Plummer sphere, isotropic (initGen_Plummer.hxx)
@@ -326,8 +342,8 @@ class Plummer : public InitGen {
     s.G = 1; s.T0 = 0; s.Tmax = 62; s.dt = 0.1; s.nSpecies = 1;
     return s;
   }
-  double valueAt(double r, double u, double j) const override { return F(phi(r), r, u, j); }
-  void generate(const Grid21 &g, double *f) const override {  // :138-160
+  double valueAt(double r, double u, double j, int) const override { return F(phi(r), r, u, j); }
+  void generate(const Grid21 &g, double *f, int) const override {  // :138-160
     std::vector<double> Phi(g.nR);
     for (int r = 0; r < g.nR; r++) Phi[r] = phi(g.R_Val[r]);
 #pragma omp parallel for schedule(static)
@@ -336,7 +352,7 @@ class Plummer : public InitGen {
         for (int r = 0; r < g.nR; r++)
           f[g.index(j, u, r)] = F(Phi[r], g.R_Val[r], g.U_Val[u], g.J_Val[j]);
   }
-  double mass() const override { return M; }
+  double mass(int) const override { return M; }
 };
 
 }  // namespace

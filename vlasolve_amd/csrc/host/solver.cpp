@@ -199,7 +199,9 @@ void Solver21::init(const Params &prm, const std::string &reload, bool device) {
   sp_.snapshotEvery = prm.get<long>("snapshotEvery", "solver", 500);
   sp_.restartEvery = prm.get<long>("restartEvery", "solver", 5000);
   noLinking_ = prm.get<bool>("noLinking", "solver", false);
-  if (sp_.nSpecies != 1) throw Failure("ERROR: only solver.nSpecies 1 is supported");
+  if (sp_.nSpecies < 1 || sp_.nSpecies > init->maxSpecies())
+    throw Failure("ERROR: solver.nSpecies " + std::to_string(sp_.nSpecies) + " is not supported by " +
+                  initType_ + " initial conditions (at most " + std::to_string(init->maxSpecies()) + ")");
   time_ = sp_.T0;
   init->setup(prm, sp_);
   // gridHandler overrides (src/grid_handler.hxx:221-234)
@@ -215,9 +217,9 @@ void Solver21::init(const Params &prm, const std::string &reload, bool device) {
     gp.Uscale[i] = scaleFromString(prm.get<std::string>("Uscale", "grid", scaleToString(gp.Uscale[i]), i));
   }
   grid_.init(gp);
-  f_.assign(grid_.nVal, 0.0);
+  f_.assign(grid_.nVal * sp_.nSpecies, 0.0);
   if (verbose) { printf("Generating initial conditions (%s[%d]) ... ", initType_.c_str(), sp_.nSpecies); fflush(stdout); }
-  init->generate(grid_, f_.data());
+  init->generate(grid_, f_.data(), sp_.nSpecies);
   if (verbose) printf("done.\n");
   // vlasovSolver21::setup (src/vlasovSolver21.hxx:533-548)
   std::string kt = prm.get<std::string>("kernelType", "solver", "delayed");
@@ -227,7 +229,8 @@ void Solver21::init(const Params &prm, const std::string &reload, bool device) {
   kernelBufferSize_ = prm.get<int>("kernelBufferSize", "solver", 0);
   kernelMass_ = 0;
   setupFeet(&init);
-  renormalize(init->mass());
+  // vlasovSolver::init renormalizes each species to its mass (src/vlasovSolver.hxx:94-95)
+  for (int s = 0; s < sp_.nSpecies; s++) renormalize(init->mass(s), s);
   if (device) upload();
 }
 
@@ -276,32 +279,36 @@ void Solver21::setupFeet(std::unique_ptr<InitGen> *init) {
         if (!(driftR_[i] < kernelRadius_)) continue;
         const double J = g.J_Val[j];
         chains_.point.push_back(i);
-        chains_.off.push_back((int64_t)chains_.val.size());
+        chains_.off.push_back((int64_t)chains_.cellR.size());
         double deltaT = dth, nr, nu;
         if (!advanceDrift(g.R_Val[r], g.U_Val[u], J, deltaT, -1, nr, nu)) throw Failure("ERROR: chain set-up failed");
         int32_t n = 0;
         while (nr < kernelRadius_) {
           chains_.cellR.push_back(nr);
           chains_.cellU.push_back(nu);
-          chains_.val.push_back((*init)->valueAt(nr, nu, J));
+          for (int s = 0; s < sp_.nSpecies; s++) chains_.val.push_back((*init)->valueAt(nr, nu, J, s));
           n++;
           deltaT += dth;
           if (!advanceDrift(g.R_Val[r], g.U_Val[u], J, deltaT, -1, nr, nu)) throw Failure("ERROR: chain set-up failed");
         }
+        // the back cell starts at 0 for every species (the reference makes it
+        // with one value, dummyCell21(R,U,1), src/vlasovSolver21.hxx:322, so
+        // that with several species its species >= 1 fall outside the cell)
         chains_.cellR.push_back(nr);
         chains_.cellU.push_back(nu);
-        chains_.val.push_back(0.0);
+        for (int s = 0; s < sp_.nSpecies; s++) chains_.val.push_back(0.0);
         chains_.len.push_back(n + 1);
         chains_.head.push_back(0);
       }
 }
 
-void Solver21::renormalize(double norm) {
-  // vlasovSolver21::renormalize (src/vlasovSolver21.hxx:468-524) with the
-  // mass of quadrature21::integrate (src/quadrature21.hxx:57-85) summed in the
-  // reference's single-thread order
+void Solver21::renormalize(double norm, int sp) {
+  // vlasovSolver21::renormalize(norm, which = sp) (src/vlasovSolver21.hxx:468-524)
+  // with the mass of species sp by quadrature21::integrate
+  // (src/quadrature21.hxx:57-85) summed in the reference's single-thread order
   if (norm <= 0) return;
   const Grid21 &g = grid_;
+  const int nS = sp_.nSpecies;
   double mass = 0;
   for (int j = 0; j < g.nJ; j++) {
     double Ct1 = 8 * M_PI * M_PI * g.J_Val[j] * g.dJ[j];
@@ -311,16 +318,16 @@ void Solver21::renormalize(double norm) {
         if (dU <= 0) continue;
         double dR = g.dR[r];
         if (dR <= 0) continue;
-        const double *c = &f_[g.index(j, u, r)];
-        double avg = ((*c) + c[1] + c[g.nR] + c[g.nR + 1]) * 0.25;
+        const double *c = &f_[g.index(j, u, r) * nS + sp];
+        double avg = ((*c) + c[nS] + c[(int64_t)g.nR * nS] + c[(int64_t)(g.nR + 1) * nS]) * 0.25;
         mass += Ct1 * dU * dR * avg;
       }
   }
   mass = mass + kernelMass_;
   double fac = norm / mass;
-  if (verbose) printf("Renormalizing mass : M[%d]=%g -> %g.\n", 0, mass, norm);
-  for (auto &v : f_) v *= fac;
-  for (auto &v : chains_.val) v *= fac;
+  if (verbose) printf("Renormalizing mass : M[%d]=%g -> %g.\n", sp, mass, norm);
+  for (size_t i = sp; i < f_.size(); i += nS) f_[i] *= fac;
+  for (size_t i = sp; i < chains_.val.size(); i += nS) chains_.val[i] *= fac;
 }
 
 vls_solver_params Solver21::deviceParams() const {
@@ -329,6 +336,7 @@ vls_solver_params Solver21::deviceParams() const {
   sp.dt = sp_.dt;
   sp.no_linking = noLinking_ ? 1 : 0;
   sp.kernel_type = kernelType_;
+  sp.n_species = sp_.nSpecies;
   return sp;
 }
 
@@ -369,8 +377,11 @@ void Solver21::uploadTo(vls_ctx *ctx, int run) const {
     bxr[c] = bout[c] ? 0.0 : g.splineCoord(chains_.cellR[b], 0);
     bxu[c] = bout[c] ? 0.0 : g.splineCoord(U, 1);
     // ring storage: slot (head + k) % len holds list position k
+    const int nS = sp_.nSpecies;
     for (int32_t k = 0; k < chains_.len[c]; k++)
-      ringVals[chains_.off[c] + (chains_.head[c] + k) % chains_.len[c]] = chains_.val[chains_.off[c] + k];
+      for (int s = 0; s < nS; s++)
+        ringVals[(chains_.off[c] + (chains_.head[c] + k) % chains_.len[c]) * nS + s] =
+            chains_.val[(chains_.off[c] + k) * nS + s];
   }
   check(vls_set_chains(ctx, run, nc, chains_.point.data(), chains_.len.data(), chains_.head.data(),
                        bxr.data(), bxu.data(), bout.data(), ringVals.data()));
@@ -385,35 +396,52 @@ void Solver21::makeSteps(int n) {
 }
 
 void Solver21::stats(double out[7]) { check(vls_stats(ctx_, 0, out)); }
+void Solver21::statsTotal(double out[7]) { check(vls_stats_total(ctx_, 0, nullptr, out)); }
 void Solver21::downloadF(double *f) { check(vls_download_f(ctx_, 0, f)); }
 void Solver21::moments(double *dM, double *accel, double *mTot) {
   check(vls_moments(ctx_, 0, dM, accel, mTot));
 }
 
 void Solver21::statistics(bool reset) {
-  // vlasovSolver21::statistics (src/vlasovSolver21.hxx:644-689)
-  double s[7];
-  stats(s);
-  FILE *f = fopen("statistics.txt", reset ? "w" : "a");
-  if (!f) throw Failure("ERROR: cannot open statistics.txt", VLS_ERR_IO);
-  if (reset) fprintf(f, "# Time Step Mass E Ek Ep S L1 L2\n");
-  fprintf(f, "%.10g %ld %10.10g %10.10g %10.10g %10.10g %10.10g %10.10g %10.10g\n", time_, stepCount_,
-          s[0], s[1], s[2], s[3], s[4], s[5], s[6]);
-  fclose(f);
+  // vlasovSolver21::statistics (src/vlasovSolver21.hxx:644-711): with several
+  // species one file per species (the same line in each, as the reference
+  // computes it), then statistics.txt with the total line
+  double line[7], tot[7];
+  check(vls_stats_total(ctx_, 0, line, tot));
+  auto put = [&](const char *fname, const double *s) {
+    FILE *f = fopen(fname, reset ? "w" : "a");
+    if (!f) throw Failure(std::string("ERROR: cannot open ") + fname, VLS_ERR_IO);
+    if (reset) fprintf(f, "# Time Step Mass E Ek Ep S L1 L2\n");
+    fprintf(f, "%.10g %ld %10.10g %10.10g %10.10g %10.10g %10.10g %10.10g %10.10g\n", time_, stepCount_,
+            s[0], s[1], s[2], s[3], s[4], s[5], s[6]);
+    fclose(f);
+  };
+  if (sp_.nSpecies > 1) {
+    for (int s = 0; s < sp_.nSpecies; s++) {
+      char fname[64];
+      snprintf(fname, sizeof fname, "statistics_S%d.txt", s);
+      put(fname, line);
+    }
+    put("statistics.txt", tot);
+  } else {
+    put("statistics.txt", line);
+  }
 }
 
 void Solver21::snapshot() {
   // vlasovSolver21::snapshot + NDfieldFromData (src/vlasovSolver21.hxx:585-598,717-773)
+  // (with several species a first, fastest dimension of nSpecies: :719-727)
   downloadF(f_.data());
   const GridParams &p = grid_.p;
-  int dims[3] = {p.Pres + 1, p.Ures[0] + 1, p.Ures[1]};
-  double x0[3] = {p.Pmin, p.Umin[0], p.Umin[1]};
-  double delta[3] = {p.Pmax - p.Pmin, p.Umax[0] - p.Umin[0], p.Umax[1] - p.Umin[1]};
+  const int sd = sp_.nSpecies > 1 ? 1 : 0;
+  int dims[4] = {sp_.nSpecies, p.Pres + 1, p.Ures[0] + 1, p.Ures[1]};
+  double x0[4] = {0, p.Pmin, p.Umin[0], p.Umin[1]};
+  double delta[4] = {(double)sp_.nSpecies, p.Pmax - p.Pmin, p.Umax[0] - p.Umin[0], p.Umax[1] - p.Umin[1]};
   char comment[80];
   snprintf(comment, sizeof comment, "%d %d %d %d %d %d", p.Pscale, 1, p.Uscale[0], 1, p.Uscale[1], 0);
   char fname[1024];
   snprintf(fname, sizeof fname, "snapshot_%4.4f.ND", time_);
-  saveNDfield(fname, dims, 3, x0, delta, f_.data(), comment);
+  saveNDfield(fname, dims + 1 - sd, 3 + sd, x0 + 1 - sd, delta + 1 - sd, f_.data(), comment);
 }
 
 void Solver21::write(const std::string &fnameNoExt) {
@@ -423,8 +451,9 @@ void Solver21::write(const std::string &fnameNoExt) {
   downloadF(f_.data());
   const int64_t nc = (int64_t)chains_.point.size();
   std::vector<int32_t> head(nc);
-  std::vector<double> ring(chains_.val.size());
+  std::vector<double> ring(chains_.val.size());  // cells x species
   if (nc) check(vls_get_chains(ctx_, 0, head.data(), ring.data()));
+  const int nSp = sp_.nSpecies;
   std::string fname = fnameNoExt + ".vls";
   FILE *f = fopen(fname.c_str(), "w");
   if (!f) throw Failure("ERROR: opening file '" + fname + "' for writing.", VLS_ERR_IO);
@@ -494,10 +523,8 @@ void Solver21::write(const std::string &fnameNoExt) {
           int64_t o = chains_.off[k];
           fwrite(&chains_.cellR[o + m], sizeof(double), 1, f);
           fwrite(&chains_.cellU[o + m], sizeof(double), 1, f);
-          int nv = 1;
-          fwrite(&nv, sizeof(int), 1, f);
-          double v = ring[o + (head[k] + m) % L];
-          fwrite(&v, sizeof(double), 1, f);
+          fwrite(&nSp, sizeof(int), 1, f);
+          fwrite(&ring[(o + (head[k] + m) % L) * nSp], sizeof(double), nSp, f);
         }
       }
     }
@@ -548,7 +575,8 @@ void Solver21::readRestart(const std::string &fnameNoExt, const Params &prm, boo
     GridParams gp = readGridParams(rd, uminJ);
     (void)full;
     grid_.init(gp);
-    f_.resize(grid_.nVal);
+    if (sp_.nSpecies < 1 || sp_.nSpecies > 5) throw Failure("ERROR: restart holds an invalid number of species", VLS_ERR_IO);
+    f_.resize(grid_.nVal * sp_.nSpecies);
     rd.raw(f_.data(), sizeof(double), f_.size());
     rd.tag(kTag21);
     rd.skip(256 * 8 * 3);
@@ -557,7 +585,6 @@ void Solver21::readRestart(const std::string &fnameNoExt, const Params &prm, boo
     kernelMass_ = rd.get<double>();
     kernelBufferSize_ = rd.get<int>();
     rd.skip(256 * 8);
-    if (sp_.nSpecies != 1) throw Failure("ERROR: only solver.nSpecies 1 is supported");
     setupFeet(nullptr);  // feet are recomputed; chains come from the file
     chains_ = Chains();
     if (kernelType_ == DELAYED) {
@@ -577,7 +604,9 @@ void Solver21::readRestart(const std::string &fnameNoExt, const Params &prm, boo
             int nv = rd.get<int>();
             std::vector<double> v(nv);
             rd.raw(v.data(), sizeof(double), nv);
-            c.v.push_back(nv ? v[0] : 0.0);
+            // one value per species; a cell the reference made with fewer
+            // (its back cell, dummyCell21(R,U,1)) holds 0 for the others
+            for (int s = 0; s < sp_.nSpecies; s++) c.v.push_back(s < nv ? v[s] : 0.0);
           }
           all.push_back(c);
         }
@@ -587,7 +616,7 @@ void Solver21::readRestart(const std::string &fnameNoExt, const Params &prm, boo
         if (c.key < 0 || c.key >= grid_.nVal || !(driftR_[c.key] < kernelRadius_))
           throw Failure("ERROR: restart chain does not match the recomputed feet", VLS_ERR_IO);
         chains_.point.push_back(c.key);
-        chains_.off.push_back((int64_t)chains_.val.size());
+        chains_.off.push_back((int64_t)chains_.cellR.size());
         chains_.len.push_back((int32_t)c.R.size());
         chains_.head.push_back(0);
         chains_.cellR.insert(chains_.cellR.end(), c.R.begin(), c.R.end());

@@ -106,10 +106,12 @@ class InitGen {
   virtual void setup(const Params &prm, const SolverParams &sp) = 0;
   virtual GridParams defaultGrid() const = 0;
   virtual SolverParams defaultSolver() const = 0;
-  virtual double valueAt(double r, double u, double j) const = 0;
-  // fill f on the grid (generate())
-  virtual void generate(const Grid21 &g, double *f) const;
-  virtual double mass() const = 0;
+  // species s at (r, u, j) (valueAt(pos, s)); Hernquist and Plummer have one
+  virtual double valueAt(double r, double u, double j, int s) const = 0;
+  // fill f on the grid, species interleaved f[i * nS + s] (generate())
+  virtual void generate(const Grid21 &g, double *f, int nS) const;
+  virtual double mass(int s) const = 0;  // getMass(s)
+  virtual int maxSpecies() const { return 1; }
 };
 
 std::unique_ptr<InitGen> makeInitGen(const std::string &type);
@@ -122,7 +124,8 @@ bool advanceDrift(double R, double U, double J, double dt, double R0, double &ne
                   double &newU);
 
 struct Chains {
-  // chain c feeds point[c]; its cells, front first, are cellR/cellU/val[off[c]..off[c]+len[c])
+  // chain c feeds point[c]; its cells, front first, are cellR/cellU[off[c]..off[c]+len[c]),
+  // cell k's species values val[k * nSpecies .. (k + 1) * nSpecies)
   std::vector<int64_t> point, off;
   std::vector<int32_t> len, head;
   std::vector<double> cellR, cellU, val;
@@ -150,7 +153,9 @@ class Solver21 {
   void statistics(bool reset);      // statistics.txt line
   void snapshot();                  // snapshot_%4.4f.ND
   void write(const std::string &fnameNoExt);  // .vls restart
-  void stats(double out[7]);
+  void stats(double out[7]);              // the species files' line
+  void statsTotal(double out[7]);         // the statistics.txt line
+  int nSpecies() const { return sp_.nSpecies; }
   void downloadF(double *f);
   void moments(double *dM, double *accel, double *mTot);
 
@@ -181,7 +186,7 @@ class Solver21 {
   std::string initType_;
 
   void setupFeet(std::unique_ptr<InitGen> *init);  // feet, chains (setup())
-  void renormalize(double norm);
+  void renormalize(double norm, int s);  // species s
   void upload();
   void readRestart(const std::string &fname, const Params &prm, bool device);
   bool checkTimer(long every) const { return every >= 0 && (stepCount_ % every) == 0; }

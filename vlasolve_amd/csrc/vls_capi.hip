@@ -57,6 +57,19 @@ void dfree(vls::Run &r, T *&p) {
 
 size_t kCoefPad(const vls::Run &r) { return 2 * (size_t)(r.nR + 2) + 8; }
 
+// The reference interleaves the species of a point (arr[i * nFields + s],
+// src/grid_base.hxx:385); the device keeps one plane per species.
+std::vector<double> to_planes(const double *in, int64_t n, int nS) {
+  std::vector<double> out((size_t)n * nS);
+  for (int64_t i = 0; i < n; i++)
+    for (int k = 0; k < nS; k++) out[(size_t)k * n + i] = in[i * nS + k];
+  return out;
+}
+void from_planes(const double *in, double *out, int64_t n, int nS) {
+  for (int64_t i = 0; i < n; i++)
+    for (int k = 0; k < nS; k++) out[i * nS + k] = in[(size_t)k * n + i];
+}
+
 // A foot can be evaluated when its integer parts lie in [0, dim-1] on both
 // axes (dim-1 only on the last knot).
 bool foot_in_domain(double xr, double xu, const vls::Run &r) {
@@ -80,7 +93,51 @@ vls::Run &run_of(vls_ctx *ctx, int run);
 // src/vlasovSolver21_spline.hxx:680-801): with linking, kick (moment, spline,
 // evaluation) then drift (spline, evaluation with the delay lines); without,
 // half drift, kick, half drift.
+// Points r.f and r.ch_vals at species k (and lets only the last species turn
+// the delay lines) for the life of the object.
+struct SpeciesPlane {
+  vls::Run &r;
+  double *f0, *v0;
+  SpeciesPlane(vls::Run &run, int k) : r(run), f0(run.f), v0(run.ch_vals) {
+    r.f = f0 + (size_t)k * r.nVal;
+    if (v0) r.ch_vals = v0 + (size_t)k * r.nCells;
+    r.rollChains = k == r.nS - 1;
+  }
+  ~SpeciesPlane() {
+    r.f = f0;
+    r.ch_vals = v0;
+    r.rollChains = true;
+  }
+};
+
+// makeOneStep with several species (src/vlasovSolver21_spline.hxx:680-801):
+// the moment of their sum, then the kick of each species with the shared
+// acceleration, then the drift of each.
+void step_species(vls::Run &r, hipStream_t s, vls::Timing *t) {
+  if (r.sp.no_linking)
+    for (int k = 0; k < r.nS; k++) {
+      SpeciesPlane sp(r, k);
+      vls::launch_spline(r, r.f, s, t);
+      vls::launch_drift(r, s);
+      vls::time_mark(t, s, "drift");
+    }
+  vls::launch_moment(r, s);
+  vls::time_mark(t, s, "moment");
+  for (int k = 0; k < r.nS; k++) {
+    SpeciesPlane sp(r, k);
+    vls::launch_spline_kick(r, s, t, false);
+  }
+  for (int k = 0; k < r.nS; k++) {
+    SpeciesPlane sp(r, k);
+    vls::launch_spline(r, r.f, s, t);
+    vls::launch_drift(r, s);
+    vls::time_mark(t, s, "drift");
+  }
+  r.stepped = true;
+}
+
 void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
+  if (r.nS > 1) return step_species(r, s, t);
   if (r.sp.no_linking) {
     vls::launch_spline(r, r.f, s, t);
     vls::launch_drift(r, s);
@@ -369,6 +426,8 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.nJ = G.nJ;
       r.sp = sp[k];
       r.nVal = (int64_t)r.nR * r.nU * r.nJ;
+      r.nS = sp[k].n_species > 1 ? sp[k].n_species : 1;
+      if (r.nS > 5) throw vls::Error(VLS_ERR_ARG, "vls_create: at most 5 species");
       r.stride = r.nR + 2;
       r.cs = (int64_t)(r.nR + 2) * (r.nU + 2);
       r.start_v = G.spline_start[1];
@@ -377,7 +436,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.Ulo = G.U_Val[0];
       r.Uhi = G.U_Val[r.nU - 1];
       vls::configure_tiles(r, maxLds);
-      r.f = dalloc<double>(r, r.nVal);
+      r.f = dalloc<double>(r, r.nVal * r.nS);
       // A foot on the last knot (integer part dim-1, fraction 0) makes the
       // 4x4 stencil touch one column past its padded row and one row past its
       // slice, with weight zero, as in the reference; the tail pad keeps
@@ -393,9 +452,9 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.Mx = dalloc<double>(r, r.nR);
       r.acc = dalloc<double>(r, r.nR);
       r.mTot = dalloc<double>(r, 1);
-      r.st_part = dalloc<double>(r, 5 * (size_t)r.nJ * r.nR);
-      r.st_q = dalloc<double>(r, 4 * (size_t)r.nJ + r.nR);
-      r.st_out = dalloc<double>(r, 8);
+      r.st_part = dalloc<double>(r, 7 * (size_t)r.nJ * r.nR);
+      r.st_q = dalloc<double>(r, 6 * (size_t)r.nJ + r.nR);
+      r.st_out = dalloc<double>(r, 14);
       // tables
       std::vector<double> Ct1(r.nJ), gfac(r.nR);
       for (int j = 0; j < r.nJ; j++) Ct1[j] = 8 * M_PI * M_PI * G.J_Val[j] * G.dJ[j];
@@ -464,9 +523,16 @@ int vls_upload_f(vls_ctx *ctx, int run, const double *f) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
     if (!f) throw vls::Error(VLS_ERR_ARG, "null f");
-    VLS_HIP_CHECK(hipMemcpyAsync(r.f, f, r.nVal * sizeof(double), hipMemcpyHostToDevice,
-                                 ctx->c.stream));
-    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    if (r.nS > 1) {
+      std::vector<double> planes = to_planes(f, r.nVal, r.nS);
+      VLS_HIP_CHECK(hipMemcpyAsync(r.f, planes.data(), planes.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->c.stream));
+      VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    } else {
+      VLS_HIP_CHECK(hipMemcpyAsync(r.f, f, r.nVal * sizeof(double), hipMemcpyHostToDevice,
+                                   ctx->c.stream));
+      VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    }
     r.have_f = true;
   });
 }
@@ -475,9 +541,17 @@ int vls_download_f(vls_ctx *ctx, int run, double *f) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
     if (!f) throw vls::Error(VLS_ERR_ARG, "null f");
-    VLS_HIP_CHECK(hipMemcpyAsync(f, r.f, r.nVal * sizeof(double), hipMemcpyDeviceToHost,
-                                 ctx->c.stream));
-    VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    if (r.nS > 1) {
+      std::vector<double> planes((size_t)r.nVal * r.nS);
+      VLS_HIP_CHECK(hipMemcpyAsync(planes.data(), r.f, planes.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->c.stream));
+      VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+      from_planes(planes.data(), f, r.nVal, r.nS);
+    } else {
+      VLS_HIP_CHECK(hipMemcpyAsync(f, r.f, r.nVal * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->c.stream));
+      VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    }
   });
 }
 
@@ -549,7 +623,7 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     r.ch_xr = dalloc<double>(r, nChains);
     r.ch_xu = dalloc<double>(r, nChains);
     r.ch_out = dalloc<uint8_t>(r, nChains);
-    r.ch_vals = dalloc<double>(r, cells);
+    r.ch_vals = dalloc<double>(r, cells * r.nS);
     h2d(r.ch_point, point, nChains, s);
     h2d(r.ch_off, off.data(), nChains, s);
     h2d(r.ch_len, len, nChains, s);
@@ -557,7 +631,13 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     h2d(r.ch_xr, back_xr, nChains, s);
     h2d(r.ch_xu, back_xu, nChains, s);
     h2d(r.ch_out, back_out, nChains, s);
-    h2d(r.ch_vals, vals, cells, s);
+    if (r.nS > 1) {
+      std::vector<double> planes = to_planes(vals, cells, r.nS);
+      h2d(r.ch_vals, planes.data(), planes.size(), s);
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      h2d(r.ch_vals, vals, cells, s);
+    }
     drop_graph(ctx->c);  // the delay-line tables moved
     if (r.have_drift) vls::mark_chains(r, s);
     VLS_HIP_CHECK(hipGetLastError());
@@ -573,9 +653,13 @@ int vls_get_chains(vls_ctx *ctx, int run, int32_t *head, double *vals) {
     if (head && r.nChains)
       VLS_HIP_CHECK(hipMemcpyAsync(head, r.ch_head, r.nChains * sizeof(int32_t),
                                    hipMemcpyDeviceToHost, s));
-    if (vals && r.nCells)
-      VLS_HIP_CHECK(hipMemcpyAsync(vals, r.ch_vals, r.nCells * sizeof(double),
+    if (vals && r.nCells) {
+      std::vector<double> planes((size_t)r.nCells * r.nS);
+      VLS_HIP_CHECK(hipMemcpyAsync(planes.data(), r.ch_vals, planes.size() * sizeof(double),
                                    hipMemcpyDeviceToHost, s));
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+      from_planes(planes.data(), vals, r.nCells, r.nS);
+    }
     VLS_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -628,16 +712,21 @@ int vls_moments(vls_ctx *ctx, int run, double *dM, double *accel, double *mTot) 
   });
 }
 
-int vls_stats(vls_ctx *ctx, int run, double out[7]) {
+int vls_stats(vls_ctx *ctx, int run, double out[7]) { return vls_stats_total(ctx, run, out, nullptr); }
+
+int vls_stats_total(vls_ctx *ctx, int run, double line[7], double total[7]) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
-    if (!out) throw vls::Error(VLS_ERR_ARG, "null output");
+    if (!line && !total) throw vls::Error(VLS_ERR_ARG, "null output");
     if (!r.have_f) throw vls::Error(VLS_ERR_STATE, "vls_stats: f not set");
     hipStream_t s = ctx->c.stream;
     vls::launch_stats(r, s);
     VLS_HIP_CHECK(hipGetLastError());
-    VLS_HIP_CHECK(hipMemcpyAsync(out, r.st_out, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+    double out[14];
+    VLS_HIP_CHECK(hipMemcpyAsync(out, r.st_out, 14 * sizeof(double), hipMemcpyDeviceToHost, s));
     VLS_HIP_CHECK(hipStreamSynchronize(s));
+    if (line) std::copy(out, out + 7, line);
+    if (total) std::copy(out + 7, out + 14, total);
   });
 }
 
@@ -658,7 +747,7 @@ int vls_set_timing(vls_ctx *ctx, int on) {
   });
 }
 
-int vls_run_info(vls_ctx *ctx, int run, int64_t out[5]) {
+int vls_run_info(vls_ctx *ctx, int run, int64_t out[6]) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
     if (!out) throw vls::Error(VLS_ERR_ARG, "null output");
@@ -667,6 +756,7 @@ int vls_run_info(vls_ctx *ctx, int run, int64_t out[5]) {
     out[2] = r.nJ;
     out[3] = r.nChains;
     out[4] = r.nCells;
+    out[5] = r.nS;
   });
 }
 
@@ -683,7 +773,10 @@ int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *val
         if (head[k] < 0 || head[k] >= len[k]) throw vls::Error(VLS_ERR_ARG, "chain head out of range");
       h2d(r.ch_head, head, r.nChains, s);
     }
-    if (r.nCells && vals) h2d(r.ch_vals, vals, r.nCells, s);
+    if (r.nCells && vals) {
+      std::vector<double> planes = to_planes(vals, r.nCells, r.nS);
+      h2d(r.ch_vals, planes.data(), planes.size(), s);
+    }
     VLS_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -709,6 +802,7 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       throw vls::Error(VLS_ERR_STATE, "vls_record: f and drift feet must be set first");
     if (r.sp.no_linking)
       throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
+    if (r.nS > 1) throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers one species only");
     // the kick's transpose finds the points of a coefficient row from
     // n ~ u + a hinv, which needs v nodes on the uniform spline knots (to
     // 0.01 of a cell; a linear v scale is there to rounding)

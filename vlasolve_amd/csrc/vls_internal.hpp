@@ -51,7 +51,9 @@ struct Timing {
 // One run of an ensemble: every array of its state in HBM.
 struct Run {
   int nR = 0, nU = 0, nJ = 0;
-  int64_t nVal = 0;
+  int64_t nVal = 0;     // points per species
+  int nS = 1;           // species: f and the delay-line values hold nS planes
+  bool rollChains = true;  // the drift turns the delay lines (last species of a step)
   int64_t cs = 0;       // padded coefficient slice size (nU+2)*(nR+2)
   int stride = 0;       // padded row stride nR+2
   int TR = 32, LS = 0;  // r-sweep tile: rows per block, odd LDS row stride (>= TR)
@@ -61,7 +63,7 @@ struct Run {
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
 
   // state
-  double *f = nullptr;      // [nJ][nU][nR]
+  double *f = nullptr;      // [nS][nJ][nU][nR]
   double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
   double *gsc = nullptr;    // fused kick: per-wave scratch for wide kicks
   bool kickScratch = false; // fused kick: always take the scratch path
@@ -82,9 +84,9 @@ struct Run {
   double *ch_xr = nullptr, *ch_xu = nullptr, *ch_vals = nullptr;
   uint8_t *ch_out = nullptr;
   // statistics
-  double *st_part = nullptr;  // [5][nJ][nR]
-  double *st_q = nullptr;     // [4][nJ]
-  double *st_out = nullptr;   // [8]
+  double *st_part = nullptr;  // [7][nJ][nR]
+  double *st_q = nullptr;     // [6][nJ] then dM [nR]
+  double *st_out = nullptr;   // [14]: the species line, the statistics.txt line
 
   // adjoint: recorded run and work arrays (allocated by vls_record)
   struct Adjoint {
@@ -133,7 +135,9 @@ void launch_moment(Run &r, hipStream_t s);       // partials of f, then the scan
 void launch_moment_scan(Run &r, hipStream_t s);  // scan of the partials in P
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 void launch_kick(Run &r, hipStream_t s);
-void launch_spline_kick(Run &r, hipStream_t s, Timing *t);  // moment, kick spline, kick
+// (moment,) kick spline, kick of the plane r.f; without the moment the
+// acceleration of the last launch_moment is used
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment = true);
 size_t kick_scratch_doubles(const Run &r);
 void launch_drift(Run &r, hipStream_t s);
 void mark_chains(Run &r, hipStream_t s);  // attach the delay lines to their points

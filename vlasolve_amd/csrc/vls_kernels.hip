@@ -86,6 +86,45 @@ k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
   P[(size_t)j * nR + r] = Pj;
 }
 
+// k_moment_partial over the sum of nS species planes (nVal apart): the
+// integrand of int21::f(0, nS), 0 + avg_0 + avg_1 + ... per cell
+// (src/integrand21.hxx:21-39), in the same tree order.
+__global__ void __launch_bounds__(kBlock)
+k_moment_partial_sp(const double *__restrict__ f, const double *__restrict__ Ct1,
+                    const double *__restrict__ dU, const double *__restrict__ dR,
+                    double *__restrict__ P, int nR, int nU, int nJ, int nS, int64_t nVal) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (r >= nR) return;
+  double Pj = 0.0;
+  if (r < nR - 1) {
+    const double ct = Ct1[j];
+    const double dr = dR[r];
+    const double *p = f + (size_t)j * nU * nR + r;
+    const int nW = (nU + 63) / 64;
+    for (int w = 0; w < nW; w++) {
+      double G[4];
+      for (int q = 0; q < 4; q++) {
+        double acc = 0.0;
+        for (int t = 0; t < 16; t++) {
+          const int u = 64 * w + 16 * q + t;
+          if (u >= nU - 1) break;
+          double avg = 0.0;
+          for (int k = 0; k < nS; k++) {
+            const double *pk = p + (size_t)k * nVal + (size_t)u * nR;
+            avg += (((pk[0] + pk[1]) + pk[nR]) + pk[nR + 1]) * 0.25;
+          }
+          acc += ((ct * dU[u]) * dr) * avg;
+        }
+        G[q] = acc;
+      }
+      const double W = ((G[0] + G[1]) + G[2]) + G[3];
+      Pj = w == 0 ? W : Pj + W;
+    }
+  }
+  P[(size_t)j * nR + r] = Pj;
+}
+
 // dM[r] = serial sum over j; exclusive radial prefix (gridHandler::accumulate
 // with addSpare=1, src/grid_handler.hxx:350-407); acceleration
 // M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).  One block: a
@@ -832,8 +871,9 @@ __device__ __forceinline__ double eval_c(const double *d, int64_t st, double xr,
 // A delay-line point takes the spline value at its back cell (or 0 when that
 // cell's v is outside the grid; src/vlasovSolver21_spline.hxx:776-786) into
 // the ring slot behind the front, reads the front (sphericalKernel::readAt,
-// src/sphericalKernel.hxx:155-161) and turns the ring by one
-// (cellChain::roll, src/cellChain.hxx:49-56).  Blocks are spread so that each
+// src/sphericalKernel.hxx:155-161) and, for the last species of the step,
+// turns the ring by one (cellChain::roll, src/cellChain.hxx:49-56; the
+// species share the ring's head).  Blocks are spread so that each
 // XCD (block id mod 8) walks a contiguous range of points: the 4x4 stencils
 // of neighbouring points share coefficient rows, which then stay in that
 // XCD's L2.
@@ -845,8 +885,9 @@ struct ChainTables {
   int32_t *head;
   const double *bxr, *bxu;
   const uint8_t *bout;
-  double *vals;
+  double *vals;  // the values of the species being drifted
   int64_t n;
+  int roll;      // turn the rings (the last species of the step)
 };
 
 __global__ void __launch_bounds__(kBlock)
@@ -875,7 +916,7 @@ k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__re
     if (back >= L) back -= L;
     v[back] = ch.bout[k] ? 0.0 : eval_c(cj, stride, ch.bxr[k], ch.bxu[k]);
     f[i] = v[h];
-    ch.head[k] = (h + 1 == L) ? 0 : h + 1;
+    if (ch.roll) ch.head[k] = (h + 1 == L) ? 0 : h + 1;
   }
 }
 
@@ -893,16 +934,21 @@ k_mark_chains(double *__restrict__ xr, double *__restrict__ xu, const int64_t *_
 // Per (j, r) partials of the moment (in its tree order) and, serial in u, of the integrands of
 // quadrature21::integrate (src/quadrature21.hxx:57-85; integrands
 // src/integrand21.hxx:82-145).
+//
+// With nS species (planes nVal apart) the mass, and the entropy and L2 norm of
+// the statistics.txt line, take the sum of the species' cell averages
+// (int21::f / entropy / L2Norm(0, nS)); the other integrands read species 0,
+// as the reference's default integrands do (src/slice21Iterator.hxx:90-94).
 __global__ void __launch_bounds__(kBlock)
 k_stats_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
                 const double *__restrict__ dU, const double *__restrict__ dR,
                 const double *__restrict__ R_Val, const double *__restrict__ U_Val,
                 const double *__restrict__ J_Val, double *__restrict__ part, int nR, int nU,
-                int nJ) {
+                int nJ, int nS, int64_t nVal) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (r >= nR) return;
-  double m = 0.0, ek = 0.0, s = 0.0, l1 = 0.0, l2 = 0.0;
+  double m = 0.0, ek = 0.0, s = 0.0, l1 = 0.0, l2 = 0.0, sAll = 0.0, l2All = 0.0;
   if (r < nR - 1) {
     const double ct = Ct1[j];
     const double dr = dR[r];
@@ -923,14 +969,26 @@ k_stats_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
           const double *q = p + (int64_t)(u + 1) * nR;
           double b0 = q[0], b1 = q[1];
           double v = (((a0 + a1) + b0) + b1) * 0.25;
+          double vm = v;  // the sum over the species
+          if (nS > 1) {
+            vm = 0.0 + v;
+            for (int k = 1; k < nS; k++) {
+              const double *pk = p + (int64_t)k * nVal + (int64_t)u * nR;
+              vm += (((pk[0] + pk[1]) + pk[nR]) + pk[nR + 1]) * 0.25;
+            }
+          }
           double du = dU[u];
           double w = (ct * du) * dr;
           double uc = U_Val[u] + 0.5 * du;
-          g += w * v;
+          g += w * vm;
           ek += w * (0.5 * v * (uc * uc + J * J / (rc * rc)));
           s += w * ((v <= 0) ? 0.0 : -v * log(v));
           l1 += w * fabs(v);
           l2 += w * (v * v);
+          if (nS > 1) {
+            sAll += w * ((vm <= 0) ? 0.0 : -vm * log(vm));
+            l2All += w * (vm * vm);
+          }
           a0 = b0;
           a1 = b1;
         }
@@ -946,23 +1004,29 @@ k_stats_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
   part[2 * n + o] = s;
   part[3 * n + o] = l1;
   part[4 * n + o] = l2;
+  part[5 * n + o] = sAll;
+  part[6 * n + o] = l2All;
 }
 
 // Serial sums over j (moment) and over r (integrands), then over j; the
 // potential energy of vlasovSolver21::potentialEnergy
 // (src/vlasovSolver21.hxx:627-642, with the gathered array shifted by one,
-// src/quadrature21.hxx:36).  out = {M, E, Ek, Ep, S, L1, L2}.
+// src/quadrature21.hxx:36).  out = {M, E, Ek, Ep, S, L1, L2} (the species
+// files' line), then the statistics.txt line: with nS species the species
+// loop of vlasovSolver21::statistics (:656-688) adds that line nS times, and
+// S and L2 are those of the summed species (:690-699); with one species the
+// same line again.
 __global__ void __launch_bounds__(kBlock)
 k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val, double G,
               double *__restrict__ dM, double *__restrict__ q, double *__restrict__ out, int nR,
-              int nJ) {
+              int nJ, int nS) {
   const int64_t n = (int64_t)nJ * nR;
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     double s = 0.0;
     for (int j = 0; j < nJ; j++) s += part[(int64_t)j * nR + r];
     dM[r] = s;
   }
-  for (int t = threadIdx.x; t < 4 * nJ; t += blockDim.x) {
+  for (int t = threadIdx.x; t < 6 * nJ; t += blockDim.x) {
     int k = t / nJ, j = t % nJ;
     const double *p = part + (int64_t)(k + 1) * n + (int64_t)j * nR;
     double s = 0.0;
@@ -980,8 +1044,8 @@ k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val,
       M += dM[i - 1];
     }
     M += dM[nR - 1];
-    double tot[4];
-    for (int k = 0; k < 4; k++) {
+    double tot[6];
+    for (int k = 0; k < 6; k++) {
       double s = 0.0;
       for (int j = 0; j < nJ; j++) s += q[k * nJ + j];
       tot[k] = s;
@@ -993,6 +1057,25 @@ k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val,
     out[4] = tot[1];
     out[5] = tot[2];
     out[6] = tot[3];
+    if (nS == 1) {
+      for (int k = 0; k < 7; k++) out[7 + k] = out[k];
+    } else {
+      double Ep_tot = 0, M_tot = 0, Ek_tot = 0, E_tot = 0, L1_tot = 0;
+      for (int k = 0; k < nS; k++) {
+        Ep_tot += out[3];
+        M_tot += out[0];
+        Ek_tot += out[2];
+        E_tot += out[1];
+        L1_tot += out[5];
+      }
+      out[7] = M_tot;
+      out[8] = E_tot;
+      out[9] = Ek_tot;
+      out[10] = Ep_tot;
+      out[11] = tot[4];
+      out[12] = L1_tot;
+      out[13] = tot[5];
+    }
   }
 }
 
@@ -1011,8 +1094,12 @@ void time_mark(Timing *t, hipStream_t s, const char *name) {
 
 void launch_moment(Run &r, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
-  hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
-                     r.nU, r.nJ);
+  if (r.nS > 1)
+    hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P,
+                       r.nR, r.nU, r.nJ, r.nS, r.nVal);
+  else
+    hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
+                       r.nU, r.nJ);
   launch_moment_scan(r, s);
 }
 
@@ -1312,11 +1399,15 @@ size_t kick_scratch_doubles(const Run &r) {
   return tiles * r.nJ * (size_t)(r.nU + 3) * 64;
 }
 
-void launch_spline_kick(Run &r, hipStream_t s, Timing *t) {
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment) {
   if (fused_kick(r) && r.gsc) {
     const LineShape *ls = line_shape(r);
     const int nW = (r.nU + 63) / 64;
-    if (nW <= kRowsMaxW) {
+    if (!moment) {
+      hipLaunchKernelGGL(ls->rows, dim3(nW, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR, r.nU, r.nJ,
+                         r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
+      time_mark(t, s, "spline_r");
+    } else if (nW <= kRowsMaxW) {
       // the row solves also form the moment partials of f
       hipLaunchKernelGGL(ls->rowsMoment, dim3(1, r.nJ), dim3(64 * nW), 0, s, r.f, r.c, r.nR,
                          r.nU, r.nJ, r.cs, 0, r.tr, r.Ct1, r.dU, r.dR, r.P);
@@ -1336,8 +1427,10 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t) {
     time_mark(t, s, "kick_line");
     return;
   }
-  launch_moment(r, s);
-  time_mark(t, s, "moment");
+  if (moment) {
+    launch_moment(r, s);
+    time_mark(t, s, "moment");
+  }
   launch_spline(r, r.f, s, t);
   launch_kick(r, s);
   time_mark(t, s, "kick");
@@ -1352,7 +1445,8 @@ void launch_kick(Run &r, hipStream_t s) {
 void launch_drift(Run &r, hipStream_t s) {
   const int64_t nb = (r.nVal + kBlock - 1) / kBlock;
   const int perXcd = (int)((nb + 7) / 8);
-  ChainTables ch{r.ch_off, r.ch_len, r.ch_head, r.ch_xr, r.ch_xu, r.ch_out, r.ch_vals, r.nChains};
+  ChainTables ch{r.ch_off,  r.ch_len, r.ch_head,  r.ch_xr,
+                 r.ch_xu,   r.ch_out, r.ch_vals,  r.nChains, r.rollChains ? 1 : 0};
   hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.nVal,
                      r.nU * r.nR, r.cs, r.nR + 2, perXcd, ch);
 }
@@ -1366,9 +1460,9 @@ void mark_chains(Run &r, hipStream_t s) {
 void launch_stats(Run &r, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   hipLaunchKernelGGL(k_stats_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.R_Val,
-                     r.U_Val, r.J_Val, r.st_part, r.nR, r.nU, r.nJ);
+                     r.U_Val, r.J_Val, r.st_part, r.nR, r.nU, r.nJ, r.nS, r.nVal);
   hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, s, r.st_part, r.R_Val, r.sp.G,
-                     r.st_q + 4 * (int64_t)r.nJ, r.st_q, r.st_out, r.nR, r.nJ);
+                     r.st_q + 6 * (int64_t)r.nJ, r.st_q, r.st_out, r.nR, r.nJ, r.nS);
 }
 
 }  // namespace vls

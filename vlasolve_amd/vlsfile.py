@@ -40,12 +40,14 @@ def _grid_params(r):
     r.take("q")
     r.skip(2048 - 4 * 6 - 8)
     return dict(Pmin=d[0], Pmax=d[1], Umin0=d[3], Umax=(d[4], d[5]), Pres=iv[0],
-                Ures=(iv[1], iv[2]), Pscale=iv[3], Uscale=(iv[4], iv[5]))
+                Ures=(iv[1], iv[2]), Pscale=iv[3], Uscale=(iv[4], iv[5]), nFields=iv[12])
 
 
 def read(path):
-    """Return dict(time, dt, stepCount, noLinking, grid, f (nJ, nU, nR),
-    kernelType, chains = {point index: (R[], U[], val[])} front first)."""
+    """Return dict(time, dt, stepCount, noLinking, grid, f (nJ, nU, nR), or
+    (nJ, nU, nR, nFields) with several fields, kernelType, chains = {point
+    index: (R[], U[], val[])} front first; val[] has one value per cell, or
+    one row of nFields values per cell (zero-padded) with several fields)."""
     raw = open(path, "rb").read()
     r = _R(raw)
     if r.take("i") != 1:
@@ -63,9 +65,10 @@ def read(path):
     _grid_params(r)
     r.tag("GRID_BASE v0.12")
     g = _grid_params(r)
-    nR, nU, nJ = g["Pres"] + 1, g["Ures"][0] + 1, g["Ures"][1]
-    f = np.frombuffer(raw, "<f8", nR * nU * nJ, r.o).reshape(nJ, nU, nR).copy()
-    r.o += 8 * nR * nU * nJ
+    nR, nU, nJ, nF = g["Pres"] + 1, g["Ures"][0] + 1, g["Ures"][1], max(g["nFields"], 1)
+    shape = (nJ, nU, nR) + ((nF,) if nF > 1 else ())
+    f = np.frombuffer(raw, "<f8", nR * nU * nJ * nF, r.o).reshape(shape).copy()
+    r.o += 8 * nR * nU * nJ * nF
     r.tag("GRID_21 v0.11")
     r.skip(3 * 2048)
     r.tag("VLASOV_SOLVER21 v0.12")
@@ -86,7 +89,11 @@ def read(path):
                     U.append(r.take("d"))
                     nv = r.take("i")
                     vals = r.take(f"{nv}d") if nv else ()
-                    V.append(vals if isinstance(vals, float) else (vals[0] if nv else 0.0))
+                    vals = (vals,) if isinstance(vals, float) else tuple(vals)
+                    if nF > 1:
+                        V.append(list(vals[:nF]) + [0.0] * (nF - len(vals[:nF])))
+                    else:
+                        V.append(vals[0] if nv else 0.0)
                 chains[key] = (np.array(R), np.array(U), np.array(V))
     r.tag("VLASOV_SOLVER21_SPLINE v0.11")
     return dict(time=time_, dt=dt2, stepCount=step, noLinking=nolink, grid=g, f=f,
