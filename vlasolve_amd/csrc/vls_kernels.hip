@@ -614,6 +614,15 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 // taken during the forward elimination.
 constexpr int kKW = 62, kEB = 8;
 
+// Lane i takes lane i+1's value (lane 63 keeps its own): a whole-wave DPP
+// shift, a VALU move with no trip through LDS.
+__device__ __forceinline__ double wave_shl1(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Long columns keep their first kYL forward values in LDS instead of VGPRs
 // (they are the last ones the back substitution needs) and use a 16-row ring,
 // so that the column leaves room in the 512 VGPRs of a wave for several
@@ -713,9 +722,10 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     }
     fo[(size_t)u * nR] = val;
   };
-  // g_m from the lane's c_m and its two right neighbours
+  // g_m from the lane's c_m and its two right neighbours (lanes 62 and 63,
+  // which write no points, combine values that are not theirs)
   auto gcomb = [&](double cm) {
-    const double c1 = __shfl_down(cm, 1), c2 = __shfl_down(cm, 2);
+    const double c1 = wave_shl1(cm), c2 = wave_shl1(c1);
     return (cm + c1 * 4) + c2;
   };
   // Lanes without a point of their own store into their scratch slot, so that
