@@ -201,43 +201,96 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
   }
 }
 
-// The kick's transposed v-solve for any column length (see k_adj_gcol):
-// the two streaming sweeps of k_adj_v over g, the descending one writing to
-// fb instead of in place.
+// The kick's transpose for columns too long for registers, in two kernels
+// around the scan.  k_adj_kick_walk: one lane per column r walks the points
+// u in order and, for each point in the v range, adds its acceleration term
+// fb*[u] sum_k B_k'(t) g_{n+k} (the kick coefficients' column combination)
+// to the column's sum, and its B_k(t) / 6 fb*[u] to rows n..n+3 of the
+// column's g, kept as a window of four rows.  n never decreases with u, so a
+// row is complete once a point targets rows above it: it then takes its
+// step of the ascending sweep of the transposed v-solve and is stored over g
+// (g[j][m][r], slice stride cs, row stride nR).  k_adj_gcol_down then runs
+// the descending sweep and writes fb (plus the transposed moment quadrature).
 __global__ void __launch_bounds__(64)
-k_adj_gcol_stream(double *__restrict__ g, double *__restrict__ fb, int nR, int nU, int64_t cs,
-                  ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
-                  const double *__restrict__ dU, const double *__restrict__ dR) {
+k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
+                const double *__restrict__ U_Val, const double *__restrict__ acc,
+                double *__restrict__ g, double *__restrict__ abp, int nR, int nU, int64_t cs,
+                double Ulo, double Uhi, double start_v, double hinv_v, ThomasAxis ax) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (r >= nR) return;
+  const int S = nR + 2, Mv = nU + 2;
+  const double a = acc[r];
+  const double *fj = fsb + (int64_t)j * nU * nR + r;
+  const double *cj = c + j * cs + r;
+  double *gc = g + j * cs + r;
+  double w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // g of rows base..base+3
+  int base = 0;
+  double c0 = 0, prev = 0, sAcc = 0;
+  // row m of g is complete: its step of the ascending sweep
+  // (c1 -= A01 c0, c2 -= A02 c0, c[t] -= u[t-1] c[t-1]), stored in place
+  auto finish = [&]() {
+    const int m = base;
+    double v = w0;
+    if (m == 0) c0 = v;
+    if (m == 1) v = fma(-ax.A01, c0, v);
+    if (m == 2) v = fma(-ax.A02, c0, v);
+    if (m >= 2) v = fma(-ax.u[m - 1], prev, v);
+    prev = v;
+    gc[(int64_t)m * nR] = v;
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = 0;
+    base++;
+  };
+  for (int u = 0; u < nU; u++) {
+    const double lam = fj[(int64_t)u * nR];
+    const double newU = U_Val[u] + a;
+    if (lam == 0.0 || (newU <= Ulo) || (newU >= Uhi)) continue;
+    double t;
+    int n;
+    split((newU - start_v) * hinv_v, n, t);
+    // acceleration term over the rows n..n+3 inside the padded column
+    double sk = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int row = min(n + k, nU + 1);
+      const double *rc = cj + (int64_t)row * S;
+      const double gk = (rc[0] + rc[1] * 4) + rc[2];
+      const double dw = k == 0 ? -dBV2(1. - t) : k == 1 ? -dBV1(1. - t) : k == 2 ? dBV1(t) : dBV2(t);
+      sk += n + k <= nU + 1 ? dw * gk : 0.0;
+    }
+    sAcc += lam * sk;
+    while (base < n) finish();
+    const double l6 = lam * (1.0 / 6.0);
+    w0 += l6 * BV2(1. - t);
+    w1 += l6 * BV1(1. - t);
+    w2 += l6 * BV1(t);
+    w3 += l6 * BV2(t);
+  }
+  while (base < Mv) finish();
+  abp[(int64_t)j * nR + r] = sAcc * (hinv_v / 36.0);
+}
+
+// The descending sweep of the kick's transposed v-solve over the rows the
+// walk left in g (see k_adj_kick_walk), rows 1..nU to fb[j][u][r] plus, with
+// dMb, the transposed moment quadrature.
+__global__ void __launch_bounds__(64)
+k_adj_gcol_down(const double *__restrict__ g, double *__restrict__ fb, int nR, int nU,
+                int64_t cs, ThomasAxis ax, const double *__restrict__ dMb,
+                const double *__restrict__ Ct1, const double *__restrict__ dU,
+                const double *__restrict__ dR) {
   const int r = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y;
   if (r >= nR) return;
   const int M = ax.M;
-  double *col = g + j * cs + r;
-  double v[kLC];
-  double c0 = 0.0, prev = 0.0;
-  for (int m0 = 0; m0 < M; m0 += kLC) {
-#pragma unroll
-    for (int i = 0; i < kLC; i++)
-      if (m0 + i < M) v[i] = col[(int64_t)(m0 + i) * nR];
-#pragma unroll
-    for (int i = 0; i < kLC; i++) {
-      const int t = m0 + i;
-      if (t < M) {
-        if (t == 0) c0 = v[i];
-        if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
-        if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
-        prev = v[i];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kLC; i++)
-      if (m0 + i < M) col[(int64_t)(m0 + i) * nR] = v[i];
-  }
-  const double tt = prev * ax.rq;
+  const double *col = g + j * cs + r;
+  const double tt = col[(int64_t)(M - 1) * nR] * ax.rq;
   const double ct = dMb ? Ct1[j] : 0.0, w = dMb ? mom_w(r, nR, dR, dMb) : 0.0;
   double *out = fb + (int64_t)j * nU * nR + r;
   double carry = 0.0;
+  double v[kLC];
   for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {
 #pragma unroll
     for (int i = 0; i < kLC; i++)
@@ -788,6 +841,12 @@ void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
 
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
                      double *abp, hipStream_t s) {
+  if (!adj_shape(r)) {  // long columns: the walk (its ascending sweep included)
+    hipLaunchKernelGGL(k_adj_kick_walk, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, fsb, cK,
+                       r.U_Val, accK, cb, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
+                       r.tv);
+    return;
+  }
   hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64, r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
                      r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
   // the fewest passes of at most kB threads over the row, split evenly
@@ -804,7 +863,7 @@ void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipS
                        r.dU, r.dR);
     return;
   }
-  hipLaunchKernelGGL(k_adj_gcol_stream, grid, dim3(64), 0, s, g, fb, r.nR, r.nU, r.cs, r.tv, dMb,
+  hipLaunchKernelGGL(k_adj_gcol_down, grid, dim3(64), 0, s, g, fb, r.nR, r.nU, r.cs, r.tv, dMb,
                      r.Ct1, r.dU, r.dR);
 }
 

@@ -156,8 +156,9 @@ void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s);
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
                      double *abp, hipStream_t s);
 void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s);
-// the kick's transposed v-solve of g (left in cb by launch_adj_kick, destroyed)
-// into fb, plus the transposed moment quadrature of dMb when given
+// the kick's transposed v-solve of g (left in cb by launch_adj_kick, destroyed;
+// for long columns launch_adj_kick has already run its ascending sweep) into
+// fb, plus the transposed moment quadrature of dMb when given
 void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipStream_t s);
 void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
                        std::vector<int64_t> &off, std::vector<uint32_t> &pts);
