@@ -244,30 +244,49 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
     w3 = 0;
     base++;
   };
-  for (int u = 0; u < nU; u++) {
-    const double lam = fj[(int64_t)u * nR];
-    const double newU = U_Val[u] + a;
-    if (lam == 0.0 || (newU <= Ulo) || (newU >= Uhi)) continue;
-    double t;
-    int n;
-    split((newU - start_v) * hinv_v, n, t);
-    // acceleration term over the rows n..n+3 inside the padded column
-    double sk = 0.0;
+  // one point ahead: the loads of point u+1 (its value, its target row and
+  // the coefficient rows there) fly while point u goes through the window
+  auto load = [&](int u, double &lam, double &t, int &n, bool &ok, double (&gk)[4]) {
+    const int uc = min(u, nU - 1);
+    lam = fj[(int64_t)uc * nR];
+    const double newU = U_Val[uc] + a;
+    ok = u < nU && lam != 0.0 && !(newU <= Ulo) && !(newU >= Uhi);
+    split(ok ? (newU - start_v) * hinv_v : 0.0, n, t);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int row = min(n + k, nU + 1);
-      const double *rc = cj + (int64_t)row * S;
-      const double gk = (rc[0] + rc[1] * 4) + rc[2];
-      const double dw = k == 0 ? -dBV2(1. - t) : k == 1 ? -dBV1(1. - t) : k == 2 ? dBV1(t) : dBV2(t);
-      sk += n + k <= nU + 1 ? dw * gk : 0.0;
+      const double *rc = cj + (int64_t)min(n + k, nU + 1) * S;
+      gk[k] = (rc[0] + rc[1] * 4) + rc[2];
     }
-    sAcc += lam * sk;
-    while (base < n) finish();
-    const double l6 = lam * (1.0 / 6.0);
-    w0 += l6 * BV2(1. - t);
-    w1 += l6 * BV1(1. - t);
-    w2 += l6 * BV1(t);
-    w3 += l6 * BV2(t);
+  };
+  double lam, t, gk[4];
+  int n;
+  bool ok;
+  load(0, lam, t, n, ok, gk);
+  for (int u = 0; u < nU; u++) {
+    double lamN, tN, gkN[4];
+    int nN;
+    bool okN;
+    load(u + 1, lamN, tN, nN, okN, gkN);
+    if (ok) {
+      // acceleration term over the rows n..n+3 inside the padded column
+      const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
+      double sk = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) sk += n + k <= nU + 1 ? dw[k] * gk[k] : 0.0;
+      sAcc += lam * sk;
+      while (base < n) finish();
+      const double l6 = lam * (1.0 / 6.0);
+      w0 += l6 * BV2(1. - t);
+      w1 += l6 * BV1(1. - t);
+      w2 += l6 * BV1(t);
+      w3 += l6 * BV2(t);
+    }
+    lam = lamN;
+    t = tN;
+    n = nN;
+    ok = okN;
+#pragma unroll
+    for (int k = 0; k < 4; k++) gk[k] = gkN[k];
   }
   while (base < Mv) finish();
   abp[(int64_t)j * nR + r] = sAcc * (hinv_v / 36.0);
