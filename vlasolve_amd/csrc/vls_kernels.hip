@@ -654,14 +654,20 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   const bool outLane = lane < kKW && k < nR;  // writes the points f[j][.][k]
   const double *col = c + (size_t)j * cs + (k < S ? k : S - 1);
   for (int u = lane; u < nU; u += 64) Us[u] = U_Val[u];
+  // the ring starts zeroed: a point whose target row is the last knot reads
+  // the row past the padded column with weight BV2(0) = 0
+#pragma unroll
+  for (int s = 0; s < kRing + 3; s++) ring[s * 64 + lane] = 0.0;
   const double a = outLane ? acc[k] : 0.0;
   __syncthreads();
   double *fo = f + (size_t)j * nU * nR + (outLane ? k : 0);
 
-  // forward elimination (rows 1..Mv-2)
+  // forward elimination (rows 1..Mv-2); the column streams in kLA rows ahead
+  // of the elimination, so that the elimination runs while it arrives
+  constexpr int kLA = Mv - 2 < 48 ? Mv - 2 : 48;
   double y[Mv];
 #pragma unroll
-  for (int m = 1; m <= Mv - 2; m++) y[m] = col[(int64_t)m * S];
+  for (int m = 1; m <= kLA; m++) y[m] = col[(int64_t)m * S];
   // target-row offsets n - u of the lane's points (while the loads fly)
   int dlo = 1 << 30, dhi = -(1 << 30);
   if (outLane) {
@@ -679,6 +685,7 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   double prev = ax.A03;
 #pragma unroll
   for (int m = 1; m <= Mv - 2; m++) {
+    if (m + kLA <= Mv - 2) y[m + kLA] = col[(int64_t)(m + kLA) * S];
     double dv = dS, ev = rdS;
     if (m <= kHead) {
       dv = ax.d[m + lz];
@@ -690,6 +697,9 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     else
       y[m] = yk;
     prev = yk;
+    // keep the load of row m + kLA in this step (the scheduler would
+    // otherwise hoist every load to the top, where they wait on each other)
+    __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
@@ -697,6 +707,9 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     dhi = max(dhi, __shfl_xor(dhi, o));
   }
   if (dlo > dhi) dlo = dhi = 0;  // no point of the wave evaluates the spline
+  // wave-uniform after the reduction: scalar registers, scalar addressing
+  dlo = __builtin_amdgcn_readfirstlane(dlo);
+  dhi = __builtin_amdgcn_readfirstlane(dhi);
   const int D = dlo;
   const bool useRing = !forceScratch && dhi - dlo <= kRing - 4 - (kEB - 1);
   double *gs = gsc + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (Mv + 1) * 64 + lane;
@@ -728,13 +741,23 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     const double c1 = wave_shl1(cm), c2 = wave_shl1(c1);
     return (cm + c1 * 4) + c2;
   };
-  // Lanes without a point of their own store into their scratch slot, so that
-  // the stores of a block need no lane branch.
-  double *fdump = gs;
   // The points whose steps are rows m0..m0+kEB-1, once row m0 is in: straight
-  // code, so that the kEB evaluations overlap each other and the sweep.
-  auto eval_block = [&](int m0) {
-    double val[kEB];
+  // code, so that the kEB evaluations overlap each other and the sweep.  The
+  // point rows u = m0 + i - D are wave-uniform: their U values and output
+  // rows are addressed from scalar registers.
+  double *fj = f + (size_t)j * nU * nR;
+  const unsigned kcol = outLane ? (unsigned)k : 0u;
+  // A block is evaluated in two halves: prep_block, one block ahead, takes
+  // the points' target rows and basis weights, which do not depend on the
+  // spline, and finish_block reads the ring and forms the sums once the rows
+  // are in, so that the weights of the next block fill the latencies of the
+  // sweep and of the ring reads.
+  struct Prep {
+    double w0, w1, w2, w3;  // BV2(1-y), BV1(1-y), BV1(y), BV2(y)
+    int at;                 // ring index of the point's first row
+    bool in;
+  };
+  auto prep_block = [&](int m0, Prep *pp) {
 #pragma unroll
     for (int i = 0; i < kEB; i++) {
       const int u = m0 + i - D;
@@ -744,17 +767,31 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
       double yy = in ? (newU - start_v) * hinv_v : 0.0;
       int n;
       split(yy, n, yy);
-      const double *rb = ring + (n & (kRing - 1)) * 64 + lane;
+      pp[i].at = (n & (kRing - 1)) * 64 + lane;
+      pp[i].in = in;
+      pp[i].w0 = BV2(1. - yy);
+      pp[i].w1 = BV1(1. - yy);
+      pp[i].w2 = BV1(yy);
+      pp[i].w3 = BV2(yy);
+    }
+  };
+  auto finish_block = [&](int m0, const Prep *pp) {
+    double val[kEB];
+#pragma unroll
+    for (int i = 0; i < kEB; i++) {
+      const double *rb = ring + pp[i].at;
       const double g0 = rb[0], g1 = rb[64], g2 = rb[128], g3 = rb[192];
       double result = 1.0 / 36.0;
-      result *= BV2(1. - yy) * g0 + BV1(1. - yy) * g1 + BV1(yy) * g2 + BV2(yy) * g3;
-      val[i] = in ? result : 0.0;
+      result *= pp[i].w0 * g0 + pp[i].w1 * g1 + pp[i].w2 * g2 + pp[i].w3 * g3;
+      val[i] = pp[i].in ? result : 0.0;
     }
     if (m0 - D >= 0 && m0 + kEB - 1 - D < nU) {
+      if (outLane) {
 #pragma unroll
-      for (int i = 0; i < kEB; i++) {
-        double *dst = outLane ? fo + (size_t)(m0 + i - D) * nR : fdump;
-        *dst = val[i];
+        for (int i = 0; i < kEB; i++) {
+          double *row = fj + (size_t)(m0 + i - D) * nR;  // uniform
+          row[kcol] = val[i];
+        }
       }
     } else {
 #pragma unroll
@@ -777,6 +814,10 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     // back substitution, last row first; a block of kEB rows is evaluated as
     // soon as it is complete
     double cn = cLast, c2 = 0.0;
+    // block b is evaluated at row m0 = Mv - (b + 1) kEB, its points are
+    // u = m0 - D .. m0 - D + kEB - 1
+    Prep pp[kEB];
+    if (Mv - kEB >= 1) prep_block(Mv - kEB, pp);
     ring_put(Mv - 1, gcomb(cn));
 #pragma unroll
     for (int m = Mv - 2; m >= 1; m--) {
@@ -786,7 +827,10 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
       c2 = cn;
       cn = ym - uv * cn;
       ring_put(m, gcomb(cn));
-      if ((Mv - 1 - m) % kEB == kEB - 1) eval_block(m);
+      if ((Mv - 1 - m) % kEB == kEB - 1) {
+        finish_block(m, pp);
+        if (m - kEB >= 1) prep_block(m - kEB, pp);
+      }
     }
     ring_put(0, gcomb((ax.A03 - ax.A01 * cn) - ax.A02 * c2));
     // rows 0..top not covered by a block, and the points whose step would
