@@ -614,12 +614,12 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 // taken during the forward elimination.
 constexpr int kKW = 62, kEB = 8;
 
-// Lane i takes lane i+1's value (lane 63 keeps its own): a whole-wave DPP
-// shift, a VALU move with no trip through LDS.
+// Lane i takes lane i+1's value (lane 63 gets 0): a whole-wave DPP shift, a
+// VALU move with no trip through LDS, into a fresh register.
 __device__ __forceinline__ double wave_shl1(double x) {
   int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -758,11 +758,15 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     bool in;
   };
   auto prep_block = [&](int m0, Prep *pp) {
+    double uv[kEB];
+#pragma unroll
+    for (int i = 0; i < kEB; i++) uv[i] = Us[min(max(m0 + i - D, 0), nU - 1)];
+    // the LDS reads of the block leave together (the scheduler would otherwise
+    // issue each just before its use and wait for it)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < kEB; i++) {
-      const int u = m0 + i - D;
-      const int uc = min(max(u, 0), nU - 1);
-      const double newU = Us[uc] + a;
+      const double newU = uv[i] + a;
       const bool in = !(newU <= Ulo) && !(newU >= Uhi);
       double yy = in ? (newU - start_v) * hinv_v : 0.0;
       int n;
@@ -776,13 +780,18 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     }
   };
   auto finish_block = [&](int m0, const Prep *pp) {
-    double val[kEB];
+    double val[kEB], g[kEB][4];
 #pragma unroll
     for (int i = 0; i < kEB; i++) {
       const double *rb = ring + pp[i].at;
-      const double g0 = rb[0], g1 = rb[64], g2 = rb[128], g3 = rb[192];
+#pragma unroll
+      for (int q = 0; q < 4; q++) g[i][q] = rb[q * 64];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kEB; i++) {
       double result = 1.0 / 36.0;
-      result *= pp[i].w0 * g0 + pp[i].w1 * g1 + pp[i].w2 * g2 + pp[i].w3 * g3;
+      result *= pp[i].w0 * g[i][0] + pp[i].w1 * g[i][1] + pp[i].w2 * g[i][2] + pp[i].w3 * g[i][3];
       val[i] = pp[i].in ? result : 0.0;
     }
     if (m0 - D >= 0 && m0 + kEB - 1 - D < nU) {
