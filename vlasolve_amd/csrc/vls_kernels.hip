@@ -560,13 +560,17 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     is = S;  // = nR + 2, a compile-time stride
     os = S;
   }
+  // the line streams in kLA rows ahead of the elimination, which runs while
+  // it arrives (issued all at once, the loads would wait on each other)
+  constexpr int kLA = M - 2 < 48 ? M - 2 : 48;
   double y[M];
 #pragma unroll
-  for (int k = 1; k <= M - 2; k++) y[k] = in[k * is];
+  for (int k = 1; k <= kLA; k++) y[k] = in[k * is];
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
 #pragma unroll
   for (int k = 1; k <= M - 2; k++) {
+    if (k + kLA <= M - 2) y[k + kLA] = in[(k + kLA) * is];
     double dv = dS, ev = rdS;
     if (k <= kHead) {
       dv = ax.d[k + lz];
@@ -574,6 +578,7 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     }
     y[k] = div_known(y[k] - c16 * prev, dv, ev);
     prev = y[k];
+    __builtin_amdgcn_sched_barrier(0);
   }
   double tt = 0.0 - ax.hi2 * y[M - 3];
   tt = tt - ax.p * y[M - 2];
@@ -1220,18 +1225,20 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
 // slice j (rows of f, contiguous in memory), W = blockDim.x / 64.
 // Right-hand sides arrive in chunks of kC elements per row through a small
 // LDS transpose (loads of whole row segments, then lane = row reads its kC
-// values); the loads of chunk c+1 are issued before the elimination of chunk
-// c so that their latency hides behind it.  Coefficients leave the same way,
-// a chunk at a time as the back substitution produces them.  Loads are
-// clamped into the wave's rows (the extra lanes of a partial wave solve a
-// copy and store nothing).
+// values); the loads of chunks c+1 and c+2 are in flight during the
+// elimination of chunk c.  Coefficients leave the same way, a chunk at a
+// time as the back substitution produces them.  Loads are clamped into the
+// wave's rows (the extra lanes of a partial wave solve a copy and store
+// nothing).
 //
 // MOM: the block holds a whole slice (W = ceil(nU/64) <= 4 waves) and also
 // forms the slice's moment partials P[j][r] from the same data, in the tree
 // order of k_moment_partial: each lane makes the terms of its row pair
-// (u, u+1) (row u+1 is the next lane's, or the next wave's first row), the
-// terms go through LDS, a thread per (wave, group, r) sums its 16 rows, and
-// one thread per r combines the groups and the waves.
+// (u, u+1) (row u+1 is the next lane's; each wave also stages the row after
+// its own), the terms go through LDS, a thread per (group, r) sums its 16
+// rows, a thread per r combines the wave's four groups into its partial
+// W_w[r], and after the forward sweep one thread per r combines the waves.
+// Until then the waves run independently.
 constexpr int kRowsMaxW = 4;
 
 template <int M, bool MOM>
@@ -1242,9 +1249,12 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
   constexpr int nIn = M - 2;           // right-hand sides per line (= nR)
   constexpr int nChIn = (nIn + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
-  __shared__ double chunkAll[(MOM ? kRowsMaxW : 1) * 64 * kP];
+  // chunk rows: the wave's 64 rows, and for the moment the row after them
+  constexpr int kCR = MOM ? 65 : 64;
+  __shared__ double chunkAll[(MOM ? kRowsMaxW : 1) * kCR * kP];
   __shared__ double termAll[MOM ? kRowsMaxW * 64 * kP : 1];
   __shared__ double gsum[MOM ? kRowsMaxW * 4 * kC : 1];
+  __shared__ double wsum[MOM ? kRowsMaxW * nIn : 1];  // per-wave partials W_w[r]
   const int W = blockDim.x >> 6;
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -1252,19 +1262,40 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   const int j = blockIdx.y;
   const int u0 = (blockIdx.x * W + w) * 64;
   const int rows = max(min(64, nU - u0), 1);
+  const int rowsE = max(min(kCR, nU - u0), 1);
   const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC elements per pass
-  double *chunk = chunkAll + w * 64 * kP;
+  double *chunk = chunkAll + w * kCR * kP;
   const double *src = f + ((size_t)j * nU + min(u0, nU - 1)) * nIn;
   double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  // The waves of a block exchange nothing until the moment's final sums:
+  // each stages its own chunks (and rows), so a wave-level barrier orders
+  // its LDS traffic.
+  auto sync = [] {
+    if (MOM) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  };
   double y[M];
-  double tmp[kC];
-  auto load_chunk = [&](int ch) {
+  // chunks of loads in flight: two (the loads of a chunk take a memory
+  // latency, the elimination of a chunk far less); the moment variant makes
+  // room for them by keeping its first kYL forward values in LDS, the last
+  // ones the back substitution needs
+  constexpr int kAh = 2;
+  constexpr int kYL = MOM && M > 160 ? 32 : 0;
+  __shared__ double ylds[kYL > 0 ? kRowsMaxW * 64 * kYL : 1];
+  double tmp[kAh][kC], tmpE[kAh];
+  auto load_chunk = [&](int ch, double *t, double &te) {
     const int e = min(ch * kC + ksub, nIn - 1);
 #pragma unroll
     for (int q = 0; q < kC; q++) {
       const int row = min(q * 4 + rsub, rows - 1);
-      tmp[q] = src[row * nIn + e];
+      t[q] = src[row * nIn + e];
     }
+    if (MOM) te = src[min(64, rowsE - 1) * nIn + e];
   };
   // moment: this lane's row pair, its weight, and the values carried from
   // the previous chunk's last element
@@ -1274,25 +1305,28 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   double aPrev = 0.0, bPrev = 0.0;
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
-  load_chunk(0);
+#pragma unroll
+  for (int ch = 0; ch < kAh; ch++)
+    if (ch < nChIn) load_chunk(ch, tmp[ch], tmpE[ch]);
 #pragma unroll
   for (int ch = 0; ch < nChIn; ch++) {
 #pragma unroll
-    for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[q];
-    __syncthreads();
+    for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[ch % kAh][q];
+    if (MOM && rsub == 0) chunk[64 * kP + ksub] = tmpE[ch % kAh];
+    sync();
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nIn) y[1 + ch * kC + i] = chunk[lane * kP + i];
     if (MOM) {
-      // row u+1: the next lane's row, or the next wave's first row
-      const double *nxt = lane < 63 ? chunk + (lane + 1) * kP : chunkAll + (w + 1) * 64 * kP;
+      // row u+1: the next chunk row (row 64 is the next wave's first row)
+      const double *nxt = chunk + (lane + 1) * kP;
       double *T = termAll + (w * 64 + lane) * kP;
 #pragma unroll
       for (int i = 0; i < kC; i++) {
         const int e = ch * kC + i;  // column of the new values; term r = e - 1
         if (e < nIn) {
           const double a1 = y[1 + e];
-          const double b1 = (lane < 63 || w + 1 < W) ? nxt[i] : 0.0;
+          const double b1 = nxt[i];
           double t = 0.0;
           if (pairOk && e >= 1) {
             const double avg = (((aPrev + a1) + bPrev) + b1) * 0.25;
@@ -1306,8 +1340,8 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
         }
       }
     }
-    __syncthreads();
-    if (ch + 1 < nChIn) load_chunk(ch + 1);
+    sync();
+    if (ch + kAh < nChIn) load_chunk(ch + kAh, tmp[ch % kAh], tmpE[ch % kAh]);
     if (MOM) {
       // a thread per (group of 16 rows, r) of its wave: serial over the rows
       const int i = lane & 15, q = lane >> 4;
@@ -1326,26 +1360,33 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
           dv = ax.d[k + lz];
           ev = ax.rd[k + lz];
         }
-        y[k] = div_known(y[k] - c16 * prev, dv, ev);
-        prev = y[k];
+        const double yk = div_known(y[k] - c16 * prev, dv, ev);
+        if (k <= kYL)
+          ylds[(k - 1) * (kRowsMaxW * 64) + threadIdx.x] = yk;
+        else
+          y[k] = yk;
+        prev = yk;
       }
     }
     if (MOM) {
-      __syncthreads();
-      if (threadIdx.x < kC) {
-        const int i = threadIdx.x, r = ch * kC + i - 1;
-        double Pj = 0.0;
-        for (int ww = 0; ww < W; ww++) {
-          const double *g = gsum + ww * 4 * kC + i;
-          const double Wsum = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
-          Pj = ww == 0 ? Wsum : Pj + Wsum;
-        }
-        if (r >= 0 && r < nR - 1) P[(size_t)j * nR + r] = Pj;
-        if (r == nR - 1) P[(size_t)j * nR + r] = 0.0;
+      sync();
+      // the wave's partial over its 64 rows: its four groups in order
+      if (lane < kC) {
+        const int i = lane, r = ch * kC + i - 1;
+        const double *g = gsum + w * 4 * kC + i;
+        if (r >= 0 && r < nIn) wsum[w * nIn + r] = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
       }
     }
   }
-  if (MOM && threadIdx.x == 0 && nChIn * kC - 1 == nR - 1) P[(size_t)j * nR + nR - 1] = 0.0;
+  if (MOM) {
+    // the slice's partials: the waves left to right
+    __syncthreads();
+    for (int r = threadIdx.x; r < nR; r += blockDim.x) {
+      double Pj = wsum[r];
+      for (int ww = 1; ww < W; ww++) Pj = Pj + wsum[ww * nIn + r];
+      P[(size_t)j * nR + r] = r < nR - 1 ? Pj : 0.0;
+    }
+  }
   double tt = 0.0 - ax.hi2 * y[M - 3];
   tt = tt - ax.p * y[M - 2];
   y[M - 1] = div_known(tt, ax.q, ax.rq);
@@ -1360,7 +1401,8 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
       if (k >= 1 && k <= M - 2) {
         double uv = uS;
         if (k <= kHead) uv = ax.u[k + lz];
-        y[k] = y[k] - uv * cn;
+        const double yk = k <= kYL ? ylds[(k - 1) * (kRowsMaxW * 64) + threadIdx.x] : y[k];
+        y[k] = yk - uv * cn;
         cn = y[k];
       }
     }
@@ -1368,13 +1410,13 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < M) chunk[lane * kP + i] = y[ch * kC + i];
-    __syncthreads();
+    sync();
     // all LDS reads first, so that their latencies overlap, then the stores
     const int k = ch * kC + ksub;
     double sv[kC];
 #pragma unroll
     for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
-    __syncthreads();
+    sync();
     if (u0 < nU) {
       if (rows == 64 && (ch + 1) * kC <= M) {
 #pragma unroll
