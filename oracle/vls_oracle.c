@@ -31,6 +31,9 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+
+/* slices per chunk of the device-order radial mass sum over j */
+#define ORC_JCHUNK 20
 #endif
 
 static char g_err[512];
@@ -534,7 +537,9 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv, int ns)
   /* device order: the rows u of a slice form waves of 64 and groups of 16
    * (u = 64w + 16q + t); G_wq = serial sum over the group's rows,
    * W_w = ((G_w0 + G_w1) + G_w2) + G_w3, P[j][r] = ((W_0 + W_1) + ...) over
-   * the waves, then dM[r] = serial sum over j */
+   * the waves; the slices form chunks of ORC_JCHUNK (j = 20c + i),
+   * S_c = serial sum over the chunk's slices, dM[r] = serial sum of the S_c
+   * (like the reference's per-thread partials, quadrature21.hxx:104-133) */
   double *P = calloc((size_t)nJ * nR, sizeof(double));
   const long nW = (nU + 63) / 64;
 #pragma omp parallel for schedule(static)
@@ -562,7 +567,11 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv, int ns)
   }
   for (r = 0; r < nR; r++) {
     double acc = 0;
-    for (long j = 0; j < nJ; j++) acc += P[(size_t)j * nR + r];
+    for (long c0 = 0; c0 < nJ; c0 += ORC_JCHUNK) {
+      double S = 0.0;
+      for (long j = c0; j < nJ && j < c0 + ORC_JCHUNK; j++) S += P[(size_t)j * nR + r];
+      acc += S;
+    }
     Mrv[r] = acc;
   }
   free(P);

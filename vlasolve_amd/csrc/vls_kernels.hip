@@ -125,40 +125,31 @@ k_moment_partial_sp(const double *__restrict__ f, const double *__restrict__ Ct1
   P[(size_t)j * nR + r] = Pj;
 }
 
-// dM[r] = serial sum over j; exclusive radial prefix (gridHandler::accumulate
-// with addSpare=1, src/grid_handler.hxx:350-407); acceleration
-// M(<r) * (G/R^2 * dt) (src/vlasovSolver21_spline.hxx:746).  One block: a
-// thread per r sums its column of partials with kB loads in flight; one
-// thread then runs the serial prefix on registers, kB entries at a time.
-__global__ void __launch_bounds__(256)
-k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, double *__restrict__ dM,
-              double *__restrict__ Mx, double *__restrict__ acc, double *__restrict__ mTot, int nR,
-              int nJ) {
-  constexpr int kB = 64;  // loads in flight per thread (one wave per SIMD)
-  extern __shared__ double sdm[];  // [nR]
-  for (int r = threadIdx.x; r < nR; r += blockDim.x) {
-    double s = 0.0;
-    int j = 0;
-    for (; j + kB <= nJ; j += kB) {
-      double v[kB];
-#pragma unroll
-      for (int k = 0; k < kB; k++) v[k] = P[(size_t)(j + k) * nR + r];
-#pragma unroll
-      for (int k = 0; k < kB; k++) s += v[k];
-    }
-    for (; j < nJ; j++) s += P[(size_t)j * nR + r];
-    dM[r] = s;
-    sdm[r] = s;
-  }
+// dM[r] = sum over j of the partials P[j][r] (in chunks of slices, below);
+// exclusive radial prefix (gridHandler::accumulate with addSpare=1,
+// src/grid_handler.hxx:350-407); acceleration M(<r) * (G/R^2 * dt)
+// (src/vlasovSolver21_spline.hxx:746).  The scan runs as one block and ends
+// in scan_prefix_out: one thread runs the serial prefix over r on registers,
+// kS entries at a time, then the block writes M(<r) and the acceleration.
+__device__ __forceinline__ void scan_prefix_out(double *sdm, const double *__restrict__ gfac,
+                                                double *__restrict__ Mx, double *__restrict__ acc,
+                                                double *__restrict__ mTot, int nR, double gf0) {
   __syncthreads();
   if (threadIdx.x == 0) {
+    // the next batch's LDS reads are issued before the current batch's adds
     constexpr int kS = 16;
     double prev = 0.0;
-    int r = 0;
-    for (; r + kS <= nR; r += kS) {
-      double v[kS];
+    double v[kS], nx[kS];
+    const int nFull = nR / kS;
 #pragma unroll
-      for (int k = 0; k < kS; k++) v[k] = sdm[r + k];
+    for (int k = 0; k < kS; k++) nx[k] = nFull > 0 ? sdm[k] : 0.0;
+    for (int b = 0; b < nFull; b++) {
+#pragma unroll
+      for (int k = 0; k < kS; k++) v[k] = nx[k];
+      if (b + 1 < nFull) {
+#pragma unroll
+        for (int k = 0; k < kS; k++) nx[k] = sdm[(b + 1) * kS + k];
+      }
 #pragma unroll
       for (int k = 0; k < kS; k++) {
         const double d = v[k];
@@ -166,9 +157,9 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
         prev += d;
       }
 #pragma unroll
-      for (int k = 0; k < kS; k++) sdm[r + k] = v[k];
+      for (int k = 0; k < kS; k++) sdm[b * kS + k] = v[k];
     }
-    for (; r < nR; r++) {
+    for (int r = nFull * kS; r < nR; r++) {
       const double d = sdm[r];
       sdm[r] = prev;
       prev += d;
@@ -179,8 +170,60 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     double m = sdm[r];
     Mx[r] = m;
-    acc[r] = m * gfac[r];
+    acc[r] = m * (r == (int)threadIdx.x ? gf0 : gfac[r]);
   }
+}
+
+// The sum over j runs in chunks of kJC slices: S_c = serial sum over the
+// slices of chunk c, dM[r] = serial sum of the S_c.  k_moment_chunks forms
+// the S_c with a block per chunk (the partials are read by many CUs at
+// once) and leaves S_c in place of the chunk's first slice of P (read by the
+// same thread only); k_moment_scan adds the chunks and runs the prefix.
+constexpr int kScanThreads = 1024, kJC = 20, kChunkThreads = 256;
+__global__ void __launch_bounds__(kChunkThreads)
+k_moment_chunks(double *__restrict__ P, int nR, int nJ) {
+  const int r = blockIdx.y * kChunkThreads + threadIdx.x;
+  if (r >= nR) return;
+  const int jb = blockIdx.x * kJC, je = min(nJ, jb + kJC);
+  double v[kJC];
+  // clamped rows are loaded, not added
+#pragma unroll
+  for (int k = 0; k < kJC; k++) v[k] = P[(size_t)min(jb + k, nJ - 1) * nR + r];
+  double S = 0.0;
+  if (je - jb == kJC) {
+#pragma unroll
+    for (int k = 0; k < kJC; k++) S += v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < kJC; k++)
+      if (jb + k < je) S += v[k];
+  }
+  P[(size_t)jb * nR + r] = S;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, double *__restrict__ dM,
+              double *__restrict__ Mx, double *__restrict__ acc, double *__restrict__ mTot, int nR,
+              int nJ) {
+  extern __shared__ double sdm[];  // [nR]
+  const int t = threadIdx.x;
+  const int nC = (nJ + kJC - 1) / kJC;
+  const double gf0 = t < nR ? gfac[t] : 0.0;
+  for (int r = t; r < nR; r += kScanThreads) {
+    constexpr int kB = 16;  // chunk sums in flight
+    double a = 0.0;
+    for (int c0 = 0; c0 < nC; c0 += kB) {
+      double v[kB];
+#pragma unroll
+      for (int k = 0; k < kB; k++) v[k] = P[(size_t)min(c0 + k, nC - 1) * kJC * nR + r];
+#pragma unroll
+      for (int k = 0; k < kB; k++)
+        if (c0 + k < nC) a += v[k];
+    }
+    dM[r] = a;
+    sdm[r] = a;
+  }
+  scan_prefix_out(sdm, gfac, Mx, acc, mTot, nR, gf0);
 }
 
 // ---------------------------------------------------------------- spline
@@ -1089,9 +1132,14 @@ k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val,
               double *__restrict__ dM, double *__restrict__ q, double *__restrict__ out, int nR,
               int nJ, int nS) {
   const int64_t n = (int64_t)nJ * nR;
+  // the radial mass in the order of the moment scan: chunks of kJC slices
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
     double s = 0.0;
-    for (int j = 0; j < nJ; j++) s += part[(int64_t)j * nR + r];
+    for (int j0 = 0; j0 < nJ; j0 += kJC) {
+      double S = 0.0;
+      for (int j = j0; j < nJ && j < j0 + kJC; j++) S += part[(int64_t)j * nR + r];
+      s += S;
+    }
     dM[r] = s;
   }
   for (int t = threadIdx.x; t < 6 * nJ; t += blockDim.x) {
@@ -1172,8 +1220,11 @@ void launch_moment(Run &r, hipStream_t s) {
 }
 
 void launch_moment_scan(Run &r, hipStream_t s) {
-  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(256), r.nR * sizeof(double), s, r.P, r.gfac,
-                     r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
+  const int nC = (r.nJ + kJC - 1) / kJC;
+  hipLaunchKernelGGL(k_moment_chunks, dim3(nC, (r.nR + kChunkThreads - 1) / kChunkThreads),
+                     dim3(kChunkThreads), 0, s, r.P, r.nR, r.nJ);
+  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kScanThreads), r.nR * sizeof(double), s, r.P,
+                     r.gfac, r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
 }
 
 void configure_tiles(Run &r, int maxLdsPerBlock) {
@@ -1284,7 +1335,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   // latency, the elimination of a chunk far less); the moment variant makes
   // room for them by keeping its first kYL forward values in LDS, the last
   // ones the back substitution needs
-  constexpr int kAh = 2;
+  constexpr int kAh = MOM ? 2 : 3;
   constexpr int kYL = MOM && M > 160 ? 32 : 0;
   __shared__ double ylds[kYL > 0 ? kRowsMaxW * 64 * kYL : 1];
   double tmp[kAh][kC], tmpE[kAh];
