@@ -1306,6 +1306,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   __shared__ double termAll[MOM ? kRowsMaxW * 64 * kP : 1];
   __shared__ double gsum[MOM ? kRowsMaxW * 4 * kC : 1];
   __shared__ double wsum[MOM ? kRowsMaxW * nIn : 1];  // per-wave partials W_w[r]
+  __shared__ double dRs[MOM ? nIn : 1];                // dR, read from LDS by the terms
   const int W = blockDim.x >> 6;
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -1335,7 +1336,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   // latency, the elimination of a chunk far less); the moment variant makes
   // room for them by keeping its first kYL forward values in LDS, the last
   // ones the back substitution needs
-  constexpr int kAh = MOM ? 2 : 3;
+  constexpr int kAh = 2;
   constexpr int kYL = MOM && M > 160 ? 32 : 0;
   __shared__ double ylds[kYL > 0 ? kRowsMaxW * 64 * kYL : 1];
   double tmp[kAh][kC], tmpE[kAh];
@@ -1354,17 +1355,34 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   const bool pairOk = MOM && u < nU - 1;
   const double ctdu = pairOk ? Ct1[j] * dU[u] : 0.0;
   double aPrev = 0.0, bPrev = 0.0;
+  if (MOM) {
+    for (int k = threadIdx.x; k < nIn; k += blockDim.x) dRs[k] = dR[k];
+    __syncthreads();
+  }
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
 #pragma unroll
   for (int ch = 0; ch < kAh; ch++)
     if (ch < nChIn) load_chunk(ch, tmp[ch], tmpE[ch]);
+  // the wave's partial over its 64 rows for chunk ch: its four groups in order
+  auto wave_partial = [&](int ch) {
+    if (lane < kC) {
+      const int i = lane, r = ch * kC + i - 1;
+      const double *g = gsum + w * 4 * kC + i;
+      if (r >= 0 && r < nIn) wsum[w * nIn + r] = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
+    }
+  };
+  // Per chunk: stage it (and issue the loads of chunk ch + kAh into the
+  // registers just staged), then the terms and the elimination, then the
+  // group sums; a group's wave partial is formed at the next chunk.
 #pragma unroll
   for (int ch = 0; ch < nChIn; ch++) {
 #pragma unroll
     for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[ch % kAh][q];
     if (MOM && rsub == 0) chunk[64 * kP + ksub] = tmpE[ch % kAh];
+    if (ch + kAh < nChIn) load_chunk(ch + kAh, tmp[ch % kAh], tmpE[ch % kAh]);
     sync();
+    if (MOM && ch > 0) wave_partial(ch - 1);
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nIn) y[1 + ch * kC + i] = chunk[lane * kP + i];
@@ -1379,9 +1397,11 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
           const double a1 = y[1 + e];
           const double b1 = nxt[i];
           double t = 0.0;
-          if (pairOk && e >= 1) {
+          if (e >= 1) {
+            // branch-free: rows without a pair keep 0
             const double avg = (((aPrev + a1) + bPrev) + b1) * 0.25;
-            t = (ctdu * dR[e - 1]) * avg;
+            const double tv = (ctdu * dRs[e - 1]) * avg;
+            t = pairOk ? tv : 0.0;
           }
           T[i] = t;
           aPrev = a1;
@@ -1390,17 +1410,6 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
           T[i] = 0.0;
         }
       }
-    }
-    sync();
-    if (ch + kAh < nChIn) load_chunk(ch + kAh, tmp[ch % kAh], tmpE[ch % kAh]);
-    if (MOM) {
-      // a thread per (group of 16 rows, r) of its wave: serial over the rows
-      const int i = lane & 15, q = lane >> 4;
-      const double *T = termAll + (w * 64 + 16 * q) * kP + i;
-      double g = 0.0;
-#pragma unroll
-      for (int t = 0; t < 16; t++) g += T[t * kP];
-      gsum[(w * 4 + q) * kC + i] = g;
     }
 #pragma unroll
     for (int i = 0; i < kC; i++) {
@@ -1419,15 +1428,20 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
         prev = yk;
       }
     }
+    sync();
     if (MOM) {
-      sync();
-      // the wave's partial over its 64 rows: its four groups in order
-      if (lane < kC) {
-        const int i = lane, r = ch * kC + i - 1;
-        const double *g = gsum + w * 4 * kC + i;
-        if (r >= 0 && r < nIn) wsum[w * nIn + r] = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
-      }
+      // a thread per (group of 16 rows, r) of its wave: serial over the rows
+      const int i = lane & 15, q = lane >> 4;
+      const double *T = termAll + (w * 64 + 16 * q) * kP + i;
+      double g = 0.0;
+#pragma unroll
+      for (int t = 0; t < 16; t++) g += T[t * kP];
+      gsum[(w * 4 + q) * kC + i] = g;
     }
+  }
+  if (MOM) {
+    sync();
+    wave_partial(nChIn - 1);
   }
   if (MOM) {
     // the slice's partials: the waves left to right
