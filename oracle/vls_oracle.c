@@ -116,6 +116,8 @@ struct orc_state {
   double *coef;          /* nJ padded coefficient slices */
   double *tmpArr;        /* per-thread Thomas work rows */
   long tmpStride;
+  double *colArr;        /* per-thread v-columns of the device-order kick */
+  long colStride;
   /* feet (vlasovSolver21.hxx:92-94, vlasovSolver21_spline.hxx:69-72) */
   double *driftR, *driftU;
   unsigned char *driftOut;
@@ -825,6 +827,64 @@ static void kick(orc_state *s, double *f)
   }
 }
 
+/* Device-order kick (nthreads == 0).  The kick evaluates the 2D spline on r
+ * nodes only, where its r-direction combination (c[r-1] + 4c[r] + c[r+1])/6
+ * gives back the r-solve's right-hand sides exactly: the (1,4,1)/6
+ * combination of the 2D coefficients is the natural v-spline of the column
+ * of f, and evaluateNxPy's (1/36) sum_k B_k(t) (c + 4c + c)[n+k] equals
+ * (1/6) sum_k B_k(t) d[n+k] with d the v-spline coefficients of the column.
+ * The device computes that form (one v-solve per column, no r-solve); it
+ * agrees with the reference's to rounding.  With tf (tangent-linear model)
+ * the tangent column goes through the same solve and the target's motion
+ * adds (hinv/6) sum_k B_k'(t) d[n+k] * tMr[r]. */
+static void kick_cols(orc_state *s, double *f, double *tf)
+{
+  const long nR = s->nR, nU = s->nU, M = s->sdim[1] + 2;
+  const double Ulo = s->U_Val[0], Uhi = s->U_Val[nU - 1];
+  const double st = s->sstart[1], hinv = s->sstep_inv[1];
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < s->nJ; j++) {
+    double *A = s->tmpArr + (size_t)thread_id() * s->tmpStride;
+    double *d = s->colArr + (size_t)thread_id() * s->colStride, *td = d + (M + 1);
+    for (long r = 0; r < nR; r++) {
+      for (long u = 0; u < nU; u++) {
+        d[u + 1] = f[(j * nU + u) * nR + r];
+        if (tf) td[u + 1] = tf[(j * nU + u) * nR + r];
+      }
+      create_1D(s, A, d, 1, 1);
+      d[M] = 0; /* the row past the column, read with weight BV2(0) = 0 */
+      if (tf) {
+        create_1D(s, A, td, 1, 1);
+        td[M] = 0;
+      }
+      for (long u = 0; u < nU; u++) {
+        const long i = (j * nU + u) * nR + r;
+        const double newU = s->U_Val[u] + s->Mr[r];
+        if ((newU <= Ulo) || (newU >= Uhi)) {
+          f[i] = 0;
+          if (tf) tf[i] = 0;
+          continue;
+        }
+        double yy = rescale(newU, s->p.Uscale[0]);
+        double y = (yy - st) * hinv;
+        y = modf(y, &yy);
+        const long n = (long)yy;
+        double result = 1.0 / 6.0;
+        result *= BV2(1. - y) * d[n] + BV1(1. - y) * d[n + 1] + BV1(y) * d[n + 2] + BV2(y) * d[n + 3];
+        f[i] = result;
+        if (tf) {
+          const double w = 1 - y;
+          const double b0 = -3 * w * w, b1 = -(3 + 6 * w - 9 * w * w), b2 = 3 + 6 * y - 9 * y * y,
+                       b3 = 3 * y * y;
+          double tr = 1.0 / 6.0;
+          tr *= BV2(1. - y) * td[n] + BV1(1. - y) * td[n + 1] + BV1(y) * td[n + 2] + BV2(y) * td[n + 3];
+          tf[i] = tr + hinv / 6.0 * (b0 * d[n] + b1 * d[n + 1] + b2 * d[n + 2] + b3 * d[n + 3]) * s->tMr[r];
+        }
+      }
+    }
+  }
+}
+
 /* drift loop of species k, vlasovSolver21_spline.hxx:769-798 (delayed-kernel
  * back values :776-786, then per point :788-796); the delay lines turn on the
  * last species' reads (:793) */
@@ -873,8 +933,12 @@ void orc_step(orc_state *s, int nsteps)
       }
     moment_and_accel(s);
     for (int k = 0; k < s->nS; k++) {
-      build_all(s, s->f + (size_t)k * s->nVal);
-      kick(s, s->f + (size_t)k * s->nVal);
+      if (s->p.nthreads == 0) {
+        kick_cols(s, s->f + (size_t)k * s->nVal, NULL);
+      } else {
+        build_all(s, s->f + (size_t)k * s->nVal);
+        kick(s, s->f + (size_t)k * s->nVal);
+      }
     }
     for (int k = 0; k < s->nS; k++) {
       build_all(s, s->f + (size_t)k * s->nVal);
@@ -968,6 +1032,8 @@ orc_state *orc_create(const orc_params *p)
   s->coef = calloc((size_t)s->nJ * s->cstride + 2 * (size_t)(s->sdim[0] + 2) + 8, sizeof(double));
   s->tmpStride = 4 * (2 + (s->sdim[0] > s->sdim[1] ? s->sdim[0] : s->sdim[1]));
   s->tmpArr = malloc((size_t)thread_count() * s->tmpStride * sizeof(double));
+  s->colStride = 2 * ((long)s->sdim[1] + 3);
+  s->colArr = malloc((size_t)thread_count() * s->colStride * sizeof(double));
 
   /* vlasovSolver21::setup, vlasovSolver21.hxx:411-446 */
   s->kernelRadius = s->R_Val[p->kernelBufferSize];
@@ -1100,11 +1166,15 @@ static void tl_step1(orc_state *s)
   for (long i = 0; i < nR; i++) s->tMr[i] *= s->p.G / (s->R_Val[i] * s->R_Val[i]) * s->p.dt;
   free(tdM);
   /* kick */
-  build_all(s, s->f);
-  tl_build(s);
   const double Ulo = s->U_Val[0], Uhi = s->U_Val[nU - 1];
+  if (s->p.nthreads == 0) {
+    kick_cols(s, s->f, s->tf);
+  } else {
+    build_all(s, s->f);
+    tl_build(s);
+  }
 #pragma omp parallel for schedule(static)
-  for (long j = 0; j < nJ; j++) {
+  for (long j = 0; j < (s->p.nthreads == 0 ? 0 : nJ); j++) {
     const double *c = s->coef + (size_t)j * s->cstride;
     const double *tc = s->tcoef + (size_t)j * s->cstride;
     for (long u = 0; u < nU; u++)
@@ -1179,7 +1249,7 @@ void orc_destroy(orc_state *s)
   }
   free(s->chains);
   free(s->R_Val); free(s->R_V); free(s->U_Val); free(s->U_V); free(s->J_Val); free(s->J_V);
-  free(s->f); free(s->coef); free(s->tmpArr);
+  free(s->f); free(s->coef); free(s->tmpArr); free(s->colArr);
   free(s->driftR); free(s->driftU); free(s->driftOut);
   free(s->driftR_I); free(s->driftU_I); free(s->driftR_F); free(s->driftU_F);
   free(s->chainOfPoint); free(s->chainStart); free(s->dM); free(s->Mr);

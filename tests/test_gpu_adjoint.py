@@ -33,9 +33,19 @@ def chain_layout(cfg):
     return ch["off"], ch["len"]
 
 
-@pytest.mark.parametrize("n,ck", [(6, 4), (5, 1)])
-def test_adjoint_matches_tangent_linear(n, ck):
-    print(f"case udf_tiny adjoint n={n} ck={ck}", flush=True)
+@pytest.mark.parametrize("n,ck,fused,keep", [(6, 4, False, None), (5, 1, False, "0"),
+                                             (6, 4, True, "2"), (6, 4, True, "0")])
+def test_adjoint_matches_tangent_linear(monkeypatch, n, ck, fused, keep):
+    """fused: the recorded steps keep the kick's column splines from the
+    fused kick (small grids run the column kernels by default).  keep caps
+    the steps the recorded run keeps itself (None: all of them here; "0":
+    every segment recomputed in the reverse sweep; "2": the last segment
+    kept, the first recomputed)."""
+    print(f"case udf_tiny adjoint n={n} ck={ck} fused={fused} keep={keep}", flush=True)
+    if fused:
+        monkeypatch.setenv("VLS_FUSED_KICK", "1")
+    if keep is not None:
+        monkeypatch.setenv("VLS_ADJ_KEEP_STEPS", keep)
     cfg = UDF_TINY
     solver = V.Solver(cfg)
     ctx = solver.context
@@ -66,8 +76,9 @@ def test_adjoint_matches_tangent_linear(n, ck):
     assert np.array_equal(ctx.download_f(0, f0.shape), _after(cfg, n))
 
 
-@pytest.mark.parametrize("pres,ures", [(15, [15, 6]), (20, [18, 5])])
-def test_adjoint_generic_lines(pres, ures):
+@pytest.mark.parametrize("pres,ures,keep", [(15, [15, 6], None), (20, [18, 5], None),
+                                            (20, [18, 5], "0")])
+def test_adjoint_generic_lines(monkeypatch, pres, ures, keep):
     """Shapes without register line solvers: the streaming line adjoints.
     With 18 padded nodes per line the last row's two coefficients fall on
     both sides of a 16-element chunk boundary."""
@@ -75,7 +86,9 @@ def test_adjoint_generic_lines(pres, ures):
 
     cfg = dict(UDF_C1, **{"grid.Pres": pres, "grid.Ures": ures})
     n, ck = 5, 2
-    print(f"case udf {pres}x{ures} adjoint n={n} ck={ck}", flush=True)
+    print(f"case udf {pres}x{ures} adjoint n={n} ck={ck} keep={keep}", flush=True)
+    if keep is not None:
+        monkeypatch.setenv("VLS_ADJ_KEEP_STEPS", keep)
     solver = V.Solver(cfg)
     ctx = solver.context
     f0 = solver.f()

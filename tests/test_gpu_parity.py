@@ -101,31 +101,47 @@ def _bit_exact_run(cfg, nsteps, label):
     assert same(dMg, dMo) and same(ag, ao) and mg == mo
 
 
+def test_fused_kick_small_bit_exact(monkeypatch):
+    """The fused kick (k_kick_line), which small grids do not take by
+    default, forced on the 65 x 65 x 32 grid."""
+    from tests.configs import UDF_C1
+
+    print("case udf_c1 with VLS_FUSED_KICK", flush=True)
+    monkeypatch.setenv("VLS_FUSED_KICK", "1")
+    _bit_exact_run(UDF_C1, 12, "fused kick")
+
+
 def test_kick_scratch_path_bit_exact(monkeypatch):
     """The fused kick's global-scratch path (taken by waves whose kick targets
     spread wider than its LDS ring), forced for every wave."""
     from tests.configs import UDF_C1
 
-    print("case udf_c1 with VLS_KICK_SCRATCH", flush=True)
+    print("case udf_c1 with VLS_FUSED_KICK, VLS_KICK_SCRATCH", flush=True)
+    monkeypatch.setenv("VLS_FUSED_KICK", "1")
     monkeypatch.setenv("VLS_KICK_SCRATCH", "1")
     _bit_exact_run(UDF_C1, 12, "scratch path")
 
 
-def test_strong_kick_bit_exact():
-    """Kicks of tens of v cells per step: ring and scratch paths side by side."""
+@pytest.mark.parametrize("fused", [False, True])
+def test_strong_kick_bit_exact(monkeypatch, fused):
+    """Kicks of tens of v cells per step: the column kernels, and the fused
+    kick's ring and scratch paths side by side."""
     from tests.configs import UDF_STRONG_KICK
 
-    print("case udf_strong_kick", flush=True)
+    print(f"case udf_strong_kick fused={fused}", flush=True)
+    if fused:
+        monkeypatch.setenv("VLS_FUSED_KICK", "1")
     _bit_exact_run(UDF_STRONG_KICK, 10, "strong kick")
 
 
 def test_unfused_kick_bit_exact(monkeypatch):
-    """The separate v-solve and kick kernels (shapes without register lines)."""
-    from tests.configs import UDF_C1
+    """The column v-solve and evaluation kernels on the 201 x 201 x 200 grid,
+    which takes the fused kick by default."""
+    from tests.configs import UDF_C2
 
-    print("case udf_c1 with VLS_NO_FUSED_KICK", flush=True)
+    print("case udf_c2 with VLS_NO_FUSED_KICK", flush=True)
     monkeypatch.setenv("VLS_NO_FUSED_KICK", "1")
-    _bit_exact_run(UDF_C1, 12, "unfused kick")
+    _bit_exact_run(UDF_C2, 3, "unfused kick")
 
 
 @pytest.mark.parametrize("pres,ures", [(20, [18, 5]), (31, [33, 6])])

@@ -204,8 +204,8 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
 // The kick's transpose for columns too long for registers, in two kernels
 // around the scan.  k_adj_kick_walk: one lane per column r walks the points
 // u in order and, for each point in the v range, adds its acceleration term
-// fb*[u] sum_k B_k'(t) g_{n+k} (the kick coefficients' column combination)
-// to the column's sum, and its B_k(t) / 6 fb*[u] to rows n..n+3 of the
+// fb*[u] sum_k B_k'(t) d_{n+k} (d the kept v-spline coefficients of the
+// column) to the column's sum, and its B_k(t) / 6 fb*[u] to rows n..n+3 of the
 // column's g, kept as a window of four rows.  n never decreases with u, so a
 // row is complete once a point targets rows above it: it then takes its
 // step of the ascending sweep of the transposed v-solve and is stored over g
@@ -216,13 +216,16 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
                 const double *__restrict__ U_Val, const double *__restrict__ acc,
                 double *__restrict__ g, double *__restrict__ abp, int nR, int nU, int64_t cs,
                 double Ulo, double Uhi, double start_v, double hinv_v, ThomasAxis ax) {
+  extern __shared__ double Us[];  // [nU]
+  for (int u = threadIdx.x; u < nU; u += 64) Us[u] = U_Val[u];
+  __syncthreads();
   const int r = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y;
-  if (r >= nR) return;
-  const int S = nR + 2, Mv = nU + 2;
+  if (r >= nR) return;  // no barriers below
+  const int Mv = nU + 2;
   const double a = acc[r];
   const double *fj = fsb + (int64_t)j * nU * nR + r;
-  const double *cj = c + j * cs + r;
+  const double *dj = c + j * cs + r;
   double *gc = g + j * cs + r;
   double w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // g of rows base..base+3
   int base = 0;
@@ -244,52 +247,48 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
     w3 = 0;
     base++;
   };
-  // one point ahead: the loads of point u+1 (its value, its target row and
-  // the coefficient rows there) fly while point u goes through the window
+  // kQ points ahead: the loads of point u+kQ (its value, its target row and
+  // the coefficient rows there) fly while points u..u+kQ-1 go through the
+  // window; the v nodes come from LDS
   auto load = [&](int u, double &lam, double &t, int &n, bool &ok, double (&gk)[4]) {
     const int uc = min(u, nU - 1);
     lam = fj[(int64_t)uc * nR];
-    const double newU = U_Val[uc] + a;
+    const double newU = Us[uc] + a;
     ok = u < nU && lam != 0.0 && !(newU <= Ulo) && !(newU >= Uhi);
     split(ok ? (newU - start_v) * hinv_v : 0.0, n, t);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const double *rc = cj + (int64_t)min(n + k, nU + 1) * S;
-      gk[k] = (rc[0] + rc[1] * 4) + rc[2];
-    }
+    for (int k = 0; k < 4; k++) gk[k] = dj[(int64_t)min(n + k, nU + 1) * nR];
   };
-  double lam, t, gk[4];
-  int n;
-  bool ok;
-  load(0, lam, t, n, ok, gk);
-  for (int u = 0; u < nU; u++) {
-    double lamN, tN, gkN[4];
-    int nN;
-    bool okN;
-    load(u + 1, lamN, tN, nN, okN, gkN);
-    if (ok) {
-      // acceleration term over the rows n..n+3 inside the padded column
-      const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
-      double sk = 0.0;
+  constexpr int kQ = 4;
+  double lamQ[kQ], tQ[kQ], gkQ[kQ][4];
+  int nQ[kQ];
+  bool okQ[kQ];
 #pragma unroll
-      for (int k = 0; k < 4; k++) sk += n + k <= nU + 1 ? dw[k] * gk[k] : 0.0;
-      sAcc += lam * sk;
-      while (base < n) finish();
-      const double l6 = lam * (1.0 / 6.0);
-      w0 += l6 * BV2(1. - t);
-      w1 += l6 * BV1(1. - t);
-      w2 += l6 * BV1(t);
-      w3 += l6 * BV2(t);
+  for (int q = 0; q < kQ; q++) load(q, lamQ[q], tQ[q], nQ[q], okQ[q], gkQ[q]);
+  for (int u0 = 0; u0 < nU; u0 += kQ) {
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      if (okQ[q]) {  // (false past the last point)
+        const double lam = lamQ[q], t = tQ[q];
+        const int n = nQ[q];
+        // acceleration term over the rows n..n+3 inside the padded column
+        const double dw[4] = {-dBV2(1. - t), -dBV1(1. - t), dBV1(t), dBV2(t)};
+        double sk = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) sk += n + k <= nU + 1 ? dw[k] * gkQ[q][k] : 0.0;
+        sAcc += lam * sk;
+        while (base < n) finish();
+        const double l6 = lam * (1.0 / 6.0);
+        w0 += l6 * BV2(1. - t);
+        w1 += l6 * BV1(1. - t);
+        w2 += l6 * BV1(t);
+        w3 += l6 * BV2(t);
+      }
+      load(u0 + q + kQ, lamQ[q], tQ[q], nQ[q], okQ[q], gkQ[q]);
     }
-    lam = lamN;
-    t = tN;
-    n = nN;
-    ok = okN;
-#pragma unroll
-    for (int k = 0; k < 4; k++) gk[k] = gkN[k];
   }
   while (base < Mv) finish();
-  abp[(int64_t)j * nR + r] = sAcc * (hinv_v / 36.0);
+  abp[(int64_t)j * nR + r] = sAcc * (hinv_v / 6.0);
 }
 
 // The descending sweep of the kick's transposed v-solve over the rows the
@@ -527,8 +526,8 @@ k_adj_chains(const double *__restrict__ fb, double *__restrict__ cb,
   if (!bout[k] && xb != 0.0) eval_c_adj(cb + (pt / nSlice) * cs, st, bxr[k], bxu[k], xb);
 }
 
-// kick^T, acceleration part: abp[j][r] = hinv/36 sum_u fb*[j][u][r]
-// sum_k B_k'(t) g_{n+k}, g the column combination of the kick's coefficients.
+// kick^T, acceleration part: abp[j][r] = hinv/6 sum_u fb*[j][u][r]
+// sum_k B_k'(t) d_{n+k}, d the kept v-spline coefficients of column r.
 // A block takes 64 columns; its kRG waves take every kRG-th row u and add up
 // their rows, and the kRG partial sums are combined through LDS.
 constexpr int kRG = 16;
@@ -541,12 +540,11 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
   const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int r = blockIdx.x * 64 + lane;
   const int j = blockIdx.y;
-  const int S = nR + 2;
   double s = 0.0;
   if (r < nR) {
     const double a = acc[r];
     const double *fj = fsb + (int64_t)j * nU * nR + r;
-    const double *cj = c + j * cs + r;
+    const double *dj = c + j * cs + r;
     // straight-line body (points outside the v range read row 0 and count
     // 0), two rows per trip so that their loads overlap
 #pragma unroll 2
@@ -562,9 +560,8 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         // past the padded column the weight is 0: read the last row instead
-        const double *rowc = cj + (int64_t)min(n + k, nU + 1) * S;
-        const double gk = (rowc[0] + rowc[1] * 4) + rowc[2];
-        sk += n + k <= nU + 1 ? dw[k] * gk : 0.0;
+        const double dk = dj[(int64_t)min(n + k, nU + 1) * nR];
+        sk += n + k <= nU + 1 ? dw[k] * dk : 0.0;
       }
       s += in ? lam * sk : 0.0;
     }
@@ -575,18 +572,15 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
     double t = part[0][lane];
 #pragma unroll
     for (int q = 1; q < kRG; q++) t += part[q][lane];
-    abp[(int64_t)j * nR + r] = t * (hinv_v / 36.0);
+    abp[(int64_t)j * nR + r] = t * (hinv_v / 6.0);
   }
 }
 
-// kick^T, coefficient part.  The kick reads the 2D coefficients c only
-// through the column combination Q c = (c[.][r] + 4 c[.][r+1] + c[.][r+2]) / 6,
-// which is exactly the interior rows of the r-direction spline system, so
-// Q R^-1 = identity at the nodes and, the r and v solves commuting, Q c =
-// V^-1 f: the kick is the 1D natural v-spline of each column of f, evaluated
-// with the weights B_k(t) / 6 (the forward computes it through the 2D
-// spline, as the reference does; the two agree to rounding).  Its transpose
-// therefore needs no r-direction adjoint: g(m, r) = sum over the points u of
+// kick^T, coefficient part.  The forward kick is the 1D natural v-spline of
+// each column of f, d = V^-1 f, evaluated with the weights B_k(t) / 6 (the
+// column form of the reference's 2D evaluation on r nodes; see
+// k_kick_line).  Its transpose therefore needs no r-direction adjoint:
+// g(m, r) = sum over the points u of
 // column r whose rows n..n+3 include m of fb*[u] B_{m-n}(t) / 6, then the
 // transposed v-solve of each column (k_adj_gcol).
 //
@@ -861,7 +855,8 @@ void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
                      double *abp, hipStream_t s) {
   if (!adj_shape(r)) {  // long columns: the walk (its ascending sweep included)
-    hipLaunchKernelGGL(k_adj_kick_walk, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, fsb, cK,
+    hipLaunchKernelGGL(k_adj_kick_walk, dim3((r.nR + 63) / 64, r.nJ), dim3(64),
+                       r.nU * sizeof(double), s, fsb, cK,
                        r.U_Val, accK, cb, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
                        r.tv);
     return;

@@ -151,17 +151,12 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
 }
 
 // One step that also keeps what the adjoint of the step needs: the kick's
-// acceleration (accOut, [nR]) and spline coefficients (cK, a padded
-// coefficient array).  The kick runs unfused (k_spline / k_kick, the same
-// values as the fused kernel, bit for bit).
+// acceleration (accOut, [nR]) and the v-spline coefficients of the columns
+// of f (cK, [j][m][r] with slice stride cs), which the kick leaves there.
 void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut) {
   vls::launch_moment(r, s);
   d2d(accOut, r.acc, r.nR, s);
-  double *keep = r.c;
-  r.c = cK;
-  vls::launch_spline(r, r.f, s, nullptr);
-  vls::launch_kick(r, s);
-  r.c = keep;
+  vls::launch_spline_kick(r, s, nullptr, false, cK);
   vls::launch_spline(r, r.f, s, nullptr);
   vls::launch_drift(r, s);
 }
@@ -844,6 +839,33 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
       A.capSeg = seg;
     }
+    // Trailing segments whose kick coefficients fit in 3/4 of the free HBM
+    // (counting the buffer this run already holds for them) are kept by the
+    // recorded run itself, so that the reverse sweep need not recompute them
+    // (all of them for grids like 129x129x64; VLS_ADJ_KEEP_STEPS caps the
+    // kept steps).
+    {
+      size_t freeB = 0, totalB = 0;
+      VLS_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+      const size_t stepB = (coef + (size_t)r.nR) * sizeof(double);
+      int64_t cap = (int64_t)((freeB + (size_t)A.capKeep * stepB) / 4 * 3 / stepB);
+      if (const char *e = getenv("VLS_ADJ_KEEP_STEPS")) cap = std::min<int64_t>(cap, atoll(e));
+      int keepSegs = 0;
+      while (keepSegs < nCk) {
+        const int from = (nCk - keepSegs - 1) * ckEvery;
+        if (nSteps - from > cap) break;
+        keepSegs++;
+      }
+      A.keptFrom = (nCk - keepSegs) * ckEvery;
+      const int kept = nSteps - A.keptFrom;
+      if (kept > A.capKeep) {
+        dfree(r, A.keepC);
+        dfree(r, A.keepAcc);
+        A.keepC = dalloc<double>(r, coef * kept);
+        A.keepAcc = dalloc<double>(r, (size_t)r.nR * kept);
+        A.capKeep = kept;
+      }
+    }
     if (!A.tilesValid) {
       std::vector<int64_t> off;
       std::vector<uint32_t> pts;
@@ -868,7 +890,11 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
         d2d(A.ckVals + nc * k, r.ch_vals, nc, s);
         d2d(A.ckHead + nch * k, r.ch_head, nch, s);
       }
-      step_once(r, s, nullptr);
+      if (n >= A.keptFrom)
+        step_keep(r, s, A.keepC + coef * (n - A.keptFrom),
+                  A.keepAcc + (size_t)r.nR * (n - A.keptFrom));
+      else
+        step_once(r, s, nullptr);
     }
     d2d(A.endF, r.f, nv, s);
     d2d(A.endVals, r.ch_vals, nc, s);
@@ -904,16 +930,21 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
     vls::launch_fill(A.ringb, (int64_t)nc, 0.0, s);
     for (int sg = A.nCk - 1; sg >= 0; sg--) {
       const int n0 = sg * A.ckEvery, n1 = std::min(n0 + A.ckEvery, A.nSteps);
-      // recompute the segment from its checkpoint, keeping each step's kick
-      // coefficients and acceleration
-      d2d(r.f, A.ckF + nv * sg, nv, s);
-      d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
-      d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
-      for (int n = n0; n < n1; n++)
-        step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0));
+      const bool kept = n0 >= A.keptFrom;
+      if (!kept) {
+        // recompute the segment from its checkpoint, keeping each step's kick
+        // coefficients and acceleration
+        d2d(r.f, A.ckF + nv * sg, nv, s);
+        d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
+        d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
+        for (int n = n0; n < n1; n++)
+          step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0));
+      }
       // reverse through the segment
       for (int n = n1 - 1; n >= n0; n--) {
-        const double *cK = A.segC + coef * (n - n0), *accK = A.segAcc + (size_t)r.nR * (n - n0);
+        const double *cK = kept ? A.keepC + coef * (n - A.keptFrom) : A.segC + coef * (n - n0);
+        const double *accK = kept ? A.keepAcc + (size_t)r.nR * (n - A.keptFrom)
+                                  : A.segAcc + (size_t)r.nR * (n - n0);
         vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
         vls::launch_adj_spline(r, A.cb, A.fsb, s);
         vls::launch_adj_kick(r, A.fsb, cK, accK, A.cb, A.abp, s);

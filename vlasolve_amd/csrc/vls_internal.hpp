@@ -99,6 +99,10 @@ struct Run {
     int32_t *endHead = nullptr;
     double *segC = nullptr;     // [capSeg][coef] the kick coefficients of each state
     double *segAcc = nullptr;   // [capSeg][nR] the kick accelerations
+    // the trailing segments kept by the recorded run itself (no recomputation
+    // in the reverse sweep): steps keptFrom..nSteps-1
+    int keptFrom = 0, capKeep = 0;
+    double *keepC = nullptr, *keepAcc = nullptr;
     double *fb = nullptr, *fsb = nullptr, *cb = nullptr;
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
     int64_t *tileOff = nullptr;  // transposed foot table of the drift, per coefficient tile
@@ -134,10 +138,14 @@ void configure_tiles(Run &r, int maxLdsPerBlock);
 void launch_moment(Run &r, hipStream_t s);       // partials of f, then the scan
 void launch_moment_scan(Run &r, hipStream_t s);  // scan of the partials in P
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
-void launch_kick(Run &r, hipStream_t s);
-// (moment,) kick spline, kick of the plane r.f; without the moment the
-// acceleration of the last launch_moment is used
-void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment = true);
+// the kick of the plane r.f in column form: v-spline of each column of f
+// into d (slice stride cs, row stride nR), then the evaluation
+void launch_kick(Run &r, hipStream_t s, double *d);
+// (moment,) kick of the plane r.f; without the moment the acceleration of
+// the last launch_moment is used; with dKeep the kick's column coefficients
+// are also left there ([j][m][r], slice stride cs)
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment = true,
+                        double *dKeep = nullptr);
 size_t kick_scratch_doubles(const Run &r);
 void launch_drift(Run &r, hipStream_t s);
 void mark_chains(Run &r, hipStream_t s);  // attach the delay lines to their points

@@ -86,6 +86,71 @@ k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
   P[(size_t)j * nR + r] = Pj;
 }
 
+// The same partials with a lane per column r of a slice (63 columns per wave;
+// column r + 1 arrives by a lane shift), walking the rows u in order with
+// two groups of 16 rows of loads in flight, the sums in the same tree.
+__device__ __forceinline__ double lane_next(double x) {  // lane i gets lane i+1's x
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+__global__ void __launch_bounds__(64)
+k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
+              const double *__restrict__ dU, const double *__restrict__ dR,
+              double *__restrict__ P, int nR, int nU) {
+  constexpr int kG = 16;
+  extern __shared__ double dUs[];  // [nU]
+  const int lane = threadIdx.x;
+  const int j = blockIdx.y;
+  const int r = blockIdx.x * 63 + lane;
+  const bool term = lane < 63 && r < nR - 1;
+  for (int u = lane; u < nU; u += 64) dUs[u] = dU[u];
+  __syncthreads();
+  const double *col = f + (size_t)j * nU * nR + min(r, nR - 1);
+  const double ct = Ct1[j], dr = term ? dR[r] : 0.0;
+  const int nG = (nU + kG - 1) / kG;         // groups of rows with data
+  const int nGt = 4 * ((nU + 63) / 64);      // groups of the tree (4 per 64 rows)
+  double cur[kG], nxt[kG];
+  auto load = [&](int g, double *b) {
+#pragma unroll
+    for (int t = 0; t < kG; t++) b[t] = col[(size_t)min(g * kG + t, nU - 1) * nR];
+  };
+  load(0, cur);
+  load(1, nxt);
+  double aOwn = cur[0], aN = lane_next(aOwn);
+  double Pj = 0.0;
+  for (int w = 0; w < nGt / 4; w++) {
+    double G[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int g = 4 * w + q;
+      double acc = 0.0;
+      if (g < nG) {
+#pragma unroll
+        for (int t = 0; t < kG; t++) {
+          const int u = g * kG + t;
+          const double b = t + 1 < kG ? cur[t + 1] : nxt[0];  // row u + 1
+          const double bN = lane_next(b);
+          if (u < nU - 1) {
+            const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+            acc += ((ct * dUs[u]) * dr) * avg;
+          }
+          aOwn = b;
+          aN = bN;
+        }
+#pragma unroll
+        for (int t = 0; t < kG; t++) cur[t] = nxt[t];
+        if (g + 2 < nG) load(g + 2, nxt);
+      }
+      G[q] = acc;
+    }
+    const double W = ((G[0] + G[1]) + G[2]) + G[3];
+    Pj = w == 0 ? W : Pj + W;
+  }
+  if (lane < 63 && r < nR) P[(size_t)j * nR + r] = term ? Pj : 0.0;
+}
+
 // k_moment_partial over the sum of nS species planes (nVal apart): the
 // integrand of int21::f(0, nS), 0 + avg_0 + avg_1 + ... per cell
 // (src/integrand21.hxx:21-39), in the same tree order.
@@ -563,18 +628,20 @@ k_spline_r_stream(const double *__restrict__ f, double *__restrict__ c, int nR, 
 }
 
 // Whole-line solve in registers, for a line length M known at compile time.
-// Lane = line: all M-2 right-hand sides are loaded up front, the forward
-// elimination and the back substitution run on the register array y[] with
-// no memory traffic and no branches, and the M coefficients are stored as
-// they are produced.  Every line of the grid is in flight at once (one wave
-// per SIMD at ~2M VGPRs), which is what a latency-bound recurrence needs.
-// Rows 1..kHead use the pivot table (whose entries from K0 <= kHead+1 on
-// equal the fixed point), later rows the fixed point dS, rdS, uS.
-//   ROWS = true : the nU rows (j, u) of every slice; rhs_k = f[j][u][k-1],
-//                 c_k -> c[j][u+1][k]      (create_2D first loop)
-//   ROWS = false: the nR+2 = S padded columns (j, col); rhs_k = c[j][k][col],
+// Lane = line: the right-hand sides stream in ahead of the forward
+// elimination, the elimination and the back substitution run on the register
+// array y[] with no memory traffic and no branches, and the M coefficients
+// are stored as they are produced.  Every line of the grid is in flight at
+// once (one wave per SIMD at ~2M VGPRs), which is what a latency-bound
+// recurrence needs.  Rows 1..kHead use the pivot table (whose entries from
+// K0 <= kHead+1 on equal the fixed point), later rows the fixed point dS,
+// rdS, uS.
+//   KICK = false: the nR+2 = S padded columns (j, col); rhs_k = c[j][k][col],
 //                 c_k -> c[j][k][col]      (create_2D second loop)
-template <int M, bool ROWS, int S>
+//   KICK = true : the nR columns of f (j, r); rhs_k = f[j][k-1][r],
+//                 d_k -> d[j][k][r] (slice stride cs, row stride nR): the
+//                 kick's column splines (see k_kick_line)
+template <int M, bool KICK, int S>
 __global__ void __launch_bounds__(64)
 k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, int nJ, int64_t cs,
        int lzmask, ThomasAxis ax) {
@@ -584,16 +651,14 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
   double *out;
   int64_t is, os;
   bool live;
-  if (ROWS) {
-    const int nLines = nJ * nU;
-    int L = blockIdx.x * 64 + lane;
-    live = L < nLines;
-    if (!live) L = nLines - 1;
-    const int j = L / nU, u = L - j * nU;
-    in = src + (size_t)L * nR - 1;  // in[k] = f[j][u][k-1]
-    out = c + (size_t)j * cs + (size_t)(u + 1) * M;
-    is = 1;
-    os = 1;
+  if (KICK) {
+    int col = blockIdx.x * 64 + lane;
+    live = col < nR;
+    if (!live) col = nR - 1;
+    in = src + (size_t)blockIdx.y * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
+    out = c + (size_t)blockIdx.y * cs + col;
+    is = S - 2;  // = nR, a compile-time stride
+    os = S - 2;
   } else {
     int col = blockIdx.x * 64 + lane;
     live = col < nR + 2;
@@ -640,36 +705,29 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 }
 
 // ---------------------------------------------------------------- fused kick
-// The v-direction solves of the kick's spline (create_2D second loop,
-// src/localSpline.hxx:1252-1254) fused with the kick itself
-// (src/vlasovSolver21_spline.hxx:750-765, evaluateNxPy
-// src/localSpline.hxx:423-441).  The kick evaluates the spline on r nodes
-// only: point (u, r) with target row n needs, for the four padded rows
-// m = n..n+3, the column combination g_m(r) = (c[m][r] + c[m][r+1]*4) +
-// c[m][r+2], in the reference's operation order.
+// The kick (src/vlasovSolver21_spline.hxx:750-765, evaluateNxPy
+// src/localSpline.hxx:423-441) evaluates the 2D spline on r nodes only.
+// There its r-direction combination (c[m][r] + 4 c[m][r+1] + c[m][r+2]) / 6
+// reproduces the r-solve's right-hand sides exactly, and the r and v solves
+// commute, so the combination is d_m(r), the natural v-spline coefficients
+// of column r of f, and evaluateNxPy's (1/36) sum_k B_k(t) (6 d_{n+k})
+// equals (1/6) sum_k B_k(t) d_{n+k}.  The kick is computed in that form: one
+// v-solve per column of f, no r-solve (it agrees with the reference's 2D
+// computation to rounding; the oracle's device order computes the same form).
 //
-// A wave takes 64 consecutive padded columns and writes the points of the
-// first kKW = 62 of them.  Lane = column: the column is solved in registers,
-// and the back substitution, which produces c_m from the last row down, forms
-// g_m with two lane shifts and keeps the last kRing rows of g in a ring in
-// LDS (each lane reads only its own column of it).  Point (u, r) is evaluated
-// at the step that produces row m = u + D, D being the smallest offset n - u
-// over the wave's points: its rows are ready by then, and they are still in
-// the ring if the offsets of the wave spread over at most kRing - 4 rows.  A
-// wave whose offsets spread wider keeps all of g in a global scratch area and
-// evaluates after the sweep.  Either way the kick's coefficients never reach
-// HBM.  The offsets come from the same expressions as the evaluation and are
-// taken during the forward elimination.
-constexpr int kKW = 62, kEB = 8;
-
-// Lane i takes lane i+1's value (lane 63 gets 0): a whole-wave DPP shift, a
-// VALU move with no trip through LDS, into a fresh register.
-__device__ __forceinline__ double wave_shl1(double x) {
-  int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
-  hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
-  return __hiloint2double(hi, lo);
-}
+// A wave takes 64 consecutive columns of a slice, in place on f.  Lane =
+// column: the column streams into registers during the forward elimination,
+// and the back substitution, which produces d_m from the last row down,
+// keeps the last kRing rows in a ring in LDS (each lane reads only its own
+// column of it).  Point (u, r) is evaluated at the step that produces row
+// m = u + D, D being the smallest offset n - u over the wave's points: its
+// rows are ready by then, and they are still in the ring if the offsets of
+// the wave spread over at most kRing - 4 rows.  A wave whose offsets spread
+// wider keeps all of d in a global scratch area and evaluates after the
+// sweep.  The offsets come from the same expressions as the evaluation and
+// are taken while the column arrives.  The column is read whole before any
+// of its points is written.
+constexpr int kKW = 64, kEB = 8;
 
 // Long columns keep their first kYL forward values in LDS instead of VGPRs
 // (they are the last ones the back substitution needs) and use a 16-row ring,
@@ -681,13 +739,12 @@ struct KickLineCfg {
   static constexpr int kYL = Mv > 160 ? 56 : 0;
 };
 
-template <int Mv, int S>
+template <int Mv, int S, bool KEEP>
 __global__ void __launch_bounds__(64)
-k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *__restrict__ U_Val,
-            const double *__restrict__ acc, double *__restrict__ gsc, int nR, int64_t cs,
-            double Ulo, double Uhi, double start_v, double hinv_v, int forceScratch, int lzmask,
-            ThomasAxis ax) {
-  constexpr int nU = Mv - 2;
+k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restrict__ acc,
+            double *__restrict__ gsc, double Ulo, double Uhi, double start_v, double hinv_v,
+            int forceScratch, int lzmask, ThomasAxis ax, double *__restrict__ dKeep, int64_t ds) {
+  constexpr int nU = Mv - 2, nR = S - 2;
   constexpr int kRing = KickLineCfg<Mv>::kRing, kYL = KickLineCfg<Mv>::kYL;
   // ring slot s holds row m with m % kRing == s; slots kRing..kRing+2 repeat
   // slots 0..2 so that the four rows n..n+3 of a point are read from slots
@@ -698,9 +755,10 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
   const int j = blockIdx.y;
-  const int k = blockIdx.x * kKW + lane;      // padded column solved by this lane
-  const bool outLane = lane < kKW && k < nR;  // writes the points f[j][.][k]
-  const double *col = c + (size_t)j * cs + (k < S ? k : S - 1);
+  const int k = blockIdx.x * kKW + lane;  // column of this lane
+  const bool outLane = k < nR;            // writes the points f[j][.][k]
+  double *fj = f + (size_t)j * nU * nR;
+  const double *col = fj + (k < nR ? k : nR - 1);
   for (int u = lane; u < nU; u += 64) Us[u] = U_Val[u];
   // the ring starts zeroed: a point whose target row is the last knot reads
   // the row past the padded column with weight BV2(0) = 0
@@ -708,14 +766,20 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   for (int s = 0; s < kRing + 3; s++) ring[s * 64 + lane] = 0.0;
   const double a = outLane ? acc[k] : 0.0;
   __syncthreads();
-  double *fo = f + (size_t)j * nU * nR + (outLane ? k : 0);
+  double *fo = fj + (outLane ? k : 0);
+  // KEEP (the adjoint's recorded steps): the coefficients d_m of the column
+  // are also stored, to dKeep[j][m][k] (slice stride ds)
+  double *dk = KEEP ? dKeep + (size_t)j * ds + (outLane ? k : 0) : nullptr;
+  auto keep = [&](int m, double d) {
+    if (KEEP && outLane) dk[(size_t)m * nR] = d;
+  };
 
-  // forward elimination (rows 1..Mv-2); the column streams in kLA rows ahead
-  // of the elimination, so that the elimination runs while it arrives
+  // forward elimination (rows 1..Mv-2, row m = point u = m - 1); the column
+  // streams in kLA rows ahead of the elimination, which runs while it arrives
   constexpr int kLA = Mv - 2 < 48 ? Mv - 2 : 48;
   double y[Mv];
 #pragma unroll
-  for (int m = 1; m <= kLA; m++) y[m] = col[(int64_t)m * S];
+  for (int m = 1; m <= kLA; m++) y[m] = col[(m - 1) * nR];
   // target-row offsets n - u of the lane's points (while the loads fly)
   int dlo = 1 << 30, dhi = -(1 << 30);
   if (outLane) {
@@ -733,7 +797,7 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   double prev = ax.A03;
 #pragma unroll
   for (int m = 1; m <= Mv - 2; m++) {
-    if (m + kLA <= Mv - 2) y[m + kLA] = col[(int64_t)(m + kLA) * S];
+    if (m + kLA <= Mv - 2) y[m + kLA] = col[(m + kLA - 1) * nR];
     double dv = dS, ev = rdS;
     if (m <= kHead) {
       dv = ax.d[m + lz];
@@ -763,37 +827,30 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
   double *gs = gsc + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (Mv + 1) * 64 + lane;
 
   auto ring_at = [&](int row) { return ring[(row & (kRing - 1)) * 64 + lane]; };
-  auto ring_put = [&](int m, double g) {  // m known at compile time
-    ring[(m & (kRing - 1)) * 64 + lane] = g;
-    if ((m & (kRing - 1)) < 3) ring[(kRing + (m & (kRing - 1))) * 64 + lane] = g;
+  auto ring_put = [&](int m, double d) {  // m known at compile time
+    ring[(m & (kRing - 1)) * 64 + lane] = d;
+    if ((m & (kRing - 1)) < 3) ring[(kRing + (m & (kRing - 1))) * 64 + lane] = d;
   };
   auto scratch_at = [&](int row) { return gs[(size_t)row * 64]; };
   // point u of this lane's column (u wave-uniform)
-  auto eval = [&](int u, auto gload) {
+  auto eval = [&](int u, auto dload) {
     const double newU = Us[u] + a;
     double val = 0;
     if (!(newU <= Ulo) && !(newU >= Uhi)) {
       double yy = (newU - start_v) * hinv_v;
       int n;
       split(yy, n, yy);
-      const double g0 = gload(n), g1 = gload(n + 1), g2 = gload(n + 2), g3 = gload(n + 3);
-      double result = 1.0 / 36.0;
-      result *= BV2(1. - yy) * g0 + BV1(1. - yy) * g1 + BV1(yy) * g2 + BV2(yy) * g3;
+      const double d0 = dload(n), d1 = dload(n + 1), d2 = dload(n + 2), d3 = dload(n + 3);
+      double result = 1.0 / 6.0;
+      result *= BV2(1. - yy) * d0 + BV1(1. - yy) * d1 + BV1(yy) * d2 + BV2(yy) * d3;
       val = result;
     }
     fo[(size_t)u * nR] = val;
-  };
-  // g_m from the lane's c_m and its two right neighbours (lanes 62 and 63,
-  // which write no points, combine values that are not theirs)
-  auto gcomb = [&](double cm) {
-    const double c1 = wave_shl1(cm), c2 = wave_shl1(c1);
-    return (cm + c1 * 4) + c2;
   };
   // The points whose steps are rows m0..m0+kEB-1, once row m0 is in: straight
   // code, so that the kEB evaluations overlap each other and the sweep.  The
   // point rows u = m0 + i - D are wave-uniform: their U values and output
   // rows are addressed from scalar registers.
-  double *fj = f + (size_t)j * nU * nR;
   const unsigned kcol = outLane ? (unsigned)k : 0u;
   // A block is evaluated in two halves: prep_block, one block ahead, takes
   // the points' target rows and basis weights, which do not depend on the
@@ -838,7 +895,7 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < kEB; i++) {
-      double result = 1.0 / 36.0;
+      double result = 1.0 / 6.0;
       result *= pp[i].w0 * g[i][0] + pp[i].w1 * g[i][1] + pp[i].w2 * g[i][2] + pp[i].w3 * g[i][3];
       val[i] = pp[i].in ? result : 0.0;
     }
@@ -875,7 +932,8 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
     // u = m0 - D .. m0 - D + kEB - 1
     Prep pp[kEB];
     if (Mv - kEB >= 1) prep_block(Mv - kEB, pp);
-    ring_put(Mv - 1, gcomb(cn));
+    ring_put(Mv - 1, cn);
+    keep(Mv - 1, cn);
 #pragma unroll
     for (int m = Mv - 2; m >= 1; m--) {
       double uv = uS;
@@ -883,13 +941,16 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
       const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
       c2 = cn;
       cn = ym - uv * cn;
-      ring_put(m, gcomb(cn));
+      ring_put(m, cn);
+      keep(m, cn);
       if ((Mv - 1 - m) % kEB == kEB - 1) {
         finish_block(m, pp);
         if (m - kEB >= 1) prep_block(m - kEB, pp);
       }
     }
-    ring_put(0, gcomb((ax.A03 - ax.A01 * cn) - ax.A02 * c2));
+    const double d0 = (ax.A03 - ax.A01 * cn) - ax.A02 * c2;
+    ring_put(0, d0);
+    keep(0, d0);
     // rows 0..top not covered by a block, and the points whose step would
     // come after row 0 (D < 0); the ring then holds rows 0..kRing-1
     if (outLane) {
@@ -897,9 +958,10 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
       for (int u = min(top - D, nU - 1); u >= 0; u--) eval(u, ring_at);
     }
   } else {
-    // all of g to the scratch area, then every point
+    // all of d to the scratch area, then every point
     double cn = cLast, c2 = 0.0;
-    gs[(size_t)(Mv - 1) * 64] = gcomb(cn);
+    gs[(size_t)(Mv - 1) * 64] = cn;
+    keep(Mv - 1, cn);
 #pragma unroll
     for (int m = Mv - 2; m >= 1; m--) {
       double uv = uS;
@@ -907,9 +969,11 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
       const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
       c2 = cn;
       cn = ym - uv * cn;
-      gs[(size_t)m * 64] = gcomb(cn);
+      gs[(size_t)m * 64] = cn;
+      keep(m, cn);
     }
-    gs[0] = gcomb((ax.A03 - ax.A01 * cn) - ax.A02 * c2);
+    gs[0] = (ax.A03 - ax.A01 * cn) - ax.A02 * c2;
+    keep(0, gs[0]);
     gs[(size_t)Mv * 64] = 0.0;  // the row past the padded column, read with weight 0
     if (outLane)
       for (int u = 0; u < nU; u++) eval(u, scratch_at);
@@ -917,19 +981,42 @@ k_kick_line(const double *__restrict__ c, double *__restrict__ f, const double *
 }
 
 // ---------------------------------------------------------------- kick
-// vlasovSolver21_spline.hxx:750-765 with localSpline::evaluateNxPy
-// (src/localSpline.hxx:423-441).
+// The kick for shapes without register lines, in the column form of the
+// fused kick (k_kick_line): k_kick_vsolve solves the natural v-spline of
+// every column of f into d ([j][m][r], slice stride ds, row stride nR; the
+// forward values take a trip through d, as in k_spline_v_stream), then
+// k_kick_cols evaluates each point as (1/6) sum_k B_k(t) d[n+k][r]
+// (src/vlasovSolver21_spline.hxx:750-765, evaluateNxPy
+// src/localSpline.hxx:423-441, in the column form).
+__global__ void __launch_bounds__(64)
+k_kick_vsolve(const double *__restrict__ f, double *__restrict__ d, int nR, int nU, int64_t ds,
+              int lzmask, ThomasAxis ax) {
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int col = blockIdx.x * 64 + lane;
+  if (col >= nR) return;  // no barriers below
+  const double *fc = f + (size_t)blockIdx.y * nU * nR + col;
+  double *dc = d + (size_t)blockIdx.y * ds + col;
+  double ym2, ym1;
+  thomas_forward<16>(
+      ax, lz, [&](int k) { return fc[(int64_t)(k - 1) * nR]; },
+      [&](int k, double y) { dc[(int64_t)k * nR] = y; }, ym2, ym1);
+  thomas_backward<16>(
+      ax, lz, ym2, ym1, [&](int k) { return dc[(int64_t)k * nR]; },
+      [&](int k, double v) { dc[(int64_t)k * nR] = v; });
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_kick(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ U_Val,
-       const double *__restrict__ acc, int nR, int nU, int nJ, int64_t cs, double Ulo, double Uhi,
-       double start_v, double hinv_v) {
+k_kick_cols(double *__restrict__ f, const double *__restrict__ d, const double *__restrict__ U_Val,
+            const double *__restrict__ acc, int nR, int nU, int64_t ds, double Ulo, double Uhi,
+            double start_v, double hinv_v) {
   const int il = blockIdx.x * blockDim.x + threadIdx.x;  // index inside the slice
   if (il >= nU * nR) return;
   const int j = blockIdx.y;
   const int u = il / nR;
   const int r = il - u * nR;
   const size_t i = (size_t)j * nU * nR + il;
-  double newU = U_Val[u] + acc[r];
+  const double newU = U_Val[u] + acc[r];
   if ((newU <= Ulo) || (newU >= Uhi)) {
     f[i] = 0;
     return;
@@ -937,16 +1024,13 @@ k_kick(double *__restrict__ f, const double *__restrict__ c, const double *__res
   double y = (newU - start_v) * hinv_v;
   int n;
   split(y, n, y);
-  const int Ny = 1 + n, Nx = r + 1;
-  const int st = nR + 2;
-  const double *d = c + (size_t)j * cs;
-  const double *r0 = d + Nx + (Ny - 1) * st;
-  const double *r1 = r0 + st;
-  const double *r2 = r1 + st;
-  const double *r3 = r2 + st;
-  double result = 1.0 / 36.0;
-  result *= BV2(1. - y) * (r0[-1] + r0[0] * 4 + r0[1]) + BV1(1. - y) * (r1[-1] + r1[0] * 4 + r1[1]) +
-            BV1(y) * (r2[-1] + r2[0] * 4 + r2[1]) + BV2(y) * (r3[-1] + r3[0] * 4 + r3[1]);
+  const double *dc = d + (size_t)j * ds + r;
+  // row n+3 is past the column (nU+2) only at y == 0, where its weight
+  // BV2(0) is 0: the last row is read instead
+  const double d0 = dc[(int64_t)n * nR], d1 = dc[(int64_t)(n + 1) * nR];
+  const double d2 = dc[(int64_t)(n + 2) * nR], d3 = dc[(int64_t)min(n + 3, nU + 1) * nR];
+  double result = 1.0 / 6.0;
+  result *= BV2(1. - y) * d0 + BV1(1. - y) * d1 + BV1(y) * d2 + BV2(y) * d3;
   f[i] = result;
 }
 
@@ -1214,8 +1298,8 @@ void launch_moment(Run &r, hipStream_t s) {
     hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P,
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
   else
-    hipLaunchKernelGGL(k_moment_partial, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR,
-                       r.nU, r.nJ);
+    hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63, r.nJ), dim3(64),
+                       r.nU * sizeof(double), s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR, r.nU);
   launch_moment_scan(r, s);
 }
 
@@ -1502,17 +1586,18 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
 using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis,
                         const double *, const double *, const double *, double *);
-using KickFn = void (*)(const double *, double *, const double *, const double *, double *, int,
-                        int64_t, double, double, double, double, int, int, ThomasAxis);
+using KickFn = void (*)(double *, const double *, const double *, double *, double, double, double,
+                        double, int, int, ThomasAxis, double *, int64_t);
 struct LineShape {
   int nR, nU;
-  RowsFn rows, rowsMoment;
-  LineFn cols;
-  KickFn kick;
+  RowsFn rows;
+  LineFn cols, kickCols;
+  KickFn kick, kickKeep;
 };
 #define VLS_LINE_SHAPE(R, U)                                                          \
-  {R, U, k_line_rows<(R) + 2, false>, k_line_rows<(R) + 2, true>,                     \
-   k_line<(U) + 2, false, (R) + 2>, k_kick_line<(U) + 2, (R) + 2>}
+  {R, U, k_line_rows<(R) + 2, false>, k_line<(U) + 2, false, (R) + 2>,               \
+   k_line<(U) + 2, true, (R) + 2>, k_kick_line<(U) + 2, (R) + 2, false>,             \
+   k_kick_line<(U) + 2, (R) + 2, true>}
 static const LineShape kLineShapes[] = {
     VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
     VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
@@ -1559,8 +1644,14 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
 
 // Doubles of global scratch the fused kick needs (0: no fused kick for this
 // shape): one (nU+3) x 64 area per wave.
+// The fused kick evaluates each column's points one after the other, so it
+// pays only when the grid has enough columns to fill the GPU with waves
+// (about 40k columns at 201x201x200); smaller grids run the column solves and
+// the evaluation as two kernels, the evaluation a thread per point.
 static bool fused_kick(const Run &r) {
-  return line_shape(r) != nullptr && !getenv("VLS_NO_FUSED_KICK");
+  const int64_t waves = (int64_t)r.nJ * ((r.nR + kKW - 1) / kKW);
+  if (getenv("VLS_FUSED_KICK")) return line_shape(r) != nullptr;
+  return line_shape(r) != nullptr && waves >= 512 && !getenv("VLS_NO_FUSED_KICK");
 }
 
 size_t kick_scratch_doubles(const Run &r) {
@@ -1569,47 +1660,33 @@ size_t kick_scratch_doubles(const Run &r) {
   return tiles * r.nJ * (size_t)(r.nU + 3) * 64;
 }
 
-void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment) {
-  if (fused_kick(r) && r.gsc) {
-    const LineShape *ls = line_shape(r);
-    const int nW = (r.nU + 63) / 64;
-    if (!moment) {
-      hipLaunchKernelGGL(ls->rows, dim3(nW, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR, r.nU, r.nJ,
-                         r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
-      time_mark(t, s, "spline_r");
-    } else if (nW <= kRowsMaxW) {
-      // the row solves also form the moment partials of f
-      hipLaunchKernelGGL(ls->rowsMoment, dim3(1, r.nJ), dim3(64 * nW), 0, s, r.f, r.c, r.nR,
-                         r.nU, r.nJ, r.cs, 0, r.tr, r.Ct1, r.dU, r.dR, r.P);
-      time_mark(t, s, "spline_r_moment");
-      launch_moment_scan(r, s);
-      time_mark(t, s, "moment_scan");
-    } else {
-      launch_moment(r, s);
-      time_mark(t, s, "moment");
-      hipLaunchKernelGGL(ls->rows, dim3(nW, r.nJ), dim3(64), 0, s, r.f, r.c, r.nR, r.nU, r.nJ,
-                         r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
-      time_mark(t, s, "spline_r");
-    }
-    hipLaunchKernelGGL(ls->kick, dim3((r.nR + kKW - 1) / kKW, r.nJ), dim3(64), 0, s, r.c, r.f,
-                       r.U_Val, r.acc, r.gsc, r.nR, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
-                       r.kickScratch ? 1 : 0, 0, r.tv);
-    time_mark(t, s, "kick_line");
-    return;
-  }
+void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *dKeep) {
   if (moment) {
     launch_moment(r, s);
     time_mark(t, s, "moment");
   }
-  launch_spline(r, r.f, s, t);
-  launch_kick(r, s);
+  if (fused_kick(r) && r.gsc) {
+    const LineShape *ls = line_shape(r);
+    hipLaunchKernelGGL(dKeep ? ls->kickKeep : ls->kick, dim3((r.nR + kKW - 1) / kKW, r.nJ),
+                       dim3(64), 0, s, r.f, r.U_Val, r.acc, r.gsc, r.Ulo, r.Uhi, r.start_v,
+                       r.hinv_v, r.kickScratch ? 1 : 0, 0, r.tv, dKeep, r.cs);
+    time_mark(t, s, "kick_line");
+    return;
+  }
+  launch_kick(r, s, dKeep ? dKeep : r.c);
   time_mark(t, s, "kick");
 }
 
-void launch_kick(Run &r, hipStream_t s) {
+void launch_kick(Run &r, hipStream_t s, double *d) {
+  if (const LineShape *ls = line_shape(r))
+    hipLaunchKernelGGL(ls->kickCols, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, r.f, d, r.nR,
+                       r.nU, r.nJ, r.cs, 0, r.tv);
+  else
+    hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, r.f, d, r.nR,
+                       r.nU, r.cs, 0, r.tv);
   dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
-  hipLaunchKernelGGL(k_kick, g, dim3(kBlock), 0, s, r.f, r.c, r.U_Val, r.acc, r.nR, r.nU, r.nJ,
-                     r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  hipLaunchKernelGGL(k_kick_cols, g, dim3(kBlock), 0, s, r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs,
+                     r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
 void launch_drift(Run &r, hipStream_t s) {
