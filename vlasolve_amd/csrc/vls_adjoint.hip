@@ -146,6 +146,10 @@ __device__ __forceinline__ double moment_adj(int u, int r, int nU, int nR, doubl
 // of the line has to fit on chip.
 constexpr int kLC = 16;
 
+// Pivot table rows read from the table: past kHead the pivots sit at their
+// fixed point (uS, rdS), which every line adjoint uses from a register.
+constexpr int kHead = 16;
+
 // one element of the descending sweep: element k (1..M-2) of the line after
 // the ascending sweep, the carry from element k+1 in and out
 __device__ __forceinline__ double adj_down(const ThomasAxis &ax, int k, double v, double tt,
@@ -153,7 +157,7 @@ __device__ __forceinline__ double adj_down(const ThomasAxis &ax, int k, double v
   v -= carry;
   if (k == ax.M - 2) v = fma(-ax.p, tt, v);
   if (k == ax.M - 3) v = fma(-ax.hi2, tt, v);
-  v *= ax.rd[k];
+  v *= k <= kHead ? ax.rd[k] : ax.rdS;  // the table is the fixed point past kHead
   carry = ax.c16 * v;
   return v;
 }
@@ -178,7 +182,7 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
         if (t == 0) c0 = v[i];
         if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
         if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? ax.u[t - 1] : ax.uS), prev, v[i]);
         prev = v[i];
       }
     }
@@ -238,7 +242,7 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
     if (m == 0) c0 = v;
     if (m == 1) v = fma(-ax.A01, c0, v);
     if (m == 2) v = fma(-ax.A02, c0, v);
-    if (m >= 2) v = fma(-ax.u[m - 1], prev, v);
+    if (m >= 2) v = fma(-(m - 1 <= kHead ? ax.u[m - 1] : ax.uS), prev, v);
     prev = v;
     gc[(int64_t)m * nR] = v;
     w0 = w1;
@@ -377,7 +381,7 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
         if (t == 0) c0 = v[i];
         if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
         if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-        if (t >= 2) v[i] = fma(-ax.u[t - 1], prev, v[i]);
+        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? ax.u[t - 1] : ax.uS), prev, v[i]);
         prev = v[i];
       }
     }
@@ -633,7 +637,6 @@ k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val
 // point (the table repeats uS and rdS there; the set-up checks K0 - 1 <=
 // kHead), so only the head is read from the table.  The adjoint is not held
 // to the forward's operation order, so its updates are fused multiply-adds.
-constexpr int kHead = 16;
 template <int M>
 __device__ __forceinline__ void thomas_adjoint_reg(const ThomasAxis &ax, double (&c)[M], int lz) {
   const double uS = ax.uS + (double)lz, rdS = ax.rdS + (double)lz;
