@@ -51,13 +51,13 @@ def algorithmic_bytes(name, nR, nU, nJ):
     n = nR * nU * nJ
     coef = (nR + 2) * (nU + 2) * nJ
     rows = (nR + 2) * nU * nJ
+    cols = nR * (nU + 2) * nJ            # the kick's column splines
     table = {
         "moment": 8 * n,                 # read f
         "spline_r": 8 * n + 8 * rows,    # read f, write the r-solved rows
-        "spline_r_moment": 8 * n + 8 * rows,  # the same, with the moment partials of f
         "spline_v": 8 * rows + 8 * coef,  # read them, write all coefficients
-        "kick": 8 * coef + 8 * n,        # read coefficients, write f
-        "kick_line": 8 * rows + 8 * n,   # fused v-solve + kick: read the r-solved rows, write f
+        "kick": 16 * n + 16 * cols,      # column splines of f, then the kick: f -> d -> f
+        "kick_line": 8 * n + 8 * n,      # fused column splines + kick: read f, write f
         "drift": 8 * coef + 8 * n,       # read coefficients, write f
     }
     return table.get(name)

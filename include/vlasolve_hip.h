@@ -190,17 +190,20 @@ int vls_mass_profile(vls_ctx *ctx, int run, double *M);
  * f and the delay-line values of the first recorded step, given dL_dM
  * ([nR], the loss's derivative in the mass profile of f_N as returned by
  * vls_mass_profile; may be NULL) and dL_df ([nJ][nU][nR], its direct
- * derivative in f_N; may be NULL).  The states between checkpoints are
- * recomputed segment by segment; the context ends in the state after the
- * recorded run.  grad_vals0 (may be NULL) has the ring layout of
+ * derivative in f_N; may be NULL).  The recorded run also keeps, for every
+ * step of the trailing checkpoint segments that fit in three quarters of the
+ * free HBM (all of them for grids like 129x129x64; the environment variable
+ * VLS_ADJ_KEEP_STEPS caps the kept steps), the kick's column splines and
+ * acceleration; vls_adjoint recomputes the other segments from their
+ * checkpoints.  The context ends in the state after the recorded run.  grad_vals0 (may be NULL) has the ring layout of
  * vls_set_chains at the first recorded step.  The drift's transpose adds
  * the points' contributions into tiles of coefficients with LDS atomics and
  * the delay lines' back cells add theirs with global floating-point atomics,
  * so the last bits of the gradient may vary from call to call.  vls_record
  * builds, on first use after vls_set_drift, the transposed foot table on the
  * host (about 1.4 entries of 4 bytes per evaluated point, plus the feet
- * interleaved, 16 bytes per point) and keeps, per step of a checkpoint
- * segment, the kick's coefficients and acceleration in HBM. */
+ * interleaved, 16 bytes per point) and holds, per step of a checkpoint
+ * segment, room for the kick's column splines and acceleration in HBM. */
 int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery);
 int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
                 double *grad_f0, double *grad_vals0);

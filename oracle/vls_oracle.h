@@ -53,11 +53,15 @@ typedef struct orc_params {
   double spM[ORC_MAX_SPECIES], spR0[ORC_MAX_SPECIES], spAlpha[ORC_MAX_SPECIES];
   double spSR[ORC_MAX_SPECIES], spGap0[ORC_MAX_SPECIES], spGap1[ORC_MAX_SPECIES];
   double spEta[ORC_MAX_SPECIES], spN[ORC_MAX_SPECIES];
-  /* Reduction order.  nthreads >= 1: the reference's order with that many
+  /* Evaluation order.  nthreads >= 1: the reference's, with that many
    * OpenMP threads under a static schedule (quadrature21.hxx:96-139).
-   * nthreads == 0: the device order, which the HIP kernels use: per slice
-   * and r, serial sums over groups of 16 rows u, combined four to a wave of
-   * 64 rows and then over the waves, then a serial sum over j. */
+   * nthreads == 0: the device order, which the HIP kernels use: the moment
+   * per slice and r as serial sums over groups of 16 rows u, combined four
+   * to a block of 64 rows and then over the blocks, then over j in chunks of
+   * 20 slices (serial within a chunk, then over the chunks); and the kick in
+   * column form (the natural v-spline of each column of f, evaluated with
+   * weights B_k/6), equal to the reference's 2D evaluation on r nodes to
+   * rounding. */
   int nthreads;
 } orc_params;
 

@@ -114,7 +114,7 @@ __device__ __forceinline__ double mom_w(int r, int nR, const double *__restrict_
 }
 
 // Transposed moment quadrature at point (u, r) of a slice (the reverse of
-// the cell averages of k_moment_partial): a quarter of each adjacent cell's
+// the cell averages of k_moment_cols): a quarter of each adjacent cell's
 // weight Ct1[j] dU dR times that cell's dM adjoint.
 __device__ __forceinline__ double moment_adj(int u, int r, int nU, int nR, double ct,
                                              const double *__restrict__ dU,

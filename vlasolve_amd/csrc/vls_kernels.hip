@@ -36,59 +36,12 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
 
 // ---------------------------------------------------------------- moment
 // Per (j, r) partial of quadrature21::integrate1D (src/quadrature21.hxx:104-133):
-// the terms ((Ct1_j*dU_u)*dR_r) * avg4 over u < nU-1, summed in a fixed tree
-// that the fused row solver below also follows: serial sums over groups of 16
-// rows (u = 64w + 16q + t), combined ((G0 + G1) + G2) + G3 into a wave of 64
-// rows, and the waves left to right.  One thread per (j, r); the row loads of
-// the next group are issued before its additions.
-__global__ void __launch_bounds__(kBlock)
-k_moment_partial(const double *__restrict__ f, const double *__restrict__ Ct1,
-                 const double *__restrict__ dU, const double *__restrict__ dR,
-                 double *__restrict__ P, int nR, int nU, int nJ) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (r >= nR) return;
-  double Pj = 0.0;
-  if (r < nR - 1) {
-    const double ct = Ct1[j];
-    const double dr = dR[r];
-    const double *p = f + (size_t)j * nU * nR + r;
-    double a0 = p[0], a1 = p[1];
-    const int nW = (nU + 63) / 64;
-    for (int w = 0; w < nW; w++) {
-      double G[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int ub = 64 * w + 16 * q;
-        double b0[16], b1[16];
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-          const int u = min(ub + t, nU - 2);
-          b0[t] = p[(size_t)(u + 1) * nR];
-          b1[t] = p[(size_t)(u + 1) * nR + 1];
-        }
-        double acc = 0.0;
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-          if (ub + t < nU - 1) {
-            double avg = (((a0 + a1) + b0[t]) + b1[t]) * 0.25;
-            acc += ((ct * dU[ub + t]) * dr) * avg;
-            a0 = b0[t];
-            a1 = b1[t];
-          }
-        }
-        G[q] = acc;
-      }
-      const double W = ((G[0] + G[1]) + G[2]) + G[3];
-      Pj = w == 0 ? W : Pj + W;
-    }
-  }
-  P[(size_t)j * nR + r] = Pj;
-}
-
-// The same partials with a lane per column r of a slice (63 columns per wave;
-// column r + 1 arrives by a lane shift), walking the rows u in order with
-// two groups of 16 rows of loads in flight, the sums in the same tree.
+// the terms ((Ct1_j*dU_u)*dR_r) * avg4 over u < nU-1, summed in a fixed tree:
+// serial sums over groups of 16 rows (u = 64w + 16q + t), combined
+// ((G0 + G1) + G2) + G3 into a block of 64 rows, and the blocks left to
+// right.  k_moment_cols takes a lane per column r of a slice (63 columns per
+// wave; column r + 1 arrives by a lane shift) and walks the rows u in order
+// with two groups of 16 rows of loads in flight.
 __device__ __forceinline__ double lane_next(double x) {  // lane i gets lane i+1's x
   int lo = __double2loint(x), hi = __double2hiint(x);
   lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
@@ -151,9 +104,9 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
   if (lane < 63 && r < nR) P[(size_t)j * nR + r] = term ? Pj : 0.0;
 }
 
-// k_moment_partial over the sum of nS species planes (nVal apart): the
-// integrand of int21::f(0, nS), 0 + avg_0 + avg_1 + ... per cell
-// (src/integrand21.hxx:21-39), in the same tree order.
+// The partials of the sum of nS species planes (nVal apart): the integrand
+// of int21::f(0, nS), 0 + avg_0 + avg_1 + ... per cell
+// (src/integrand21.hxx:21-39), in the same tree order, a thread per (j, r).
 __global__ void __launch_bounds__(kBlock)
 k_moment_partial_sp(const double *__restrict__ f, const double *__restrict__ Ct1,
                     const double *__restrict__ dU, const double *__restrict__ dR,
@@ -1356,145 +1309,59 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
 }
 
 // Row version of the register line solver with coalesced memory access.
-// Wave w of block b takes up to 64 consecutive rows u0 = 64 (b*W + w) ... of
-// slice j (rows of f, contiguous in memory), W = blockDim.x / 64.
-// Right-hand sides arrive in chunks of kC elements per row through a small
-// LDS transpose (loads of whole row segments, then lane = row reads its kC
-// values); the loads of chunks c+1 and c+2 are in flight during the
-// elimination of chunk c.  Coefficients leave the same way, a chunk at a
-// time as the back substitution produces them.  Loads are clamped into the
-// wave's rows (the extra lanes of a partial wave solve a copy and store
-// nothing).
-//
-// MOM: the block holds a whole slice (W = ceil(nU/64) <= 4 waves) and also
-// forms the slice's moment partials P[j][r] from the same data, in the tree
-// order of k_moment_partial: each lane makes the terms of its row pair
-// (u, u+1) (row u+1 is the next lane's; each wave also stages the row after
-// its own), the terms go through LDS, a thread per (group, r) sums its 16
-// rows, a thread per r combines the wave's four groups into its partial
-// W_w[r], and after the forward sweep one thread per r combines the waves.
-// Until then the waves run independently.
-constexpr int kRowsMaxW = 4;
-
-template <int M, bool MOM>
-__global__ void __launch_bounds__(MOM ? 256 : 64)
+// A wave (one block) takes up to 64 consecutive rows u0 = 64 b ... of slice
+// j (rows of f, contiguous in memory).  Right-hand sides arrive in chunks of
+// kC elements per row through a small LDS transpose (loads of whole row
+// segments, then lane = row reads its kC values); the loads of chunks c+1
+// and c+2 are in flight during the elimination of chunk c.  Coefficients
+// leave the same way, a chunk at a time as the back substitution produces
+// them.  Loads are clamped into the wave's rows (the extra lanes of a
+// partial wave solve a copy and store nothing).
+template <int M>
+__global__ void __launch_bounds__(64)
 k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU, int nJ,
-            int64_t cs, int lzmask, ThomasAxis ax, const double *__restrict__ Ct1,
-            const double *__restrict__ dU, const double *__restrict__ dR, double *__restrict__ P) {
+            int64_t cs, int lzmask, ThomasAxis ax) {
   constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
   constexpr int nIn = M - 2;           // right-hand sides per line (= nR)
   constexpr int nChIn = (nIn + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
-  // chunk rows: the wave's 64 rows, and for the moment the row after them
-  constexpr int kCR = MOM ? 65 : 64;
-  __shared__ double chunkAll[(MOM ? kRowsMaxW : 1) * kCR * kP];
-  __shared__ double termAll[MOM ? kRowsMaxW * 64 * kP : 1];
-  __shared__ double gsum[MOM ? kRowsMaxW * 4 * kC : 1];
-  __shared__ double wsum[MOM ? kRowsMaxW * nIn : 1];  // per-wave partials W_w[r]
-  __shared__ double dRs[MOM ? nIn : 1];                // dR, read from LDS by the terms
-  const int W = blockDim.x >> 6;
-  const int w = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  __shared__ double chunk[64 * kP];
+  const int lane = threadIdx.x;
   const int lz = lane & lzmask;
   const int j = blockIdx.y;
-  const int u0 = (blockIdx.x * W + w) * 64;
+  const int u0 = blockIdx.x * 64;
   const int rows = max(min(64, nU - u0), 1);
-  const int rowsE = max(min(kCR, nU - u0), 1);
   const int rsub = lane / kC, ksub = lane % kC;  // staging role: 4 rows x kC elements per pass
-  double *chunk = chunkAll + w * kCR * kP;
   const double *src = f + ((size_t)j * nU + min(u0, nU - 1)) * nIn;
   double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
-  // The waves of a block exchange nothing until the moment's final sums:
-  // each stages its own chunks (and rows), so a wave-level barrier orders
-  // its LDS traffic.
-  auto sync = [] {
-    if (MOM) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-      __syncthreads();
-    }
-  };
   double y[M];
   // chunks of loads in flight: two (the loads of a chunk take a memory
-  // latency, the elimination of a chunk far less); the moment variant makes
-  // room for them by keeping its first kYL forward values in LDS, the last
-  // ones the back substitution needs
+  // latency, the elimination of a chunk far less)
   constexpr int kAh = 2;
-  constexpr int kYL = MOM && M > 160 ? 32 : 0;
-  __shared__ double ylds[kYL > 0 ? kRowsMaxW * 64 * kYL : 1];
-  double tmp[kAh][kC], tmpE[kAh];
-  auto load_chunk = [&](int ch, double *t, double &te) {
+  double tmp[kAh][kC];
+  auto load_chunk = [&](int ch, double *t) {
     const int e = min(ch * kC + ksub, nIn - 1);
 #pragma unroll
     for (int q = 0; q < kC; q++) {
       const int row = min(q * 4 + rsub, rows - 1);
       t[q] = src[row * nIn + e];
     }
-    if (MOM) te = src[min(64, rowsE - 1) * nIn + e];
   };
-  // moment: this lane's row pair, its weight, and the values carried from
-  // the previous chunk's last element
-  const int u = u0 + lane;
-  const bool pairOk = MOM && u < nU - 1;
-  const double ctdu = pairOk ? Ct1[j] * dU[u] : 0.0;
-  double aPrev = 0.0, bPrev = 0.0;
-  if (MOM) {
-    for (int k = threadIdx.x; k < nIn; k += blockDim.x) dRs[k] = dR[k];
-    __syncthreads();
-  }
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
 #pragma unroll
   for (int ch = 0; ch < kAh; ch++)
-    if (ch < nChIn) load_chunk(ch, tmp[ch], tmpE[ch]);
-  // the wave's partial over its 64 rows for chunk ch: its four groups in order
-  auto wave_partial = [&](int ch) {
-    if (lane < kC) {
-      const int i = lane, r = ch * kC + i - 1;
-      const double *g = gsum + w * 4 * kC + i;
-      if (r >= 0 && r < nIn) wsum[w * nIn + r] = ((g[0] + g[kC]) + g[2 * kC]) + g[3 * kC];
-    }
-  };
+    if (ch < nChIn) load_chunk(ch, tmp[ch]);
   // Per chunk: stage it (and issue the loads of chunk ch + kAh into the
-  // registers just staged), then the terms and the elimination, then the
-  // group sums; a group's wave partial is formed at the next chunk.
+  // registers just staged), then the elimination.
 #pragma unroll
   for (int ch = 0; ch < nChIn; ch++) {
 #pragma unroll
     for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = tmp[ch % kAh][q];
-    if (MOM && rsub == 0) chunk[64 * kP + ksub] = tmpE[ch % kAh];
-    if (ch + kAh < nChIn) load_chunk(ch + kAh, tmp[ch % kAh], tmpE[ch % kAh]);
-    sync();
-    if (MOM && ch > 0) wave_partial(ch - 1);
+    if (ch + kAh < nChIn) load_chunk(ch + kAh, tmp[ch % kAh]);
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < nIn) y[1 + ch * kC + i] = chunk[lane * kP + i];
-    if (MOM) {
-      // row u+1: the next chunk row (row 64 is the next wave's first row)
-      const double *nxt = chunk + (lane + 1) * kP;
-      double *T = termAll + (w * 64 + lane) * kP;
-#pragma unroll
-      for (int i = 0; i < kC; i++) {
-        const int e = ch * kC + i;  // column of the new values; term r = e - 1
-        if (e < nIn) {
-          const double a1 = y[1 + e];
-          const double b1 = nxt[i];
-          double t = 0.0;
-          if (e >= 1) {
-            // branch-free: rows without a pair keep 0
-            const double avg = (((aPrev + a1) + bPrev) + b1) * 0.25;
-            const double tv = (ctdu * dRs[e - 1]) * avg;
-            t = pairOk ? tv : 0.0;
-          }
-          T[i] = t;
-          aPrev = a1;
-          bPrev = b1;
-        } else {
-          T[i] = 0.0;
-        }
-      }
-    }
 #pragma unroll
     for (int i = 0; i < kC; i++) {
       const int k = 1 + ch * kC + i;
@@ -1504,37 +1371,11 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
           dv = ax.d[k + lz];
           ev = ax.rd[k + lz];
         }
-        const double yk = div_known(y[k] - c16 * prev, dv, ev);
-        if (k <= kYL)
-          ylds[(k - 1) * (kRowsMaxW * 64) + threadIdx.x] = yk;
-        else
-          y[k] = yk;
-        prev = yk;
+        y[k] = div_known(y[k] - c16 * prev, dv, ev);
+        prev = y[k];
       }
     }
-    sync();
-    if (MOM) {
-      // a thread per (group of 16 rows, r) of its wave: serial over the rows
-      const int i = lane & 15, q = lane >> 4;
-      const double *T = termAll + (w * 64 + 16 * q) * kP + i;
-      double g = 0.0;
-#pragma unroll
-      for (int t = 0; t < 16; t++) g += T[t * kP];
-      gsum[(w * 4 + q) * kC + i] = g;
-    }
-  }
-  if (MOM) {
-    sync();
-    wave_partial(nChIn - 1);
-  }
-  if (MOM) {
-    // the slice's partials: the waves left to right
     __syncthreads();
-    for (int r = threadIdx.x; r < nR; r += blockDim.x) {
-      double Pj = wsum[r];
-      for (int ww = 1; ww < W; ww++) Pj = Pj + wsum[ww * nIn + r];
-      P[(size_t)j * nR + r] = r < nR - 1 ? Pj : 0.0;
-    }
   }
   double tt = 0.0 - ax.hi2 * y[M - 3];
   tt = tt - ax.p * y[M - 2];
@@ -1550,8 +1391,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
       if (k >= 1 && k <= M - 2) {
         double uv = uS;
         if (k <= kHead) uv = ax.u[k + lz];
-        const double yk = k <= kYL ? ylds[(k - 1) * (kRowsMaxW * 64) + threadIdx.x] : y[k];
-        y[k] = yk - uv * cn;
+        y[k] = y[k] - uv * cn;
         cn = y[k];
       }
     }
@@ -1559,13 +1399,13 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 #pragma unroll
     for (int i = 0; i < kC; i++)
       if (ch * kC + i < M) chunk[lane * kP + i] = y[ch * kC + i];
-    sync();
+    __syncthreads();
     // all LDS reads first, so that their latencies overlap, then the stores
     const int k = ch * kC + ksub;
     double sv[kC];
 #pragma unroll
     for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
-    sync();
+    __syncthreads();
     if (u0 < nU) {
       if (rows == 64 && (ch + 1) * kC <= M) {
 #pragma unroll
@@ -1584,8 +1424,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 // Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
 // standard grids.  Other shapes use the LDS-tiled kernels above.
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
-using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis,
-                        const double *, const double *, const double *, double *);
+using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
 using KickFn = void (*)(double *, const double *, const double *, double *, double, double, double,
                         double, int, int, ThomasAxis, double *, int64_t);
 struct LineShape {
@@ -1595,7 +1434,7 @@ struct LineShape {
   KickFn kick, kickKeep;
 };
 #define VLS_LINE_SHAPE(R, U)                                                          \
-  {R, U, k_line_rows<(R) + 2, false>, k_line<(U) + 2, false, (R) + 2>,               \
+  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>,                      \
    k_line<(U) + 2, true, (R) + 2>, k_kick_line<(U) + 2, (R) + 2, false>,             \
    k_kick_line<(U) + 2, (R) + 2, true>}
 static const LineShape kLineShapes[] = {
@@ -1615,7 +1454,7 @@ static const LineShape *line_shape(const Run &r) {
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   if (const LineShape *ls = line_shape(r)) {
     hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
-                       r.nU, r.nJ, r.cs, 0, r.tr, nullptr, nullptr, nullptr, nullptr);
+                       r.nU, r.nJ, r.cs, 0, r.tr);
     time_mark(t, s, "spline_r");
     hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
