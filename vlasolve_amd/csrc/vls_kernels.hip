@@ -40,68 +40,64 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
 // serial sums over groups of 16 rows (u = 64w + 16q + t), combined
 // ((G0 + G1) + G2) + G3 into a block of 64 rows, and the blocks left to
 // right.  k_moment_cols takes a lane per column r of a slice (63 columns per
-// wave; column r + 1 arrives by a lane shift) and walks the rows u in order
-// with two groups of 16 rows of loads in flight.
+// block; column r + 1 arrives by a lane shift) and a wave per block of 64
+// rows (its rows and the next one loaded at once); the blocks' partials meet
+// in LDS.
 __device__ __forceinline__ double lane_next(double x) {  // lane i gets lane i+1's x
   int lo = __double2loint(x), hi = __double2hiint(x);
   lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
   hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
               const double *__restrict__ dU, const double *__restrict__ dR,
               double *__restrict__ P, int nR, int nU) {
-  constexpr int kG = 16;
-  extern __shared__ double dUs[];  // [nU]
-  const int lane = threadIdx.x;
+  extern __shared__ double sh[];  // dU [nU], then the blocks' partials W [nW][64]
+  double *dUs = sh, *Wl = sh + nU;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int j = blockIdx.y;
   const int r = blockIdx.x * 63 + lane;
   const bool term = lane < 63 && r < nR - 1;
-  for (int u = lane; u < nU; u += 64) dUs[u] = dU[u];
+  const int nW = (nU + 63) / 64;  // blocks of 64 rows
+  for (int u = threadIdx.x; u < nU; u += blockDim.x) dUs[u] = dU[u];
   __syncthreads();
   const double *col = f + (size_t)j * nU * nR + min(r, nR - 1);
   const double ct = Ct1[j], dr = term ? dR[r] : 0.0;
-  const int nG = (nU + kG - 1) / kG;         // groups of rows with data
-  const int nGt = 4 * ((nU + 63) / 64);      // groups of the tree (4 per 64 rows)
-  double cur[kG], nxt[kG];
-  auto load = [&](int g, double *b) {
+  // wave wv takes the blocks w = wv, wv + 4, ...: its 64 rows and the next
+  // one, all loaded at once
+  for (int w = wv; w < nW; w += 4) {
+    double row[65];
 #pragma unroll
-    for (int t = 0; t < kG; t++) b[t] = col[(size_t)min(g * kG + t, nU - 1) * nR];
-  };
-  load(0, cur);
-  load(1, nxt);
-  double aOwn = cur[0], aN = lane_next(aOwn);
-  double Pj = 0.0;
-  for (int w = 0; w < nGt / 4; w++) {
+    for (int t = 0; t < 65; t++) row[t] = col[(size_t)min(64 * w + t, nU - 1) * nR];
     double G[4];
+    double aOwn = row[0], aN = lane_next(aOwn);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int g = 4 * w + q;
       double acc = 0.0;
-      if (g < nG) {
 #pragma unroll
-        for (int t = 0; t < kG; t++) {
-          const int u = g * kG + t;
-          const double b = t + 1 < kG ? cur[t + 1] : nxt[0];  // row u + 1
-          const double bN = lane_next(b);
-          if (u < nU - 1) {
-            const double avg = (((aOwn + aN) + b) + bN) * 0.25;
-            acc += ((ct * dUs[u]) * dr) * avg;
-          }
-          aOwn = b;
-          aN = bN;
+      for (int t = 16 * q; t < 16 * q + 16; t++) {
+        const int u = 64 * w + t;
+        const double b = row[t + 1];  // row u + 1
+        const double bN = lane_next(b);
+        if (u < nU - 1) {
+          const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+          acc += ((ct * dUs[u]) * dr) * avg;
         }
-#pragma unroll
-        for (int t = 0; t < kG; t++) cur[t] = nxt[t];
-        if (g + 2 < nG) load(g + 2, nxt);
+        aOwn = b;
+        aN = bN;
       }
       G[q] = acc;
     }
-    const double W = ((G[0] + G[1]) + G[2]) + G[3];
-    Pj = w == 0 ? W : Pj + W;
+    Wl[w * 64 + lane] = ((G[0] + G[1]) + G[2]) + G[3];
   }
-  if (lane < 63 && r < nR) P[(size_t)j * nR + r] = term ? Pj : 0.0;
+  __syncthreads();
+  if (wv == 0) {
+    // the blocks left to right
+    double Pj = Wl[lane];
+    for (int w = 1; w < nW; w++) Pj = Pj + Wl[w * 64 + lane];
+    if (lane < 63 && r < nR) P[(size_t)j * nR + r] = term ? Pj : 0.0;
+  }
 }
 
 // The partials of the sum of nS species planes (nVal apart): the integrand
@@ -1251,8 +1247,9 @@ void launch_moment(Run &r, hipStream_t s) {
     hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P,
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
   else
-    hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63, r.nJ), dim3(64),
-                       r.nU * sizeof(double), s, r.f, r.Ct1, r.dU, r.dR, r.P, r.nR, r.nU);
+    hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63, r.nJ), dim3(256),
+                       (r.nU + 64 * ((r.nU + 63) / 64)) * sizeof(double), s, r.f, r.Ct1, r.dU,
+                       r.dR, r.P, r.nR, r.nU);
   launch_moment_scan(r, s);
 }
 
