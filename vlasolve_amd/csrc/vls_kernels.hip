@@ -69,7 +69,8 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
   for (int w = wv; w < nW; w += 4) {
     double row[65];
 #pragma unroll
-    for (int t = 0; t < 65; t++) row[t] = col[(size_t)min(64 * w + t, nU - 1) * nR];
+    for (int t = 0; t < 65; t++)  // rows past the last one are not loaded (no term reads them)
+      row[t] = 64 * w + t < nU ? col[(size_t)(64 * w + t) * nR] : 0.0;
     double G[4];
     double aOwn = row[0], aN = lane_next(aOwn);
 #pragma unroll
