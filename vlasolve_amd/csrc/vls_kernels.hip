@@ -64,31 +64,42 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
   __syncthreads();
   const double *col = f + (size_t)j * nU * nR + min(r, nR - 1);
   const double ct = Ct1[j], dr = term ? dR[r] : 0.0;
-  // wave wv takes the blocks w = wv, wv + 4, ...: its 64 rows and the next
-  // one, all loaded at once
+  // wave wv takes the blocks w = wv, wv + 4, ...; a block's rows arrive in
+  // two halves of 32 rows (and the row after each), each half's loads all
+  // at once: few enough registers for four waves per SIMD, so that the
+  // blocks of a 200-slice grid are all resident together
   for (int w = wv; w < nW; w += 4) {
-    double row[65];
-#pragma unroll
-    for (int t = 0; t < 65; t++)  // rows past the last one are not loaded (no term reads them)
-      row[t] = 64 * w + t < nU ? col[(size_t)(64 * w + t) * nR] : 0.0;
     double G[4];
-    double aOwn = row[0], aN = lane_next(aOwn);
+    double aOwn = 0.0, aN = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      double acc = 0.0;
+    for (int h = 0; h < 2; h++) {
+      const int u0 = 64 * w + 32 * h;
+      double row[33];  // row[t]: row u0 + t; row u0 itself is carried in aOwn
+      if (h == 0) row[0] = u0 < nU ? col[(size_t)u0 * nR] : 0.0;
 #pragma unroll
-      for (int t = 16 * q; t < 16 * q + 16; t++) {
-        const int u = 64 * w + t;
-        const double b = row[t + 1];  // row u + 1
-        const double bN = lane_next(b);
-        if (u < nU - 1) {
-          const double avg = (((aOwn + aN) + b) + bN) * 0.25;
-          acc += ((ct * dUs[u]) * dr) * avg;
-        }
-        aOwn = b;
-        aN = bN;
+      for (int t = 1; t < 33; t++)  // rows past the last one are not loaded (no term reads them)
+        row[t] = u0 + t < nU ? col[(size_t)(u0 + t) * nR] : 0.0;
+      if (h == 0) {
+        aOwn = row[0];
+        aN = lane_next(aOwn);
       }
-      G[q] = acc;
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 16 * q; t < 16 * q + 16; t++) {
+          const int u = u0 + t;
+          const double b = row[t + 1];  // row u + 1
+          const double bN = lane_next(b);
+          if (u < nU - 1) {
+            const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+            acc += ((ct * dUs[u]) * dr) * avg;
+          }
+          aOwn = b;
+          aN = bN;
+        }
+        G[2 * h + q] = acc;
+      }
     }
     Wl[w * 64 + lane] = ((G[0] + G[1]) + G[2]) + G[3];
   }
