@@ -34,6 +34,16 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
   n = (int)ip;
 }
 
+// Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8), each XCD
+// with its own L2.  xcd_tile maps block b of a 1D grid of nB blocks to a tile
+// index so that consecutive tiles run on one XCD (a bijection of 0..nB-1):
+// the column tiles of one slice, which share the cache lines at their edges,
+// then meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nB) {
+  const int x = b & 7, q = nB >> 3, rem = nB & 7;
+  return x * q + min(x, rem) + (b >> 3);
+}
+
 // ---------------------------------------------------------------- moment
 // Per (j, r) partial of quadrature21::integrate1D (src/quadrature21.hxx:104-133):
 // the terms ((Ct1_j*dU_u)*dR_r) * avg4 over u < nU-1, summed in a fixed tree:
@@ -56,8 +66,10 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
   extern __shared__ double sh[];  // dU [nU], then the blocks' partials W [nW][64]
   double *dUs = sh, *Wl = sh + nU;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int j = blockIdx.y;
-  const int r = blockIdx.x * 63 + lane;
+  const int nX = (nR + 62) / 63;  // column tiles per slice
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nX;
+  const int r = (tile - j * nX) * 63 + lane;
   const bool term = lane < 63 && r < nR - 1;
   const int nW = (nU + 63) / 64;  // blocks of 64 rows
   for (int u = threadIdx.x; u < nU; u += blockDim.x) dUs[u] = dU[u];
@@ -608,23 +620,26 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
        int lzmask, ThomasAxis ax) {
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
+  const int nX = ((KICK ? nR : nR + 2) + 63) / 64;  // column tiles per slice
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nX, bx = tile - j * nX;
   const double *in;
   double *out;
   int64_t is, os;
   bool live;
   if (KICK) {
-    int col = blockIdx.x * 64 + lane;
+    int col = bx * 64 + lane;
     live = col < nR;
     if (!live) col = nR - 1;
-    in = src + (size_t)blockIdx.y * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
-    out = c + (size_t)blockIdx.y * cs + col;
+    in = src + (size_t)j * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
+    out = c + (size_t)j * cs + col;
     is = S - 2;  // = nR, a compile-time stride
     os = S - 2;
   } else {
-    int col = blockIdx.x * 64 + lane;
+    int col = bx * 64 + lane;
     live = col < nR + 2;
     if (!live) col = nR + 1;
-    in = c + (size_t)blockIdx.y * cs + col;
+    in = c + (size_t)j * cs + col;
     out = const_cast<double *>(in);
     is = S;  // = nR + 2, a compile-time stride
     os = S;
@@ -715,8 +730,9 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
   __shared__ double Us[nU];
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
-  const int j = blockIdx.y;
-  const int k = blockIdx.x * kKW + lane;  // column of this lane
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / ((nR + kKW - 1) / kKW);
+  const int k = (tile - j * ((nR + kKW - 1) / kKW)) * kKW + lane;  // column of this lane
   const bool outLane = k < nR;            // writes the points f[j][.][k]
   double *fj = f + (size_t)j * nU * nR;
   const double *col = fj + (k < nR ? k : nR - 1);
@@ -785,7 +801,7 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
   dhi = __builtin_amdgcn_readfirstlane(dhi);
   const int D = dlo;
   const bool useRing = !forceScratch && dhi - dlo <= kRing - 4 - (kEB - 1);
-  double *gs = gsc + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (Mv + 1) * 64 + lane;
+  double *gs = gsc + (size_t)tile * (Mv + 1) * 64 + lane;
 
   auto ring_at = [&](int row) { return ring[(row & (kRing - 1)) * 64 + lane]; };
   auto ring_put = [&](int m, double d) {  // m known at compile time
@@ -1259,7 +1275,7 @@ void launch_moment(Run &r, hipStream_t s) {
     hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P,
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
   else
-    hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63, r.nJ), dim3(256),
+    hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63 * r.nJ), dim3(256),
                        (r.nU + 64 * ((r.nU + 63) / 64)) * sizeof(double), s, r.f, r.Ct1, r.dU,
                        r.dR, r.P, r.nR, r.nU);
   launch_moment_scan(r, s);
@@ -1465,7 +1481,7 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
     hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tr);
     time_mark(t, s, "spline_r");
-    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
+    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
     time_mark(t, s, "spline_v");
     return;
@@ -1515,7 +1531,7 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *d
   }
   if (fused_kick(r) && r.gsc) {
     const LineShape *ls = line_shape(r);
-    hipLaunchKernelGGL(dKeep ? ls->kickKeep : ls->kick, dim3((r.nR + kKW - 1) / kKW, r.nJ),
+    hipLaunchKernelGGL(dKeep ? ls->kickKeep : ls->kick, dim3((r.nR + kKW - 1) / kKW * r.nJ),
                        dim3(64), 0, s, r.f, r.U_Val, r.acc, r.gsc, r.Ulo, r.Uhi, r.start_v,
                        r.hinv_v, r.kickScratch ? 1 : 0, 0, r.tv, dKeep, r.cs);
     time_mark(t, s, "kick_line");
@@ -1527,7 +1543,7 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *d
 
 void launch_kick(Run &r, hipStream_t s, double *d) {
   if (const LineShape *ls = line_shape(r))
-    hipLaunchKernelGGL(ls->kickCols, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, r.f, d, r.nR,
+    hipLaunchKernelGGL(ls->kickCols, dim3((r.nR + 63) / 64 * r.nJ), dim3(64), 0, s, r.f, d, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
   else
     hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, r.f, d, r.nR,
