@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "vls_internal.hpp"
+#include "vls_tile.hpp"
 
 namespace vls {
 
@@ -165,10 +166,11 @@ __device__ __forceinline__ double adj_down(const ThomasAxis &ax, int k, double v
 // v lines (lane = padded column), in place: coalesced rows of 64 columns
 __global__ void __launch_bounds__(64)
 k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
+  const Tile2 tl = xcd_tile2((S + 63) / 64);
+  const int k = tl.x * 64 + threadIdx.x;
   if (k >= S) return;
   const int M = ax.M;
-  double *col = cb + blockIdx.y * cs + k;
+  double *col = cb + tl.j * cs + k;
   double v[kLC];
   double c0 = 0.0, prev = 0.0;
   for (int m0 = 0; m0 < M; m0 += kLC) {
@@ -223,8 +225,9 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
   extern __shared__ double Us[];  // [nU]
   for (int u = threadIdx.x; u < nU; u += 64) Us[u] = U_Val[u];
   __syncthreads();
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y;
+  const Tile2 tl = xcd_tile2((nR + 63) / 64);
+  const int r = tl.x * 64 + threadIdx.x;
+  const int j = tl.j;
   if (r >= nR) return;  // no barriers below
   const int Mv = nU + 2;
   const double a = acc[r];
@@ -303,8 +306,9 @@ k_adj_gcol_down(const double *__restrict__ g, double *__restrict__ fb, int nR, i
                 int64_t cs, ThomasAxis ax, const double *__restrict__ dMb,
                 const double *__restrict__ Ct1, const double *__restrict__ dU,
                 const double *__restrict__ dR) {
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y;
+  const Tile2 tl = xcd_tile2((nR + 63) / 64);
+  const int r = tl.x * 64 + threadIdx.x;
+  const int j = tl.j;
   if (r >= nR) return;
   const int M = ax.M;
   const double *col = g + j * cs + r;
@@ -542,8 +546,9 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
                  double start_v, double hinv_v) {
   __shared__ double part[kRG][64];
   const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
-  const int j = blockIdx.y;
+  const Tile2 tl = xcd_tile2((nR + 63) / 64);
+  const int r = tl.x * 64 + lane;
+  const int j = tl.j;
   double s = 0.0;
   if (r < nR) {
     const double a = acc[r];
@@ -660,10 +665,11 @@ __device__ __forceinline__ void thomas_adjoint_reg(const ThomasAxis &ax, double 
 template <int M, int S>
 __global__ void __launch_bounds__(64)
 k_adj_v_reg(double *__restrict__ cb, int64_t cs, int lzmask, ThomasAxis ax) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
+  const Tile2 tl = xcd_tile2((S + 63) / 64);
+  const int k = tl.x * 64 + threadIdx.x;
   const int lz = threadIdx.x & lzmask;
   if (k >= S) return;
-  double *col = cb + blockIdx.y * cs + k;
+  double *col = cb + tl.j * cs + k;
   double c[M];
 #pragma unroll
   for (int m = 0; m < M; m++) c[m] = col[(int64_t)m * S];
@@ -681,8 +687,9 @@ k_adj_gcol(const double *__restrict__ g, double *__restrict__ fb, int nR, int64_
            ThomasAxis ax, const double *__restrict__ dMb, const double *__restrict__ Ct1,
            const double *__restrict__ dU, const double *__restrict__ dR) {
   constexpr int nU = M - 2;
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y;
+  const Tile2 tl = xcd_tile2((nR + 63) / 64);
+  const int r = tl.x * 64 + threadIdx.x;
+  const int j = tl.j;
   const int lz = threadIdx.x & lzmask;
   if (r >= nR) return;
   const double *col = g + j * cs + r;
@@ -844,12 +851,12 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
 
 void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
   if (const AdjShape *a = adj_shape(r)) {
-    hipLaunchKernelGGL(a->v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.cs, 0, r.tv);
+    hipLaunchKernelGGL(a->v, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, cb, r.cs, 0, r.tv);
     hipLaunchKernelGGL(a->r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nU, r.cs, 0,
                        r.tr);
     return;
   }
-  hipLaunchKernelGGL(k_adj_v, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, cb, r.nR + 2, r.cs,
+  hipLaunchKernelGGL(k_adj_v, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, cb, r.nR + 2, r.cs,
                      r.tv);
   hipLaunchKernelGGL(k_adj_r, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, cb, out, r.nR, r.nU,
                      r.cs, r.tr);
@@ -858,13 +865,13 @@ void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
 void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *accK, double *cb,
                      double *abp, hipStream_t s) {
   if (!adj_shape(r)) {  // long columns: the walk (its ascending sweep included)
-    hipLaunchKernelGGL(k_adj_kick_walk, dim3((r.nR + 63) / 64, r.nJ), dim3(64),
+    hipLaunchKernelGGL(k_adj_kick_walk, dim3((r.nR + 63) / 64 * r.nJ), dim3(64),
                        r.nU * sizeof(double), s, fsb, cK,
                        r.U_Val, accK, cb, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v,
                        r.tv);
     return;
   }
-  hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64, r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
+  hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64 * r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
                      r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
   // the fewest passes of at most kB threads over the row, split evenly
   const int passes = (r.nR + kB - 1) / kB;
@@ -874,7 +881,7 @@ void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *
 }
 
 void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipStream_t s) {
-  const dim3 grid((r.nR + 63) / 64, r.nJ);
+  const dim3 grid((r.nR + 63) / 64 * r.nJ);
   if (const AdjShape *a = adj_shape(r)) {
     hipLaunchKernelGGL(a->gcol, grid, dim3(64), 0, s, g, fb, r.nR, r.cs, 0, r.tv, dMb, r.Ct1,
                        r.dU, r.dR);

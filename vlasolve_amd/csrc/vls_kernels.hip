@@ -13,6 +13,7 @@
 
 #include "vls_div.hpp"
 #include "vls_internal.hpp"
+#include "vls_tile.hpp"
 
 namespace vls {
 
@@ -34,15 +35,6 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
   n = (int)ip;
 }
 
-// Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8), each XCD
-// with its own L2.  xcd_tile maps block b of a 1D grid of nB blocks to a tile
-// index so that consecutive tiles run on one XCD (a bijection of 0..nB-1):
-// the column tiles of one slice, which share the cache lines at their edges,
-// then meet in one L2.
-__device__ __forceinline__ int xcd_tile(int b, int nB) {
-  const int x = b & 7, q = nB >> 3, rem = nB & 7;
-  return x * q + min(x, rem) + (b >> 3);
-}
 
 // ---------------------------------------------------------------- moment
 // Per (j, r) partial of quadrature21::integrate1D (src/quadrature21.hxx:104-133):
@@ -492,9 +484,10 @@ k_spline_v_stream(double *__restrict__ c, int nR, int64_t cs, int lzmask, Thomas
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;  // a zero the compiler treats as per-lane
   const int S = nR + 2;
-  const int col = blockIdx.x * 64 + lane;
+  const Tile2 t = xcd_tile2((S + 63) / 64);
+  const int col = t.x * 64 + lane;
   if (col >= S) return;  // no barriers below
-  double *tab = c + (size_t)blockIdx.y * cs + col;
+  double *tab = c + (size_t)t.j * cs + col;
   double ym2, ym1;
   thomas_forward<16>(
       ax, lz, [&](int k) { return tab[(int64_t)k * S]; },
@@ -970,10 +963,11 @@ k_kick_vsolve(const double *__restrict__ f, double *__restrict__ d, int nR, int 
               int lzmask, ThomasAxis ax) {
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
-  const int col = blockIdx.x * 64 + lane;
+  const Tile2 t = xcd_tile2((nR + 63) / 64);
+  const int col = t.x * 64 + lane;
   if (col >= nR) return;  // no barriers below
-  const double *fc = f + (size_t)blockIdx.y * nU * nR + col;
-  double *dc = d + (size_t)blockIdx.y * ds + col;
+  const double *fc = f + (size_t)t.j * nU * nR + col;
+  double *dc = d + (size_t)t.j * ds + col;
   double ym2, ym1;
   thomas_forward<16>(
       ax, lz, [&](int k) { return fc[(int64_t)(k - 1) * nR]; },
@@ -1496,7 +1490,7 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   }
   time_mark(t, s, "spline_r");
   if (r.streamV) {
-    hipLaunchKernelGGL(k_spline_v_stream, dim3((r.nR + 2 + 63) / 64, r.nJ), dim3(64), 0, s, r.c,
+    hipLaunchKernelGGL(k_spline_v_stream, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, r.c,
                        r.nR, r.cs, 0, r.tv);
   } else {
     dim3 gv((r.nR + 2 + r.TC - 1) / r.TC, r.nJ);
@@ -1546,7 +1540,7 @@ void launch_kick(Run &r, hipStream_t s, double *d) {
     hipLaunchKernelGGL(ls->kickCols, dim3((r.nR + 63) / 64 * r.nJ), dim3(64), 0, s, r.f, d, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tv);
   else
-    hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64, r.nJ), dim3(64), 0, s, r.f, d, r.nR,
+    hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64 * r.nJ), dim3(64), 0, s, r.f, d, r.nR,
                        r.nU, r.cs, 0, r.tv);
   dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
   hipLaunchKernelGGL(k_kick_cols, g, dim3(kBlock), 0, s, r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs,
