@@ -750,24 +750,25 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
   double y[Mv];
 #pragma unroll
   for (int m = 1; m <= kLA; m++) y[m] = col[(m - 1) * nR];
-  // target-row offsets n - u of the lane's points (while the loads fly)
+  // target-row offsets n - u of the lane's points: point u = m - 1 at step
+  // m of the elimination, whose dependent chain leaves the VALU idle most of
+  // the time (its U value read from LDS one step ahead)
   int dlo = 1 << 30, dhi = -(1 << 30);
-  if (outLane) {
-#pragma unroll 4
-    for (int u = 0; u < nU; u++) {
-      const double nu = Us[u] + a;
-      if (!(nu <= Ulo) && !(nu >= Uhi)) {
-        const int n = (int)trunc((nu - start_v) * hinv_v);
-        dlo = min(dlo, n - u);
-        dhi = max(dhi, n - u);
-      }
-    }
-  }
+  double usNext = Us[0];
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
 #pragma unroll
   for (int m = 1; m <= Mv - 2; m++) {
     if (m + kLA <= Mv - 2) y[m + kLA] = col[(m + kLA - 1) * nR];
+    {
+      const double nu = usNext + a;
+      if (m < nU) usNext = Us[m];
+      if (outLane && !(nu <= Ulo) && !(nu >= Uhi)) {
+        const int n = (int)trunc((nu - start_v) * hinv_v);
+        dlo = min(dlo, n - (m - 1));
+        dhi = max(dhi, n - (m - 1));
+      }
+    }
     double dv = dS, ev = rdS;
     if (m <= kHead) {
       dv = ax.d[m + lz];
