@@ -55,17 +55,16 @@ __global__ void __launch_bounds__(256)
 k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
               const double *__restrict__ dU, const double *__restrict__ dR,
               double *__restrict__ P, int nR, int nU) {
-  extern __shared__ double sh[];  // dU [nU], then the blocks' partials W [nW][64]
-  double *dUs = sh, *Wl = sh + nU;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  extern __shared__ double Wl[];  // the blocks' partials W [nW][64]
+  const int lane = threadIdx.x & 63;
+  // wave-uniform: the rows u below are too, so dU[u] is one load per wave
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nX = (nR + 62) / 63;  // column tiles per slice
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int j = tile / nX;
   const int r = (tile - j * nX) * 63 + lane;
   const bool term = lane < 63 && r < nR - 1;
   const int nW = (nU + 63) / 64;  // blocks of 64 rows
-  for (int u = threadIdx.x; u < nU; u += blockDim.x) dUs[u] = dU[u];
-  __syncthreads();
   const double *col = f + (size_t)j * nU * nR + min(r, nR - 1);
   const double ct = Ct1[j], dr = term ? dR[r] : 0.0;
   // wave wv takes the blocks w = wv, wv + 4, ...; a block's rows arrive in
@@ -97,7 +96,7 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
           const double bN = lane_next(b);
           if (u < nU - 1) {
             const double avg = (((aOwn + aN) + b) + bN) * 0.25;
-            acc += ((ct * dUs[u]) * dr) * avg;
+            acc += ((ct * dU[u]) * dr) * avg;
           }
           aOwn = b;
           aN = bN;
@@ -729,13 +728,14 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
   const bool outLane = k < nR;            // writes the points f[j][.][k]
   double *fj = f + (size_t)j * nU * nR;
   const double *col = fj + (k < nR ? k : nR - 1);
-  for (int u = lane; u < nU; u += 64) Us[u] = U_Val[u];
-  // the ring starts zeroed: a point whose target row is the last knot reads
-  // the row past the padded column with weight BV2(0) = 0
+  // loads in order: the U values, the acceleration, then the first kLA rows
+  // of the column (below), so that the LDS copy of the U values waits only
+  // for the first ones
+  constexpr int nUq = (nU + 63) / 64;
+  double uq[nUq];
 #pragma unroll
-  for (int s = 0; s < kRing + 3; s++) ring[s * 64 + lane] = 0.0;
+  for (int q = 0; q < nUq; q++) uq[q] = U_Val[min(lane + 64 * q, nU - 1)];
   const double a = outLane ? acc[k] : 0.0;
-  __syncthreads();
   double *fo = fj + (outLane ? k : 0);
   // KEEP (the adjoint's recorded steps): the coefficients d_m of the column
   // are also stored, to dKeep[j][m][k] (slice stride ds)
@@ -748,8 +748,22 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
   // streams in kLA rows ahead of the elimination, which runs while it arrives
   constexpr int kLA = Mv - 2 < 48 ? Mv - 2 : 48;
   double y[Mv];
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int m = 1; m <= kLA; m++) y[m] = col[(m - 1) * nR];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < nUq; q++)
+    if (lane + 64 * q < nU) Us[lane + 64 * q] = uq[q];
+  // the ring starts zeroed: a point whose target row is the last knot reads
+  // the row past the padded column with weight BV2(0) = 0
+#pragma unroll
+  for (int s = 0; s < kRing + 3; s++) ring[s * 64 + lane] = 0.0;
+  // the block is one wave: a wave-level barrier orders the LDS writes before
+  // the reads of other lanes, without waiting for the column's loads
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // target-row offsets n - u of the lane's points: point u = m - 1 at step
   // m of the elimination, whose dependent chain leaves the VALU idle most of
   // the time (its U value read from LDS one step ahead)
@@ -1271,7 +1285,7 @@ void launch_moment(Run &r, hipStream_t s) {
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
   else
     hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63 * r.nJ), dim3(256),
-                       (r.nU + 64 * ((r.nU + 63) / 64)) * sizeof(double), s, r.f, r.Ct1, r.dU,
+                       64 * ((r.nU + 63) / 64) * sizeof(double), s, r.f, r.Ct1, r.dU,
                        r.dR, r.P, r.nR, r.nU);
   launch_moment_scan(r, s);
 }
