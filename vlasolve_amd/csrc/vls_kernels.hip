@@ -43,8 +43,8 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
 // ((G0 + G1) + G2) + G3 into a block of 64 rows, and the blocks left to
 // right.  k_moment_cols takes a lane per column r of a slice (63 columns per
 // block; column r + 1 arrives by a lane shift) and a wave per block of 64
-// rows (its rows and the next one loaded at once); the blocks' partials meet
-// in LDS.
+// rows (loaded in two halves of 32 rows and the row after each); the blocks'
+// partials meet in LDS.
 __device__ __forceinline__ double lane_next(double x) {  // lane i gets lane i+1's x
   int lo = __double2loint(x), hi = __double2hiint(x);
   lo = __builtin_amdgcn_mov_dpp(lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
