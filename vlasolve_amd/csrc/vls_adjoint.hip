@@ -67,13 +67,6 @@ __device__ __forceinline__ void split(double x, int &n, double &t) {
   n = (int)ip;
 }
 
-__global__ void __launch_bounds__(kB)
-k_interleave(const double *__restrict__ a, const double *__restrict__ b, double2 *__restrict__ ab,
-             int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-  if (i < n) ab[i] = make_double2(a[i], b[i]);
-}
-
 __global__ void __launch_bounds__(kB) k_fill(double *__restrict__ p, int64_t n, double v) {
   const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
   if (i < n) p[i] = v;
@@ -802,11 +795,6 @@ k_adj_rows(const double *__restrict__ cb, double *__restrict__ out, int nU, int6
 
 }  // namespace
 
-void launch_interleave(const double *a, const double *b, double *ab, int64_t n, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_interleave, dim3(nblocks(n)), dim3(kB), 0, s, a, b, (double2 *)ab, n);
-}
-
 void launch_fill(double *p, int64_t n, double v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kB), 0, s, p, n, v);
 }
@@ -841,7 +829,7 @@ void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const
                       int64_t stepOff, hipStream_t s) {
   const int ntm = (r.nU + 2 + kTile - 1) / kTile, ntk = (r.nR + 2 + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_adj_drift_tiles, dim3(ntm * ntk, r.nJ), dim3(64), 0, s, fb, cb,
-                     (const double2 *)r.adj.footXY, r.adj.tileOff, r.adj.tilePts,
+                     (const double2 *)r.feet, r.adj.tileOff, r.adj.tilePts,
                      (int64_t)r.nU * r.nR, r.cs, r.nR, r.nU, ntk);
   if (r.nChains)
     hipLaunchKernelGGL(k_adj_chains, dim3(nblocks(r.nChains)), dim3(kB), 0, s, fb, cb, r.ch_point,

@@ -440,8 +440,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       VLS_HIP_CHECK(hipMemsetAsync(r.c, 0, ((size_t)r.nJ * r.cs + kCoefPad(r)) * sizeof(double), s));
       if (size_t n = vls::kick_scratch_doubles(r)) r.gsc = dalloc<double>(r, n);
       r.kickScratch = getenv("VLS_KICK_SCRATCH") != nullptr;
-      r.xr = dalloc<double>(r, r.nVal);
-      r.xu = dalloc<double>(r, r.nVal);
+      r.feet = dalloc<double>(r, 2 * r.nVal);
       r.P = dalloc<double>(r, (size_t)r.nJ * r.nR);
       r.dM = dalloc<double>(r, r.nR);
       r.Mx = dalloc<double>(r, r.nR);
@@ -571,8 +570,9 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
     }
     r.hflag.assign(flag, flag + r.nVal);
     hipStream_t s = ctx->c.stream;
-    h2d(r.xr, xf.data(), r.nVal, s);
-    h2d(r.xu, uf.data(), r.nVal, s);
+    std::vector<double> ft(2 * r.nVal);
+    for (int64_t i = 0; i < r.nVal; i++) ft[2 * i] = xf[i], ft[2 * i + 1] = uf[i];
+    h2d(r.feet, ft.data(), 2 * r.nVal, s);
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.hxr.swap(xf);
     r.hxu.swap(uf);
@@ -873,13 +873,10 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
       dfree(r, A.tileOff);
       dfree(r, A.tilePts);
-      dfree(r, A.footXY);
       A.tileOff = dalloc<int64_t>(r, off.size());
       A.tilePts = dalloc<uint32_t>(r, pts.size());
-      A.footXY = dalloc<double>(r, 2 * nv);
       h2d(A.tileOff, off.data(), off.size(), s);
       h2d(A.tilePts, pts.data(), pts.size(), s);
-      vls::launch_interleave(r.xr, r.xu, A.footXY, (int64_t)nv, s);
       VLS_HIP_CHECK(hipStreamSynchronize(s));
       A.tilesValid = true;
     }

@@ -67,8 +67,9 @@ struct Run {
   double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
   double *gsc = nullptr;    // fused kick: per-wave scratch for wide kicks
   bool kickScratch = false; // fused kick: always take the scratch path
-  // static feet (xr <= -1 marks points outside the grid or on delay lines)
-  double *xr = nullptr, *xu = nullptr;
+  // static feet, interleaved (xr, xu) per point: one 16-byte load each
+  // (xr <= -1 marks points outside the grid or on delay lines)
+  double *feet = nullptr;  // [nVal][2]
   std::vector<uint8_t> hflag;  // host copy of the VLS_FOOT_* kinds
   // moment
   double *P = nullptr;      // [nJ][nR] per-slice partials
@@ -107,7 +108,6 @@ struct Run {
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
     int64_t *tileOff = nullptr;  // transposed foot table of the drift, per coefficient tile
     uint32_t *tilePts = nullptr;
-    double *footXY = nullptr;    // the folded feet interleaved (xr, xu)
     bool tilesValid = false;
     bool recorded = false;
   } adj;
@@ -156,7 +156,6 @@ void time_mark(Timing *t, hipStream_t s, const char *name);
 // adjoint launchers (vls_adjoint.hip)
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 // ab[2i] = a[i], ab[2i+1] = b[i]
-void launch_interleave(const double *a, const double *b, double *ab, int64_t n, hipStream_t s);
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s);
 // transposed spline solves, cb (destroyed) -> out

@@ -1072,22 +1072,22 @@ struct ChainTables {
 };
 
 __global__ void __launch_bounds__(kBlock)
-k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__restrict__ xr,
-        const double *__restrict__ xu, int64_t nVal, int nSlice, int64_t cs, int stride,
-        int perXcd, ChainTables ch) {
+k_drift(double *__restrict__ f, const double *__restrict__ c, const double2 *__restrict__ feet,
+        int64_t nVal, int nSlice, int64_t cs, int stride, int perXcd, ChainTables ch) {
   const int b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7) * perXcd + (b >> 3);  // XCD-contiguous block
   const int64_t i = lb * kBlock + threadIdx.x;
   if (i >= nVal) return;
-  const double x = xr[i];
+  const double2 ft = feet[i];
+  const double x = ft.x;
   const int64_t j = i / nSlice;
   const double *cj = c + j * cs;
   if (x > kFootOut) {
-    f[i] = eval_c(cj, stride, x, xu[i]);
+    f[i] = eval_c(cj, stride, x, ft.y);
   } else if (x == kFootOut) {
     f[i] = 0;
   } else {
-    const double kd = xu[i];
+    const double kd = ft.y;
     if (!(kd >= 0) || kd >= (double)ch.n) return;  // no delay line attached
     const int64_t k = (int64_t)kd;
     const int L = ch.len[k];
@@ -1103,12 +1103,10 @@ k_drift(double *__restrict__ f, const double *__restrict__ c, const double *__re
 
 // Attach delay line k to its point: xr = kFootChain, xu = k.
 __global__ void __launch_bounds__(kBlock)
-k_mark_chains(double *__restrict__ xr, double *__restrict__ xu, const int64_t *__restrict__ point,
-              int64_t n) {
+k_mark_chains(double2 *__restrict__ feet, const int64_t *__restrict__ point, int64_t n) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  xr[point[k]] = kFootChain;
-  xu[point[k]] = (double)k;
+  feet[point[k]] = make_double2(kFootChain, (double)k);
 }
 
 // ---------------------------------------------------------------- statistics
@@ -1567,14 +1565,15 @@ void launch_drift(Run &r, hipStream_t s) {
   const int perXcd = (int)((nb + 7) / 8);
   ChainTables ch{r.ch_off,  r.ch_len, r.ch_head,  r.ch_xr,
                  r.ch_xu,   r.ch_out, r.ch_vals,  r.nChains, r.rollChains ? 1 : 0};
-  hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c, r.xr, r.xu, r.nVal,
+  hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c,
+                     (const double2 *)r.feet, r.nVal,
                      r.nU * r.nR, r.cs, r.nR + 2, perXcd, ch);
 }
 
 void mark_chains(Run &r, hipStream_t s) {
   if (r.nChains == 0) return;
-  hipLaunchKernelGGL(k_mark_chains, dim3(blocks_for(r.nChains)), dim3(kBlock), 0, s, r.xr, r.xu,
-                     r.ch_point, r.nChains);
+  hipLaunchKernelGGL(k_mark_chains, dim3(blocks_for(r.nChains)), dim3(kBlock), 0, s,
+                     (double2 *)r.feet, r.ch_point, r.nChains);
 }
 
 void launch_stats(Run &r, hipStream_t s) {
