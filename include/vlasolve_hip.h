@@ -191,7 +191,7 @@ int vls_mass_profile(vls_ctx *ctx, int run, double *M);
  * ([nR], the loss's derivative in the mass profile of f_N as returned by
  * vls_mass_profile; may be NULL) and dL_df ([nJ][nU][nR], its direct
  * derivative in f_N; may be NULL).  The recorded run also keeps, for every
- * step of the trailing checkpoint segments that fit in three quarters of the
+ * step of the trailing checkpoint segments that fit in seven eighths of the
  * free HBM (all of them for grids like 129x129x64; the environment variable
  * VLS_ADJ_KEEP_STEPS caps the kept steps), the kick's column splines and
  * acceleration; vls_adjoint recomputes the other segments from their

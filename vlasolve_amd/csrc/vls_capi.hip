@@ -839,7 +839,7 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
       A.capSeg = seg;
     }
-    // Trailing segments whose kick coefficients fit in 3/4 of the free HBM
+    // Trailing segments whose kick coefficients fit in 7/8 of the free HBM
     // (counting the buffer this run already holds for them) are kept by the
     // recorded run itself, so that the reverse sweep need not recompute them
     // (all of them for grids like 129x129x64; VLS_ADJ_KEEP_STEPS caps the
@@ -848,7 +848,7 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       size_t freeB = 0, totalB = 0;
       VLS_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
       const size_t stepB = (coef + (size_t)r.nR) * sizeof(double);
-      int64_t cap = (int64_t)((freeB + (size_t)A.capKeep * stepB) / 4 * 3 / stepB);
+      int64_t cap = (int64_t)((freeB + (size_t)A.capKeep * stepB) / 8 * 7 / stepB);
       if (const char *e = getenv("VLS_ADJ_KEEP_STEPS")) cap = std::min<int64_t>(cap, atoll(e));
       int keepSegs = 0;
       while (keepSegs < nCk) {
