@@ -153,10 +153,13 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
 // One step that also keeps what the adjoint of the step needs: the kick's
 // acceleration (accOut, [nR]) and the v-spline coefficients of the columns
 // of f (cK, [j][m][r] with slice stride cs), which the kick leaves there.
-void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut) {
+// Without advance the step stops after the kick (the state after it is not
+// needed: the last step of a recomputed segment).
+void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut, bool advance = true) {
   vls::launch_moment(r, s);
   d2d(accOut, r.acc, r.nR, s);
   vls::launch_spline_kick(r, s, nullptr, false, cK);
+  if (!advance) return;
   vls::launch_spline(r, r.f, s, nullptr);
   vls::launch_drift(r, s);
 }
@@ -930,12 +933,15 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
       const bool kept = n0 >= A.keptFrom;
       if (!kept) {
         // recompute the segment from its checkpoint, keeping each step's kick
-        // coefficients and acceleration
+        // coefficients and acceleration; the state after its last step is
+        // not needed (the next segment has its own checkpoint, and the
+        // recorded end state is restored below)
         d2d(r.f, A.ckF + nv * sg, nv, s);
         d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
         d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
         for (int n = n0; n < n1; n++)
-          step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0));
+          step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0),
+                    n + 1 < n1);
       }
       // reverse through the segment
       for (int n = n1 - 1; n >= n0; n--) {
