@@ -346,11 +346,17 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
   // rows x columns [e0, e0 + kLC) of a (rows x width) array <-> LDS, lane-row view
   auto gather = [&](const double *a, int width, int e0) {
     const int e = e0 + ksub;
+    // every load of the chunk leaves before the first LDS write (the
+    // scheduler would otherwise issue a few, wait for them, and go on)
+    double t[kLC];
 #pragma unroll
     for (int q = 0; q < kLC; q++) {
       const int row = min(q * 4 + rsub, rows - 1);
-      chunk[(q * 4 + rsub) * kP + ksub] = e < width ? a[(size_t)row * width + e] : 0.0;
+      t[q] = a[(size_t)row * width + min(e, width - 1)];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < kLC; q++) chunk[(q * 4 + rsub) * kP + ksub] = e < width ? t[q] : 0.0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kLC; i++) v[i] = chunk[lane * kP + i];
