@@ -519,11 +519,17 @@ k_spline_r_stream(const double *__restrict__ f, double *__restrict__ c, int nR, 
   // columns [e0, e0 + kC) of a (rows x width) array into v[] of the lane's row
   auto gather = [&](const double *a, int width, int e0) {
     const int e = e0 + ksub;
+    // every load of the chunk leaves before the first LDS write (the
+    // scheduler would otherwise issue a few, wait for them, and go on)
+    double t[kC];
 #pragma unroll
     for (int q = 0; q < kC; q++) {
       const int row = min(q * 4 + rsub, rows - 1);
-      chunk[(q * 4 + rsub) * kP + ksub] = e < width ? a[(size_t)row * width + e] : 0.0;
+      t[q] = a[(size_t)row * width + min(e, width - 1)];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < kC; q++) chunk[(q * 4 + rsub) * kP + ksub] = e < width ? t[q] : 0.0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kC; i++) v[i] = chunk[lane * kP + i];
