@@ -171,6 +171,9 @@ int vls_check_division(const double *b, int nb, int64_t samples, uint64_t seed,
 /* Sizes of one run: {nR, nU, nJ, number of delay lines, delay-line cells,
  * species}. */
 int vls_run_info(vls_ctx *ctx, int run, int64_t out[6]);
+/* Device memory the run holds: number of allocations and their bytes (the
+ * adjoint's checkpoints included).  No counterpart in the reference. */
+int vls_memory(vls_ctx *ctx, int run, int64_t *count, int64_t *bytes);
 /* Overwrite the delay-line state (the inverse of vls_get_chains): ring heads
  * and ring values in the layout of vls_set_chains.  Either may be NULL. */
 int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *vals);
@@ -179,6 +182,18 @@ int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *val
  * gridHandler::accumulate, src/quadrature21.hxx:88-142,
  * src/grid_handler.hxx:350-407).  Refreshes the moments of vls_moments. */
 int vls_mass_profile(vls_ctx *ctx, int run, double *M);
+
+/* Shell density and gravitational potential of the current f, from its
+ * moment dM (as vls_mass_profile computes it) and the enclosed mass
+ * Mx[r] = M(<R_r): rho[k] = dM[k] / ((4 pi/3)(R[k+1]^3 - R[k]^3)) for
+ * k < nR-1, rho[nR-1] = 0; phi[i] = -G (Mx[i]/R[i] + sum_{i<=k<nR-1} dM[k]/rbar[k])
+ * with rbar[k] = (R[k]+R[k+1])/2, the sum taken from the outermost shell in.
+ * The reference has no such output: its yorick density() integrates
+ * snapshots over (u, j) (yorick/vlasolve.i:39-64) and potentialEnergy pairs
+ * each shell with the mass below it (src/vlasovSolver21.hxx:627-642); these
+ * are the same physics from the step's own quadrature.  Either pointer may
+ * be NULL.  Refreshes the moments of vls_moments. */
+int vls_density_potential(vls_ctx *ctx, int run, double *rho, double *phi);
 
 /* Discrete adjoint of the step.  No counterpart in the reference (it has no
  * derivative); it differentiates exactly the discrete map that vls_step

@@ -395,3 +395,44 @@ def run_reference(cfg: dict, workdir, threads: int = 1, timeout: float = 600.0):
     if res.returncode != 0:
         raise RuntimeError(f"reference failed ({res.returncode}): {res.stderr[-2000:]}")
     return res.stdout
+
+
+# ---------------------------------------------------------------------------
+def density_potential(R, dM, Mx=None, G=1.0):
+    """Density and gravitational potential of the radial shells, from the
+    moment dM[r] of integrate1D (quadrature21.hxx:88-142) and the exclusive
+    enclosed mass Mx[r] = M(<R_r) (gridHandler::accumulate with addSpare=1,
+    grid_handler.hxx:350-407; computed from dM when not given).
+
+    The reference has no rho or potential output of its own: its yorick
+    density() integrates snapshots over (u, j) (yorick/vlasolve.i:39-64) and
+    potentialEnergy pairs each shell with the mass enclosed below it
+    (vlasovSolver21.hxx:627-642).  Defined here from the step's own moment:
+
+      rho[k] = dM[k] / ((4 pi / 3) (R[k+1]^3 - R[k]^3)),  rho[nR-1] = 0
+      phi[i] = -G (Mx[i] / R[i] + S[i]),  S[i] = sum_{i <= k < nR-1} dM[k] / rbar[k]
+
+    with rbar[k] = (R[k] + R[k+1]) / 2 the shell's midpoint and S summed from
+    the outermost shell inwards.  Evaluated with Python floats (IEEE double,
+    one rounding per operation) in exactly this order, so that the device's
+    kernel, which runs the same operations, agrees bit for bit."""
+    R = [float(x) for x in R]
+    dM = [float(x) for x in dM]
+    n = len(R)
+    if Mx is None:
+        Mx, m = [], 0.0
+        for k in range(n):
+            Mx.append(m)
+            m = m + dM[k]
+    Mx = [float(x) for x in Mx]
+    vol = (4.0 * math.pi) / 3.0
+    rho = [0.0] * n
+    for k in range(n - 1):
+        rho[k] = dM[k] / (vol * (R[k + 1] * R[k + 1] * R[k + 1] - R[k] * R[k] * R[k]))
+    phi = [0.0] * n
+    S = 0.0
+    for i in range(n - 1, -1, -1):
+        if i < n - 1:
+            S = dM[i] / (0.5 * (R[i] + R[i + 1])) + S
+        phi[i] = -(G * (Mx[i] / R[i] + S))
+    return np.array(rho), np.array(phi)

@@ -66,3 +66,30 @@ def test_reference_reproducibility_floor_recorded():
     # the reference itself differs between 1 and 8 threads by summation order
     floor = FIX["reference_floor_1_vs_8_threads_udf_c1"]
     assert 0 < floor < 1e-12
+
+
+def test_long_fixture_initial_row():
+    """The 1000-step reference fixture of the 201 x 201 x 200 grid starts
+    from the oracle's initial conditions (its first statistics row), and its
+    stored moment, density and potential follow from its stored final f."""
+    from tests.configs import UDF_C2
+
+    z = np.load(GOLD / "udf_c2_1000.npz")
+    o = O.Oracle(UDF_C2, nthreads=1)
+    s = o.stats_total()
+    row = z["stats"][0]
+    assert row[1] == 0
+    for i, key in enumerate(("M", "E", "Ek", "Ep", "S", "L1", "L2")):
+        assert s[key] == pytest.approx(row[2 + i], rel=2e-9, abs=1e-300), key
+    R = o.axes()[0]
+    rho, phi = O.density_potential(R, z["dM"], z["M"], G=1.0)
+    assert np.array_equal(rho, z["rho"]) and np.array_equal(phi, z["phi"])
+    assert z["M"][-1] + z["dM"][-1] == pytest.approx(float(z["mtot"]), rel=1e-14)
+
+
+def test_cli_fixture_agrees_with_case_fixture():
+    """The command-line fixture and the udf_c1 case fixture are the same run
+    (the same statistics rows)."""
+    cli = json.loads((GOLD / "ref_cli_c1.json").read_text())
+    case = FIX["cases"]["udf_c1"]["stats"]
+    assert cli["stats"] == case

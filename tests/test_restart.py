@@ -108,3 +108,24 @@ def test_reference_continues_from_our_restart(tmp_path):
     rel = np.abs(fg - ref["f"]).max() / np.abs(ref["f"]).max()
     print(f"reference continuation vs GPU: max rel diff {rel:.2e}")
     assert rel <= 1e-11
+
+
+@needs_ref
+def test_host_refuses_nonzero_kernel_mass(tmp_path):
+    """The reference adds a restart's kernelMass to the enclosed mass of the
+    kick (vlasovSolver21_spline.hxx:738-742) and always writes 0; the device
+    step has no such term, so a file with another value is refused."""
+    import struct
+
+    path = _ref_final(UDF_TINY, 3, tmp_path)
+    raw = bytearray(path.read_bytes())
+    tag = b"VLASOV_SOLVER21 v0.12"
+    at = raw.find(struct.pack("<i", len(tag)) + tag)
+    assert at > 0
+    o = at + 4 + len(tag) + 4  # the tag, then kernelType (int), then kernelMass (double)
+    assert struct.unpack_from("<d", raw, o)[0] == 0.0
+    struct.pack_into("<d", raw, o, 0.5)
+    bad = tmp_path / "patched.vls"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(V.VlsError, match="kernelMass"):
+        V.HostSetup({"reload": str(bad.with_suffix(""))})

@@ -31,7 +31,7 @@ EXPORTS = (
     "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
     "vls_record", "vls_adjoint", "vls_set_graph", "vlsd_ensemble_create", "vlsd_ensemble_destroy",
     "vlsd_ensemble_context", "vls_stats_total", "vlsd_species", "vlsd_stats_total",
-    "vlsd_host_species",
+    "vlsd_host_species", "vls_memory", "vls_density_potential",
 )
 
 
@@ -106,6 +106,8 @@ def lib():
         L.vls_record.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.vls_adjoint.argtypes = [vp, C.c_int, P(d), P(d), P(d), P(d)]
         L.vls_set_graph.argtypes = [vp, C.c_int]
+        L.vls_memory.argtypes = [vp, C.c_int, P(i64), P(i64)]
+        L.vls_density_potential.argtypes = [vp, C.c_int, P(d), P(d)]
         L.vlsd_ensemble_create.argtypes = [C.c_int, P(C.c_int), P(P(C.c_char_p)), P(vp)]
         L.vlsd_ensemble_destroy.argtypes = [vp]
         L.vlsd_ensemble_destroy.restype = None
@@ -308,11 +310,28 @@ class Context:
         _check(lib().vls_put_chains(self._h, run, _ptr(h, C.c_int32) if h is not None else None,
                                     _ptr(v) if v is not None else None))
 
+    def memory(self, run=0):
+        """(allocations, bytes) of device memory the run holds"""
+        n, b = C.c_int64(), C.c_int64()
+        _check(lib().vls_memory(self._h, run, C.byref(n), C.byref(b)))
+        return n.value, b.value
+
+    def clear_chains(self, run=0):
+        """Detach every delay line (vls_set_chains with no chains)."""
+        _check(lib().vls_set_chains(self._h, run, 0, None, None, None, None, None, None, None))
+
     def mass_profile(self, run=0):
         """M(<R_r) of the current f"""
         out = np.empty(self.info(run)["nR"])
         _check(lib().vls_mass_profile(self._h, run, _ptr(out)))
         return out
+
+    def density_potential(self, run=0):
+        """(rho[nR], phi[nR]) of the current f: shell densities and potential"""
+        n = self.info(run)["nR"]
+        rho, phi = np.empty(n), np.empty(n)
+        _check(lib().vls_density_potential(self._h, run, _ptr(rho), _ptr(phi)))
+        return rho, phi
 
     def record(self, n_steps, ck_every, run=0):
         _check(lib().vls_record(self._h, run, int(n_steps), int(ck_every)))

@@ -583,6 +583,14 @@ void Solver21::readRestart(const std::string &fnameNoExt, const Params &prm, boo
     rd.tag(kTagSolver21);
     kernelType_ = rd.get<int>();
     kernelMass_ = rd.get<double>();
+    // The reference adds kernelMass to the enclosed mass of the kick and of
+    // the potential energy (vlasovSolver21_spline.hxx:738-742,
+    // vlasovSolver21.hxx:632) and always writes 0 (vlasovSolver21.hxx:542-543);
+    // the device step has no such term, so a file that carries another value
+    // is refused rather than stepped differently.
+    if (kernelMass_ != 0.0)
+      throw Failure("ERROR: restart holds a nonzero kernelMass, which this solver does not model",
+                    VLS_ERR_IO);
     kernelBufferSize_ = rd.get<int>();
     rd.skip(256 * 8);
     setupFeet(nullptr);  // feet are recomputed; chains come from the file

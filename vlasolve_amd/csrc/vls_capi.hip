@@ -39,6 +39,7 @@ T *dalloc(vls::Run &r, size_t n) {
   if (n == 0) n = 1;
   VLS_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
   r.allocs.push_back(p);
+  r.allocBytes.push_back(n * sizeof(T));
   return static_cast<T *>(p);
 }
 
@@ -49,6 +50,7 @@ void dfree(vls::Run &r, T *&p) {
   for (size_t i = 0; i < r.allocs.size(); i++)
     if (r.allocs[i] == (void *)p) {
       r.allocs.erase(r.allocs.begin() + i);
+      r.allocBytes.erase(r.allocBytes.begin() + i);
       break;
     }
   VLS_HIP_CHECK(hipFree((void *)p));
@@ -449,6 +451,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.Mx = dalloc<double>(r, r.nR);
       r.acc = dalloc<double>(r, r.nR);
       r.mTot = dalloc<double>(r, 1);
+      r.rhoPhi = dalloc<double>(r, 2 * (size_t)r.nR);
       r.st_part = dalloc<double>(r, 7 * (size_t)r.nJ * r.nR);
       r.st_q = dalloc<double>(r, 6 * (size_t)r.nJ + r.nR);
       r.st_out = dalloc<double>(r, 14);
@@ -580,6 +583,7 @@ int vls_set_drift(vls_ctx *ctx, int run, const double *xr, const double *xu, con
     r.hxr.swap(xf);
     r.hxu.swap(uf);
     r.adj.tilesValid = false;  // the adjoint's transposed foot table follows the feet
+    r.adj.recorded = false;    // a recording made with other feet is void
     vls::mark_chains(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
@@ -610,8 +614,20 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
       off[k] = cells;
       cells += len[k];
     }
-    // replace any previous chain arrays
+    // replace any previous chain arrays (after the work that may still read
+    // them); a recording made with other delay lines is void
     hipStream_t s = ctx->c.stream;
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    drop_graph(ctx->c);
+    dfree(r, r.ch_point);
+    dfree(r, r.ch_off);
+    dfree(r, r.ch_len);
+    dfree(r, r.ch_head);
+    dfree(r, r.ch_xr);
+    dfree(r, r.ch_xu);
+    dfree(r, r.ch_out);
+    dfree(r, r.ch_vals);
+    r.adj.recorded = false;
     r.nChains = nChains;
     r.nCells = cells;
     r.ch_point = dalloc<int64_t>(r, nChains);
@@ -636,8 +652,14 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     } else {
       h2d(r.ch_vals, vals, cells, s);
     }
-    drop_graph(ctx->c);  // the delay-line tables moved
-    if (r.have_drift) vls::mark_chains(r, s);
+    if (r.have_drift) {
+      // clear the marks of any earlier delay lines, then attach these
+      std::vector<double> ft(2 * r.nVal);
+      for (int64_t i = 0; i < r.nVal; i++) ft[2 * i] = r.hxr[i], ft[2 * i + 1] = r.hxu[i];
+      h2d(r.feet, ft.data(), 2 * r.nVal, s);
+      vls::mark_chains(r, s);
+      VLS_HIP_CHECK(hipStreamSynchronize(s));
+    }
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
     r.have_chains = true;
@@ -758,6 +780,18 @@ int vls_run_info(vls_ctx *ctx, int run, int64_t out[6]) {
   });
 }
 
+int vls_memory(vls_ctx *ctx, int run, int64_t *count, int64_t *bytes) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (count) *count = (int64_t)r.allocs.size();
+    if (bytes) {
+      int64_t b = 0;
+      for (size_t x : r.allocBytes) b += (int64_t)x;
+      *bytes = b;
+    }
+  });
+}
+
 int vls_put_chains(vls_ctx *ctx, int run, const int32_t *head, const double *vals) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
@@ -788,6 +822,24 @@ int vls_mass_profile(vls_ctx *ctx, int run, double *M) {
     vls::launch_moment(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipMemcpyAsync(M, r.Mx, r.nR * sizeof(double), hipMemcpyDeviceToHost, s));
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_density_potential(vls_ctx *ctx, int run, double *rho, double *phi) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!rho && !phi) throw vls::Error(VLS_ERR_ARG, "null output");
+    if (!r.have_f) throw vls::Error(VLS_ERR_STATE, "vls_density_potential: f not set");
+    hipStream_t s = ctx->c.stream;
+    vls::launch_moment(r, s);
+    vls::launch_density_potential(r, r.rhoPhi, r.rhoPhi + r.nR, s);
+    VLS_HIP_CHECK(hipGetLastError());
+    if (rho)
+      VLS_HIP_CHECK(hipMemcpyAsync(rho, r.rhoPhi, r.nR * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (phi)
+      VLS_HIP_CHECK(hipMemcpyAsync(phi, r.rhoPhi + r.nR, r.nR * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
     VLS_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -824,7 +876,12 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
     }
     // the chain-sized arrays follow the current delay lines
     const bool chainsChanged = A.cells != (int64_t)nc || A.chains != (int64_t)nch;
+    if (chainsChanged || A.capCk < nCk || A.capSeg < seg)
+      VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier adjoint work may still use them
     if (chainsChanged) {
+      dfree(r, A.ringb);
+      dfree(r, A.endVals);
+      dfree(r, A.endHead);
       A.ringb = dalloc<double>(r, nc);
       A.endVals = dalloc<double>(r, nc);
       A.endHead = dalloc<int32_t>(r, nch);
@@ -832,12 +889,17 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.chains = nch;
     }
     if (A.capCk < nCk || chainsChanged) {
+      dfree(r, A.ckF);
+      dfree(r, A.ckVals);
+      dfree(r, A.ckHead);
       A.ckF = dalloc<double>(r, nv * nCk);
       A.ckVals = dalloc<double>(r, nc * nCk);
       A.ckHead = dalloc<int32_t>(r, nch * nCk);
       A.capCk = nCk;
     }
     if (A.capSeg < seg) {
+      dfree(r, A.segC);
+      dfree(r, A.segAcc);
       A.segC = dalloc<double>(r, coef * seg);
       A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
       A.capSeg = seg;
@@ -862,6 +924,7 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
       A.keptFrom = (nCk - keepSegs) * ckEvery;
       const int kept = nSteps - A.keptFrom;
       if (kept > A.capKeep) {
+        VLS_HIP_CHECK(hipStreamSynchronize(s));
         dfree(r, A.keepC);
         dfree(r, A.keepAcc);
         A.keepC = dalloc<double>(r, coef * kept);
@@ -914,6 +977,9 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
     vls::Run &r = run_of(ctx, run);
     auto &A = r.adj;
     if (!A.recorded) throw vls::Error(VLS_ERR_STATE, "vls_adjoint: no recorded run (vls_record)");
+    if (A.cells != r.nCells || A.chains != r.nChains || !A.tilesValid)
+      throw vls::Error(VLS_ERR_STATE,
+                       "vls_adjoint: the delay lines or feet changed since vls_record");
     if (!grad_f0) throw vls::Error(VLS_ERR_ARG, "null gradient output");
     hipStream_t s = ctx->c.stream;
     const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;

@@ -74,6 +74,7 @@ struct Run {
   // moment
   double *P = nullptr;      // [nJ][nR] per-slice partials
   double *dM = nullptr, *Mx = nullptr, *acc = nullptr, *mTot = nullptr;
+  double *rhoPhi = nullptr;  // [2][nR]: shell density and potential (vls_density_potential)
   // tables
   double *R_Val = nullptr, *U_Val = nullptr, *J_Val = nullptr;
   double *dR = nullptr, *dU = nullptr, *Ct1 = nullptr, *gfac = nullptr;
@@ -116,6 +117,7 @@ struct Run {
 
   bool have_f = false, have_drift = false, have_chains = false, stepped = false;
   std::vector<void *> allocs;
+  std::vector<size_t> allocBytes;  // size of each entry of allocs
 };
 
 struct Context {
@@ -150,6 +152,8 @@ size_t kick_scratch_doubles(const Run &r);
 void launch_drift(Run &r, hipStream_t s);
 void mark_chains(Run &r, hipStream_t s);  // attach the delay lines to their points
 void launch_stats(Run &r, hipStream_t s);
+// rho[nR], phi[nR] (device) from the moment and enclosed mass of the last launch_moment
+void launch_density_potential(Run &r, double *rho, double *phi, hipStream_t s);
 
 void time_mark(Timing *t, hipStream_t s, const char *name);
 

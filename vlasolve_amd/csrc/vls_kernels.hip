@@ -1269,6 +1269,28 @@ k_stats_final(const double *__restrict__ part, const double *__restrict__ R_Val,
   }
 }
 
+// Shell density and potential from the moment (the definitions of
+// vls_density_potential): rho[k] = dM[k] / (vol (R[k+1]^3 - R[k]^3)) with
+// vol = 4 pi / 3, and phi[i] = -G (Mx[i] / R[i] + S[i]), S the suffix sum of
+// dM[k] / rbar[k] over the shells k >= i, rbar[k] = (R[k] + R[k+1]) / 2,
+// added from the outermost shell inwards by one thread.
+__global__ void __launch_bounds__(kBlock)
+k_density_potential(const double *__restrict__ R, const double *__restrict__ dM,
+                    const double *__restrict__ Mx, double G, double vol, double *__restrict__ rho,
+                    double *__restrict__ phi, int nR) {
+  for (int k = threadIdx.x; k < nR; k += blockDim.x)
+    rho[k] = k < nR - 1
+                 ? dM[k] / (vol * (R[k + 1] * R[k + 1] * R[k + 1] - R[k] * R[k] * R[k]))
+                 : 0.0;
+  if (threadIdx.x == 0) {
+    double S = 0.0;
+    for (int i = nR - 1; i >= 0; i--) {
+      if (i < nR - 1) S = dM[i] / (0.5 * (R[i] + R[i + 1])) + S;
+      phi[i] = -(G * (Mx[i] / R[i] + S));
+    }
+  }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -1580,6 +1602,11 @@ void mark_chains(Run &r, hipStream_t s) {
   if (r.nChains == 0) return;
   hipLaunchKernelGGL(k_mark_chains, dim3(blocks_for(r.nChains)), dim3(kBlock), 0, s,
                      (double2 *)r.feet, r.ch_point, r.nChains);
+}
+
+void launch_density_potential(Run &r, double *rho, double *phi, hipStream_t s) {
+  hipLaunchKernelGGL(k_density_potential, dim3(1), dim3(kBlock), 0, s, r.R_Val, r.dM, r.Mx, r.sp.G,
+                     (4.0 * M_PI) / 3.0, rho, phi, r.nR);
 }
 
 void launch_stats(Run &r, hipStream_t s) {
