@@ -10,13 +10,19 @@ phase-space points.  A step is one full makeOneStep (moment, radial scan,
 kick spline + evaluation, drift spline + evaluation, delay lines), with f
 resident in HBM.  value = points x K / wall seconds of K steps.
 
-Prints one JSON line with, besides the throughput, a roofline object for the
-dominant kernel (algorithmic bytes per launch over its average duration,
-measured with HIP events on the solver's stream over a second run of the same
-number of steps, since the events themselves cost ~20 us a step) and a
+Prints one JSON line with, besides the throughput, a roofline object: for
+the dominant kernel, its algorithmic bytes per launch over its average
+duration, measured with HIP events on the solver's stream placed just before
+and just after each of its launches (one kernel timed per pass, the other
+kernels undisturbed), the per-step time of every kernel measured the same
+way, and the whole step's algorithmic bytes over ms_per_step; a
 cpu_baseline object: the reference executable, built from its own sources
-into oracle/_ref, timed on this host's cores on a bounded sample of the same
-workload (or the oracle, as a port, if the reference was not built).
+into oracle/_ref, timed on this host's cores (all of this process's CPUs, at
+most 16, and one) on a bounded sample of the same workload (or the oracle,
+as a port, if the reference was not built), with nproc and the lscpu
+summary; and the other workloads' legs: the PLDF 129x129x64 adjoint/forward
+cost ratio, the 32-run ensemble speedup, and the Hernquist 513x513x256
+forward throughput and adjoint/forward cost ratio.
 """
 from __future__ import annotations
 
@@ -63,6 +69,11 @@ def algorithmic_bytes(name, nR, nU, nJ):
     return table.get(name)
 
 
+# The kernels of one step of the workload, as vls_set_timing_kernel names them
+# ("moment" covers the moment partials, the chunk sums and the scan).
+STEP_KERNELS = ("moment", "kick_line", "spline_r", "spline_v", "drift")
+STEP_BYTES_PER_POINT = 72
+
 # The adjoint's workload: PLDF power-law sphere (n=1, alpha=1, eta=0.1, M=1,
 # smoothing 0.5) on 129 log-r x 129 v x 64 quadratic-j nodes, dt=0.005,
 # 2000 steps, checkpoints every 50 steps.
@@ -103,8 +114,8 @@ def adjoint_ratio(V):
 
 # The large adjoint workload: the Hernquist sphere (M=1, R0=1) on its default
 # ranges, 513 log-r x 513 v x 256 log-j nodes (67,371,264 points), dt=0.0025,
-# 4000 steps with checkpoints every 63 steps (about sqrt(4000)).  Opt-in
-# (--hernquist-adjoint): about a minute of GPU time and 80 GB of HBM.
+# 4000 steps with checkpoints every 63 steps (about sqrt(4000)): about 1.5
+# minutes of GPU time and 80 GB of HBM (--no-hernquist skips it).
 HQ_WORKLOAD = {"init.type": "Hernquist", "init.M": 1, "init.R0": 1, "grid.Pres": 512,
                "grid.Ures": [512, 256], "solver.dt": 0.0025}
 HQ_STEPS, HQ_CK = 4000, 63
@@ -196,6 +207,8 @@ def ensemble_speedup(V):
 
 
 def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    at most 16 (the CPU share of one GPU on a shared box)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -203,28 +216,72 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(steps: int):
+def host_description():
+    """nproc and the lscpu summary of the host that runs the baseline."""
+    out = {"nproc": None, "affinity": None, "model": None, "sockets": None,
+           "cores_per_socket": None, "threads_per_core": None}
+    try:
+        out["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout)
+    except (OSError, ValueError):
+        pass
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core"}
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                out[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except OSError:
+        pass
+    return out
+
+
+def _reference_rate(steps, threads):
+    from oracle import oracle as O
+
+    cfg = dict(WORKLOAD)
+    cfg.update({"solver.Tmax": 0.005 * steps, "solver.statsEvery": -1,
+                "solver.snapshotEvery": -1, "solver.restartEvery": -1})
+    with tempfile.TemporaryDirectory() as wd:
+        env_bind = os.environ.get("OMP_PROC_BIND")
+        os.environ["OMP_PROC_BIND"] = "close"
+        try:
+            out = O.run_reference(cfg, wd, threads=threads, timeout=900)
+        finally:
+            if env_bind is None:
+                os.environ.pop("OMP_PROC_BIND", None)
+            else:
+                os.environ["OMP_PROC_BIND"] = env_bind
+    m = re.search(r"Computed (\d+) timesteps in ([0-9.eE+-]+)s", out)
+    if not m:
+        raise RuntimeError("cannot parse the reference's timing line")
+    return int(m.group(1)), float(m.group(2))
+
+
+def cpu_baseline(steps: int, steps_1core: int):
     """Reference executable (or the oracle) on a bounded sample: the same
-    grid and initial conditions, `steps` steps, stats/snapshots/restarts off."""
+    grid and initial conditions, stats/snapshots/restarts off, on all the
+    host threads this process may use (at most 16) and on one."""
     from oracle import oracle as O
 
     threads = cpu_threads()
     npts = 201 * 201 * 200
-    cfg = dict(WORKLOAD)
-    cfg.update({"solver.Tmax": 0.005 * steps, "solver.statsEvery": -1,
-                "solver.snapshotEvery": -1, "solver.restartEvery": -1})
+    host = host_description()
     if O.REF_EXE.exists():
-        with tempfile.TemporaryDirectory() as wd:
-            out = O.run_reference(cfg, wd, threads=threads, timeout=900)
-        m = re.search(r"Computed (\d+) timesteps in ([0-9.eE+-]+)s", out)
-        if not m:
-            raise RuntimeError("cannot parse the reference's timing line")
-        nsteps, secs = int(m.group(1)), float(m.group(2))
+        nsteps, secs = _reference_rate(steps, threads)
+        n1, s1 = _reference_rate(steps_1core, 1)
         return {"value": npts * nsteps / secs, "unit": "point-updates/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"reference vlasolve built from its sources (clang -O3, OpenMP), "
-                          f"same 201x201x200 UDF grid, {nsteps} steps in {secs:.2f} s "
-                          f"(its own step-loop timer), OMP_NUM_THREADS={threads}"}
+                "kind": "reference", "value_1core": npts * n1 / s1, "host": host,
+                "sample": f"reference vlasolve built from its sources (clang -O3, OpenMP, "
+                          f"OMP_PROC_BIND=close), same 201x201x200 UDF grid: {nsteps} steps in "
+                          f"{secs:.2f} s at OMP_NUM_THREADS={threads}, {n1} steps in {s1:.2f} s "
+                          f"at 1 thread (its own step-loop timer)"}
     os.environ["OMP_NUM_THREADS"] = str(threads)
     orc = O.Oracle(WORKLOAD, nthreads=threads)
     orc.step(1)
@@ -232,7 +289,7 @@ def cpu_baseline(steps: int):
     orc.step(steps)
     secs = time.perf_counter() - t0
     return {"value": npts * steps / secs, "unit": "point-updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host,
             "sample": f"CPU oracle (C restatement, OpenMP), same 201x201x200 UDF grid, "
                       f"{steps} steps in {secs:.2f} s, {threads} threads"}
 
@@ -263,8 +320,10 @@ def main():
                     help="steps of the CPU baseline sample (default 150, ~10-20 s of CPU work)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost ratio")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-run ensemble")
-    ap.add_argument("--hernquist-adjoint", action="store_true",
-                    help="also the adjoint ratio on the 513x513x256 Hernquist grid (~1 min)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=12,
+                    help="steps of the one-thread CPU baseline sample")
+    ap.add_argument("--no-hernquist", action="store_true",
+                    help="skip the 513x513x256 Hernquist forward and adjoint leg (~1.5 min)")
     ap.add_argument("--hq-steps", type=int, default=HQ_STEPS,
                     help="steps of the Hernquist adjoint leg (shorter runs for profiling)")
     ap.add_argument("--hq-ck", type=int, default=HQ_CK, help="its checkpoint interval")
@@ -287,33 +346,40 @@ def main():
     ctx.synchronize()
     wall = time.perf_counter() - t0
     value = npts * args.steps / wall
-    # Per-kernel durations: the same number of steps again with a HIP event
-    # recorded on the solver's stream between consecutive kernels.  The events
-    # cost about 20 us a step, so they stay out of the timed steps above.
-    ctx.set_timing(True)
-    ctx.synchronize()
-    solver.step(args.steps)
-    ctx.synchronize()
-    times = ctx.kernel_times(cap=16 * (args.steps + 1))
+    # Per-kernel durations: for each kernel of the step, the same number of
+    # steps again with a HIP event on the solver's stream just before and
+    # just after each of its launches (the other kernels run as in the timed
+    # steps above).
+    per = {}
+    for name in STEP_KERNELS:
+        ctx.set_timing(True, kernel=name)
+        ctx.synchronize()
+        solver.step(args.steps)
+        ctx.synchronize()
+        times = ctx.kernel_times(cap=2 * (args.steps + 1))
+        if times:
+            per[name] = (sum(ms for _, ms in times), len(times))
     ctx.set_timing(False)
     if args.calibrate_copy:
         V.copy_probe(args.calibrate_copy << 20)
 
-    per = {}
-    for name, ms in times:
-        tot, cnt = per.get(name, (0.0, 0))
-        per[name] = (tot + ms, cnt + 1)
     dominant = max((k for k in per if algorithmic_bytes(k, 1, 1, 1)), key=lambda k: per[k][0])
     tot, cnt = per[dominant]
     avg_s = tot / cnt / 1e3
     alg = algorithmic_bytes(dominant, solver.nR, solver.nU, solver.nJ)
     achieved = alg / avg_s / 1e9
+    step_alg = STEP_BYTES_PER_POINT * npts
+    step_achieved = step_alg / (wall / args.steps) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_for(dominant), "kernel": dominant,
                 "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": alg,
-                "kernels_us_per_step": {k: round(v[0] / args.steps * 1e3, 2)
-                                        for k, v in sorted(per.items())}}
+                "kernels_us_per_step": {k: round(v[0] / v[1] * 1e3, 2)
+                                        for k, v in sorted(per.items())},
+                "step": {"algorithmic_bytes": step_alg, "achieved": round(step_achieved, 1),
+                         "frac": round(step_achieved / HBM_PEAK_GBS, 4),
+                         "basis": "72 B per point per step (moment 8, kick 16, r-solve 16, "
+                                  "v-solve 16, drift 16) over ms_per_step"}}
 
     adj = None
     if not args.no_adjoint:
@@ -330,13 +396,16 @@ def main():
             ens = {"speedup": None, "error": str(e)[:200]}
 
     hq = None
-    if args.hernquist_adjoint:
-        hq = adjoint_ratio_hernquist(V, args.hq_steps, args.hq_ck)
+    if not args.no_hernquist:
+        try:
+            hq = adjoint_ratio_hernquist(V, args.hq_steps, args.hq_ck)
+        except Exception as e:  # reported, not fatal for the forward figure
+            hq = {"ratio": None, "error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args.cpu_steps or 150)
+            cpu = cpu_baseline(args.cpu_steps or 150, args.cpu_steps_1core)
         except Exception as e:  # reported, not fatal for the GPU figure
             cpu = {"value": None, "error": str(e)[:200]}
 
@@ -354,7 +423,7 @@ def main():
         "ensemble": ens,
     }
     if hq is not None:
-        line["adjoint_hernquist"] = hq
+        line["hernquist"] = hq
     print(json.dumps(line))
 
 

@@ -154,6 +154,23 @@ int vls_download_coef(vls_ctx *ctx, int run, int j, double *coef);
  * the number of kernels recorded.  Enabled with vls_set_timing(ctx, 1). */
 int vls_set_timing(vls_ctx *ctx, int on);
 int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n);
+/* Schedule tuner: times the run's line-solve schedules on its current data
+ * and keeps the fastest -- register lines (shapes that have them), or the
+ * LDS-tiled solvers with the tile size chosen from a set of candidates, or
+ * the streaming solvers, for each direction -- then the fused or two-kernel
+ * kick on a copy of f.  run = -1 tunes every run of the context.  Every
+ * schedule runs the same operations in the same order: the choice changes
+ * the time of a step, never its bits.  Choices are cached per grid shape for
+ * the life of the process; desc (may be NULL) receives one line per run
+ * describing them with the measured times.  No counterpart in the reference,
+ * which runs one model per process with a fixed schedule. */
+int vls_autotune(vls_ctx *ctx, int run, char *desc, int descLen);
+
+/* Restrict the timing to one kernel ("kick_line", "drift", "spline_r",
+ * "spline_v", "moment", "kick"): an event before and after each of its
+ * launches only, so that the other kernels run as without timing; NULL
+ * times every kernel again.  No counterpart in the reference. */
+int vls_set_timing_kernel(vls_ctx *ctx, const char *kernel);
 
 /* Host-side helpers for tests (no device needed).  vls_thomas_table returns
  * the data-independent factors of the natural spline solve of one axis
@@ -223,6 +240,20 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery);
 int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
                 double *grad_f0, double *grad_vals0);
 
+/* The adjoint of every run of an ensemble at once (runs of different grids
+ * allowed): vls_record_batch records all runs for the same nSteps and
+ * ckEvery, each run on a stream of its own so that they execute
+ * concurrently, with the kept steps sharing seven eighths of the free HBM in
+ * proportion to the runs' sizes; vls_adjoint_batch then runs every reverse
+ * sweep concurrently the same way.  The arrays hold one pointer per run
+ * (dL_dM, dL_df, grad_vals0 and their entries may be NULL, as for
+ * vls_adjoint).  Each run's gradient is what vls_record and vls_adjoint give
+ * for that run alone, up to the rounding of the floating-point atomics.  No
+ * counterpart in the reference. */
+int vls_record_batch(vls_ctx *ctx, int nSteps, int ckEvery);
+int vls_adjoint_batch(vls_ctx *ctx, const double *const *dL_dM, const double *const *dL_df,
+                      double *const *grad_f0, double *const *grad_vals0);
+
 /* ---------------------------------------------------------------- layer 2 */
 
 typedef struct vlsd_solver vlsd_solver;
@@ -274,6 +305,20 @@ int vlsd_host_f(vlsd_host *h, double *f);  /* layout of vls_upload_f */
 int vlsd_host_feet(vlsd_host *h, double *driftR, double *driftU, uint8_t *flag);
 int vlsd_host_chains(vlsd_host *h, int64_t *point, int32_t *len, double *cellR, double *cellU,
                      double *val);
+/* Derivative of the initial state in an initial-condition parameter of
+ * species `species`, the grid held fixed: df (layout of vlsd_host_f) and
+ * dvals (layout of vlsd_host_chains' val) are d/dparam of f0 and of the
+ * delay-line values after the renormalisation to the species mass
+ * (vlasovSolver21::renormalize, src/vlasovSolver21.hxx:468-524), written out
+ * by hand from the generators' formulas: UDF "M", "R0", "alpha",
+ * "smoothingRadius" (initGen_UniformDensitySphere.hxx:43-76,158-173); PLDF
+ * "M", "eta" (with R0 = 1/eta following it unless init.R0 is given),
+ * "alpha", "R0", "smoothingRadius" (initGen_PowerLawDensitySphere.hxx:44-83,
+ * 181-189).  With the gradients of vls_adjoint (grad_f0, and grad_vals0 at a
+ * first recorded step whose rings have head 0, as a new run's do) the
+ * gradient in the parameter is <grad_f0, df> + <grad_vals0, dvals>.  No
+ * counterpart in the reference. */
+int vlsd_host_dparam(vlsd_host *h, const char *param, int species, double *df, double *dvals);
 
 #ifdef __cplusplus
 }

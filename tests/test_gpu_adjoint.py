@@ -224,3 +224,48 @@ def test_gradient_wrt_pldf_eta():
     ad = float((g0 * df).sum() + (gv_front * dv_front).sum())
     print(f"dL/deta finite differences {fd:.10e}  adjoint {ad:.10e}")
     assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+def test_autograd_parameter_gradient():
+    """The initial-condition parameter through the torch harness: the PLDF
+    virial ratio eta -> initial state (InitialState, the set-up's
+    derivative) -> run (MassProfileRun, the adjoint) -> mass-profile loss,
+    against central differences of the whole run in eta."""
+    torch = pytest.importorskip("torch")
+    from vlasolve_amd.autograd import InitialState, MassProfileRun
+
+    print("case pldf eta through the autograd harness", flush=True)
+    cfg = dict(PLDF_SMALL, **{"grid.Pres": 32, "grid.Ures": [32, 12]})
+    eta, n, ck, h = 0.1, 6, 3, 1e-6
+    solver = V.Solver(cfg)
+    ctx = solver.context
+    head0, _ = ctx.get_chains()
+    assert not head0.any()  # a new run: the ring layout is front first
+    ctx.step(n)
+    Mt = ctx.mass_profile() * 1.02
+
+    theta = torch.tensor([eta], requires_grad=True)
+    f0, v0 = InitialState().apply(theta, cfg, ["eta"])
+    M = MassProfileRun().apply(f0, v0, ctx, n, ck, head0)
+    loss = ((M - torch.from_numpy(Mt)) ** 2).sum()
+    loss.backward()
+
+    def run_loss(e):
+        s = V.Solver(dict(cfg, **{"init.eta": e}))
+        c = s.context
+        c.step(n)
+        val = float(((c.mass_profile() - Mt) ** 2).sum())
+        s.close()
+        return val
+
+    # the loss is strongly curved in eta: the second-order central difference
+    # at h carries a truncation error of about 1e-6 here, the fourth-order one
+    # (h and 2h combined) none that matters
+    Lp, Lm = run_loss(eta + h), run_loss(eta - h)
+    fd2 = (Lp - Lm) / (2 * h)
+    fd4 = (8 * (Lp - Lm) - (run_loss(eta + 2 * h) - run_loss(eta - 2 * h))) / (12 * h)
+    g = float(theta.grad[0])
+    print(f"dL/deta finite differences {fd2:.10e} (2nd order) {fd4:.10e} (4th order)  "
+          f"harness {g:.10e}")
+    assert abs(fd4 - g) <= 1e-6 * abs(fd4)
+    assert abs(fd2 - g) <= 1e-5 * abs(fd2)

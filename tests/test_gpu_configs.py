@@ -6,10 +6,15 @@
   the streaming solvers of 515-node lines at their real shapes.
 * The seeded 32-run uneven UDF ensemble: every member, stepped inside the
   ensemble's graph, against its own oracle run, bit for bit.
-* UDF 201 x 201 x 200 for 1000 steps against the reference executable
-  (fixture tests/golden/udf_c2_1000.npz, written by make_golden_long.py):
-  statistics every 25 steps to the reference's printed precision, and f,
-  the mass profile, density and potential at the end to 1e-11.
+* UDF 201 x 201 x 200 for 1000 steps against the reference executable at
+  one OpenMP thread (fixture tests/golden/udf_c2_1000.npz, written by
+  make_golden_long.py): statistics every 25 steps to the reference's printed
+  precision, and f, the mass profile, density and potential at the end to
+  1e-11.  The reference's result depends on its thread count (it sums the
+  moment in per-thread chunks): at 8 threads its final f differs from its
+  own one-thread f by about 1.5e-11 of max |f| in the slices near j = 0;
+  the fixture records that floor and the test prints it beside the GPU's
+  difference.
 * Density and potential (vls_density_potential) bit for bit against the
   oracle's restatement applied to the oracle's moment.
 """
@@ -99,11 +104,16 @@ def test_udf_201_1000_steps_against_reference():
     print(f"case udf 201x201x200, {nsteps} steps vs the reference", flush=True)
     gpu = V.Solver(UDF_C2)
     ctx = gpu.context
+    # the reference writes statistics before steps 0, 25, ..., 975 (none
+    # after the last step)
     rows = {int(round(r[1])): r for r in z["stats"]}
+    assert sorted(rows) == list(range(0, nsteps, 25))
     worst = 0.0
     for k in range(0, nsteps + 1, 25):
         if k:
             gpu.step(25)
+        if k not in rows:
+            continue
         s = gpu.stats_total()
         ref = rows[k]
         for i, key in enumerate(("M", "E", "Ek", "Ep", "S", "L1", "L2")):
@@ -116,7 +126,9 @@ def test_udf_201_1000_steps_against_reference():
         worst = max(worst, d)
     d = np.abs(f.reshape(-1)[z["sample_idx"]] - z["sample_f"]).max() / fmax
     worst = max(worst, d)
-    print(f"  f: max |gpu - reference| / max|f| = {worst:.3e}", flush=True)
+    floor = max(float(z[k]) for k in ("floor_slice0", "floor_slice199", "floor_samples"))
+    print(f"  f: max |gpu - reference| / max|f| = {worst:.3e} (the reference at "
+          f"{int(z['floor_threads'])} threads against itself at 1: {floor:.3e})", flush=True)
     assert worst <= 1e-11
     M = ctx.mass_profile()
     rho, phi = ctx.density_potential()

@@ -31,7 +31,8 @@ EXPORTS = (
     "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
     "vls_record", "vls_adjoint", "vls_set_graph", "vlsd_ensemble_create", "vlsd_ensemble_destroy",
     "vlsd_ensemble_context", "vls_stats_total", "vlsd_species", "vlsd_stats_total",
-    "vlsd_host_species", "vls_memory", "vls_density_potential",
+    "vlsd_host_species", "vls_memory", "vls_density_potential", "vls_set_timing_kernel", "vlsd_host_dparam",
+    "vls_record_batch", "vls_adjoint_batch", "vls_autotune",
 )
 
 
@@ -83,6 +84,7 @@ def lib():
         L.vls_stats_total.argtypes = [vp, C.c_int, P(d), P(d)]
         L.vls_download_coef.argtypes = [vp, C.c_int, C.c_int, P(d)]
         L.vls_set_timing.argtypes = [vp, C.c_int]
+        L.vls_set_timing_kernel.argtypes = [vp, C.c_char_p]
         L.vls_kernel_times.argtypes = [vp, P(C.c_char_p), P(d), P(C.c_int)]
         L.vlsd_create.argtypes = [C.c_int, P(C.c_char_p), P(vp)]
         L.vlsd_destroy.argtypes = [vp]
@@ -105,6 +107,9 @@ def lib():
         L.vls_mass_profile.argtypes = [vp, C.c_int, P(d)]
         L.vls_record.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.vls_adjoint.argtypes = [vp, C.c_int, P(d), P(d), P(d), P(d)]
+        L.vls_record_batch.argtypes = [vp, C.c_int, C.c_int]
+        L.vls_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_int]
+        L.vls_adjoint_batch.argtypes = [vp, vp, vp, vp, vp]
         L.vls_set_graph.argtypes = [vp, C.c_int]
         L.vls_memory.argtypes = [vp, C.c_int, P(i64), P(i64)]
         L.vls_density_potential.argtypes = [vp, C.c_int, P(d), P(d)]
@@ -240,6 +245,7 @@ class Context:
     def __init__(self, handle=None, owner=None, geometries=None, params=None):
         self._owner = owner
         self._own = False
+        self.n_runs = getattr(owner, "n", 1) if handle is not None else len(geometries)
         if handle is not None:
             self._h = C.c_void_p(handle)
             return
@@ -287,7 +293,10 @@ class Context:
         _check(lib().vls_stats(self._h, run, _ptr(out)))
         return dict(zip(STAT_NAMES, out))
 
-    def set_timing(self, on=True):
+    def set_timing(self, on=True, kernel=None):
+        """HIP-event timing of the next vls_step calls: every kernel, or only
+        the launches of `kernel` (an event before and after each)"""
+        _check(lib().vls_set_timing_kernel(self._h, kernel.encode() if kernel else None))
         _check(lib().vls_set_timing(self._h, int(bool(on))))
 
     def info(self, run=0):
@@ -348,6 +357,43 @@ class Context:
                                  _ptr(gf) if gf is not None else None, _ptr(g0), _ptr(gv)))
         return g0, gv
 
+    def autotune(self, run=-1):
+        """vls_autotune: time and keep the fastest line-solve and kick
+        schedules of a run (-1: every run); returns their description"""
+        buf = C.create_string_buffer(8192)
+        _check(lib().vls_autotune(self._h, int(run), buf, len(buf)))
+        return buf.value.decode()
+
+    def record_batch(self, n_steps, ck_every):
+        """vls_record_batch: every run recorded together"""
+        _check(lib().vls_record_batch(self._h, int(n_steps), int(ck_every)))
+
+    def adjoint_batch(self, dL_dM=None, dL_df=None):
+        """[(grad_f0, grad_vals0) per run] of vls_adjoint_batch; dL_dM and
+        dL_df are lists with one entry (or None) per run"""
+        n = self.n_runs
+        infos = [self.info(k) for k in range(n)]
+        keep = []
+
+        def arr(items, shape_of):
+            if items is None:
+                return None
+            ptrs = (C.c_void_p * n)()
+            for k in range(n):
+                if items[k] is not None:
+                    a = np.ascontiguousarray(items[k], np.float64).reshape(shape_of(k))
+                    keep.append(a)
+                    ptrs[k] = a.ctypes.data
+            return ptrs
+
+        gm = arr(dL_dM, lambda k: (infos[k]["nR"],))
+        gf = arr(dL_df, lambda k: (infos[k]["nJ"], infos[k]["nU"], infos[k]["nR"]))
+        outs = [(np.empty((i["nJ"], i["nU"], i["nR"])), np.empty(i["cells"])) for i in infos]
+        g0 = (C.c_void_p * n)(*[o[0].ctypes.data for o in outs])
+        gv = (C.c_void_p * n)(*[o[1].ctypes.data for o in outs])
+        _check(lib().vls_adjoint_batch(self._h, gm, gf, g0, gv))
+        return outs
+
     def kernel_times(self, cap=4096):
         names = (C.c_char_p * cap)()
         ms = np.zeros(cap)
@@ -385,6 +431,10 @@ class Ensemble:
 
     def set_graph(self, on=True):
         _check(lib().vls_set_graph(self.context._h, int(bool(on))))
+
+    def autotune(self):
+        """the fastest schedule of every run (vls_autotune); its description"""
+        return self.context.autotune(-1)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -446,6 +496,19 @@ class HostSetup:
         out = np.empty(self.shape)
         _check(lib().vlsd_host_f(self._h, _ptr(out)), driver=True)
         return out
+
+    def dparam(self, param, species=0):
+        """(df0, dvals0): derivative of the initial f (layout of f()) and of
+        the delay-line values (layout of chains()["val"]) in an
+        initial-condition parameter, the grid held fixed"""
+        L = lib()
+        L.vlsd_host_dparam.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        df = np.empty(self.shape)
+        dv = np.empty(self.n_cells * self.n_species)
+        _check(L.vlsd_host_dparam(self._h, param.encode(), int(species), _ptr(df), _ptr(dv)),
+               driver=True)
+        return df, dv
 
     def feet(self):
         R = np.empty(self.points)

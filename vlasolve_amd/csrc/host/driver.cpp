@@ -242,6 +242,16 @@ int vlsd_host_feet(vlsd_host *h, double *driftR, double *driftU, uint8_t *flag) 
   });
 }
 
+int vlsd_host_dparam(vlsd_host *h, const char *param, int species, double *df, double *dvals) {
+  return guarded([&] {
+    if (!h || !param) throw vlh::Failure("null argument");
+    std::vector<double> a, b;
+    h->solver.initDerivative(param, species, a, b);
+    if (df) std::copy(a.begin(), a.end(), df);
+    if (dvals) std::copy(b.begin(), b.end(), dvals);
+  });
+}
+
 int vlsd_host_chains(vlsd_host *h, int64_t *point, int32_t *len, double *cellR, double *cellU,
                      double *val) {
   return guarded([&] {

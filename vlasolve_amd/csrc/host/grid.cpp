@@ -194,6 +194,39 @@ class UDF : public InitGen {
     if (res < 1.E-200) res = 1.E-200;
     return res;
   }
+  // d/d(M, R0, alpha, smoothingRadius) of computeF: with z = (R0 - r)/s,
+  // sm = (1 + erf z)/2 and E = C1 exp(-Q/C2) (Q = u^2 + j^2/r^2, times the
+  // gap factor), f = sm E, dsm = exp(-z^2)/sqrt(pi) dz and
+  // d ln E = d ln rho0 - 1.5 d ln sig0^2 + (Q/C2) d ln sig0^2, where
+  // rho0 = 3M/(4 pi R0^3) and sig0^2 = alpha/(5 R0)
+  bool dValueAt(double r, double u, double j, int s, const std::string &param, double &val,
+                double &dval) const override {
+    double dM = 0, dR0 = 0, dA = 0, dS = 0;
+    if (param == "M") dM = 1;
+    else if (param == "R0") dR0 = 1;
+    else if (param == "alpha") dA = 1;
+    else if (param == "smoothingRadius") dS = 1;
+    else return false;
+    val = valueAt(r, u, j, s);
+    if (val <= 1.E-200) {
+      dval = 0;
+      return true;
+    }
+    const double z = (R0[s] - r) / sr[s];
+    const double sm = 0.5 * (1. + std::erf(z));
+    const double Q = u * u + (j * j) / (r * r);
+    double E = C1[s] * std::exp(-Q / C2[s]);
+    if (gap1[s] > 0) {
+      double sm0 = 0.5 * (1. + std::erf((gap0[s] - r) / 0.15));
+      double sm1 = 0.5 * (1. + std::erf(-(gap1[s] - r) / 0.15));
+      E *= (sm0 + sm1);
+    }
+    const double dlnS = dA / alpha[s] - dR0 / R0[s];
+    const double dlnE = dM / M[s] - 3 * dR0 / R0[s] - 1.5 * dlnS + Q / C2[s] * dlnS;
+    const double dsm = std::exp(-z * z) / std::sqrt(M_PI) * (dR0 - z * dS) / sr[s];
+    dval = dsm * E + sm * E * dlnE;
+    return true;
+  }
   double mass(int s) const override { return M[s]; }
 };
 
@@ -202,6 +235,7 @@ class PLDF : public InitGen {
   int nS = 1;
   double M[5] = {}, alpha[5] = {}, eta[5] = {}, n[5] = {}, sr[5] = {}, R0[5] = {};
   double rho0[5] = {}, sigO2[5] = {}, sigR2[5] = {}, C0[5] = {}, C1[5] = {}, C2[5] = {};
+  bool R0given = false;  // init.R0 set; otherwise R0 = 1/eta follows eta
 
  public:
   std::string type() const override { return "PLDF"; }
@@ -217,6 +251,7 @@ class PLDF : public InitGen {
       sr[i] = prm.get<double>("smoothingRadius", "init", 0.5, i);
       R0[i] = 1. / eta[i];
       R0[i] = prm.get<double>("R0", "init", R0[i], i);
+      R0given = prm.isDefined("R0", "init");
       rho0[i] = ((3.0 - n[i]) * M[i]) / (4 * M_PI * std::pow(R0[i], 3.0 - n[i]));
       sigO2[i] = (3.0 - n[i]) / (5.0 - 2 * n[i]) * (M[i] * eta[i]) / (R0[i] * (alpha[i] * 0.5 + 1));
       sigR2[i] = sigO2[i] * 0.5 * alpha[i];
@@ -250,6 +285,41 @@ class PLDF : public InitGen {
         sm * C0[s] * std::exp(-u * u * C1[s] - ((j * j) / (r * r)) * C2[s]) * std::pow(r, -n[s]);
     if (res < 1.E-200) res = 1.E-200;
     return res;
+  }
+  // d/d(M, eta, alpha, R0, smoothingRadius) of computeF (eta moves R0 = 1/eta
+  // too unless init.R0 is given): f = sm E, E = C0 exp(-u^2 C1 - (j/r)^2 C2)
+  // r^-n, with d ln rho0 = dM/M - (3-n) dR0/R0,
+  // d ln sigO2 = dM/M + deta/eta - dR0/R0 - (dalpha/2)/(alpha/2 + 1),
+  // d ln sigR2 = d ln sigO2 + dalpha/alpha, d ln C0 = d ln rho0 - d ln sigR2/2
+  // - d ln sigO2, dC1 = -C1 d ln sigR2, dC2 = -C2 d ln sigO2
+  bool dValueAt(double r, double u, double j, int s, const std::string &param, double &val,
+                double &dval) const override {
+    double dM = 0, dEta = 0, dA = 0, dR0 = 0, dS = 0;
+    if (param == "M") dM = 1;
+    else if (param == "eta") {
+      dEta = 1;
+      if (!R0given) dR0 = -1. / (eta[s] * eta[s]);
+    } else if (param == "alpha") dA = 1;
+    else if (param == "R0") dR0 = 1;
+    else if (param == "smoothingRadius") dS = 1;
+    else return false;
+    val = valueAt(r, u, j, s);
+    if (val <= 1.E-200) {
+      dval = 0;
+      return true;
+    }
+    const double z = (R0[s] - r) / sr[s];
+    const double sm = 0.5 * (1. + std::erf(z));
+    const double jr2 = (j * j) / (r * r);
+    const double E = C0[s] * std::exp(-u * u * C1[s] - jr2 * C2[s]) * std::pow(r, -n[s]);
+    const double dlnRho0 = dM / M[s] - (3.0 - n[s]) * dR0 / R0[s];
+    const double dlnSO = dM / M[s] + dEta / eta[s] - dR0 / R0[s] - (0.5 * dA) / (alpha[s] * 0.5 + 1);
+    const double dlnSR = dlnSO + dA / alpha[s];
+    const double dlnC0 = dlnRho0 - 0.5 * dlnSR - dlnSO;
+    const double dlnE = dlnC0 + u * u * C1[s] * dlnSR + jr2 * C2[s] * dlnSO;
+    const double dsm = std::exp(-z * z) / std::sqrt(M_PI) * (dR0 - z * dS) / sr[s];
+    dval = dsm * E + sm * E * dlnE;
+    return true;
   }
   double mass(int s) const override { return M[s]; }
 };

@@ -231,6 +231,7 @@ void Solver21::init(const Params &prm, const std::string &reload, bool device) {
   setupFeet(&init);
   // vlasovSolver::init renormalizes each species to its mass (src/vlasovSolver.hxx:94-95)
   for (int s = 0; s < sp_.nSpecies; s++) renormalize(init->mass(s), s);
+  initGen_ = std::move(init);
   if (device) upload();
 }
 
@@ -302,13 +303,10 @@ void Solver21::setupFeet(std::unique_ptr<InitGen> *init) {
       }
 }
 
-void Solver21::renormalize(double norm, int sp) {
-  // vlasovSolver21::renormalize(norm, which = sp) (src/vlasovSolver21.hxx:468-524)
-  // with the mass of species sp by quadrature21::integrate
-  // (src/quadrature21.hxx:57-85) summed in the reference's single-thread order
-  if (norm <= 0) return;
+double Solver21::quadratureMass(const double *f, int nS, int sp) const {
+  // quadrature21::integrate (src/quadrature21.hxx:57-85) summed in the
+  // reference's single-thread order
   const Grid21 &g = grid_;
-  const int nS = sp_.nSpecies;
   double mass = 0;
   for (int j = 0; j < g.nJ; j++) {
     double Ct1 = 8 * M_PI * M_PI * g.J_Val[j] * g.dJ[j];
@@ -318,16 +316,66 @@ void Solver21::renormalize(double norm, int sp) {
         if (dU <= 0) continue;
         double dR = g.dR[r];
         if (dR <= 0) continue;
-        const double *c = &f_[g.index(j, u, r) * nS + sp];
+        const double *c = &f[g.index(j, u, r) * nS + sp];
         double avg = ((*c) + c[nS] + c[(int64_t)g.nR * nS] + c[(int64_t)(g.nR + 1) * nS]) * 0.25;
         mass += Ct1 * dU * dR * avg;
       }
   }
+  return mass;
+}
+
+void Solver21::renormalize(double norm, int sp) {
+  // vlasovSolver21::renormalize(norm, which = sp) (src/vlasovSolver21.hxx:468-524)
+  // with the mass of species sp
+  if (norm <= 0) return;
+  const int nS = sp_.nSpecies;
+  double mass = quadratureMass(f_.data(), nS, sp);
   mass = mass + kernelMass_;
   double fac = norm / mass;
   if (verbose) printf("Renormalizing mass : M[%d]=%g -> %g.\n", sp, mass, norm);
   for (size_t i = sp; i < f_.size(); i += nS) f_[i] *= fac;
   for (size_t i = sp; i < chains_.val.size(); i += nS) chains_.val[i] *= fac;
+}
+
+void Solver21::initDerivative(const std::string &param, int s, std::vector<double> &df,
+                              std::vector<double> &dvals) const {
+  if (!initGen_) throw Failure("ERROR: no initial conditions to differentiate (a reloaded run)");
+  const int nS = sp_.nSpecies;
+  if (s < 0 || s >= nS) throw Failure("ERROR: species index out of range");
+  const Grid21 &g = grid_;
+  // the generator's values (before the renormalisation) and their derivatives
+  std::vector<double> raw(g.nVal), draw(g.nVal);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int j = 0; j < g.nJ; j++)
+    for (int u = 0; u < g.nU; u++)
+      for (int r = 0; r < g.nR; r++) {
+        const int64_t i = g.index(j, u, r);
+        if (!initGen_->dValueAt(g.R_Val[r], g.U_Val[u], g.J_Val[j], s, param, raw[i], draw[i]))
+          bad = 1;
+      }
+  if (bad)
+    throw Failure("ERROR: " + initType_ + " initial conditions have no derivative in '" + param + "'");
+  // f0 = raw * fac, fac = M / (mass(raw) + kernelMass): the mass is linear in f
+  const double Q = quadratureMass(raw.data(), 1, 0) + kernelMass_;
+  const double dQ = quadratureMass(draw.data(), 1, 0);
+  const double M = initGen_->mass(s), dM = param == "M" ? 1.0 : 0.0;
+  const double fac = M > 0 ? M / Q : 1.0, dfac = M > 0 ? dM / Q - M * dQ / (Q * Q) : 0.0;
+  df.assign(f_.size(), 0.0);
+  for (int64_t i = 0; i < g.nVal; i++) df[i * nS + s] = draw[i] * fac + raw[i] * dfac;
+  // the delay-line cells hold the generator's values at their positions,
+  // scaled by the same factor; each line's back cell starts at 0
+  const Chains &c = chains_;
+  dvals.assign(c.val.size(), 0.0);
+  for (size_t k = 0; k < c.point.size(); k++) {
+    const int j = (int)(c.point[k] / ((int64_t)g.nU * g.nR));
+    for (int32_t q = 0; q + 1 < c.len[k]; q++) {
+      const int64_t cell = c.off[k] + q;
+      double v = 0, dv = 0;
+      initGen_->dValueAt(c.cellR[cell], c.cellU[cell], g.J_Val[j], s, param, v, dv);
+      dvals[cell * nS + s] = dv * fac + v * dfac;
+    }
+  }
 }
 
 vls_solver_params Solver21::deviceParams() const {

@@ -112,6 +112,15 @@ class InitGen {
   virtual void generate(const Grid21 &g, double *f, int nS) const;
   virtual double mass(int s) const = 0;  // getMass(s)
   virtual int maxSpecies() const { return 1; }
+  // valueAt and its derivative in the parameter `param` of species s ("M",
+  // "R0", ...; the grid held fixed); false when the generator has no such
+  // derivative.  val equals valueAt bit for bit; where valueAt returns its
+  // floor the derivative is 0.
+  virtual bool dValueAt(double r, double u, double j, int s, const std::string &param,
+                        double &val, double &dval) const {
+    (void)r; (void)u; (void)j; (void)s; (void)param; (void)val; (void)dval;
+    return false;
+  }
 };
 
 std::unique_ptr<InitGen> makeInitGen(const std::string &type);
@@ -148,6 +157,13 @@ class Solver21 {
   const std::vector<uint8_t> &driftOut() const { return driftOut_; }
   const Chains &chains() const { return chains_; }
   double kernelRadius() const { return kernelRadius_; }
+  // Derivative of the initial state (f after the renormalisation, layout of
+  // hostF(), and the delay-line values, layout of chains().val) in the
+  // initial-condition parameter `param` of species s, the grid held fixed:
+  // the derivative of valueAt through the renormalisation f <- f M / mass(f)
+  // (vlasovSolver21::renormalize, src/vlasovSolver21.hxx:468-524).
+  void initDerivative(const std::string &param, int s, std::vector<double> &df,
+                      std::vector<double> &dvals) const;
   void run();                       // vlasovSolver::run
   void makeSteps(int n);            // n x makeOneStep, time and stepCount advance
   void statistics(bool reset);      // statistics.txt line
@@ -184,6 +200,11 @@ class Solver21 {
   Chains chains_;
   vls_ctx *ctx_ = nullptr;
   std::string initType_;
+  std::unique_ptr<InitGen> initGen_;  // the generator of the initial state (not on reloads)
+
+  // quadrature21::integrate mass of species sp of an interleaved array of nS
+  // species (the reference's single-thread order), as renormalize sums it
+  double quadratureMass(const double *f, int nS, int sp) const;
 
   void setupFeet(std::unique_ptr<InitGen> *init);  // feet, chains (setup())
   void renormalize(double norm, int s);  // species s

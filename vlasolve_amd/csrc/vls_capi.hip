@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,7 @@ void step_species(vls::Run &r, hipStream_t s, vls::Timing *t) {
     for (int k = 0; k < r.nS; k++) {
       SpeciesPlane sp(r, k);
       vls::launch_spline(r, r.f, s, t);
+      vls::time_begin(t, s, "drift");
       vls::launch_drift(r, s);
       vls::time_mark(t, s, "drift");
     }
@@ -132,6 +134,7 @@ void step_species(vls::Run &r, hipStream_t s, vls::Timing *t) {
   for (int k = 0; k < r.nS; k++) {
     SpeciesPlane sp(r, k);
     vls::launch_spline(r, r.f, s, t);
+    vls::time_begin(t, s, "drift");
     vls::launch_drift(r, s);
     vls::time_mark(t, s, "drift");
   }
@@ -142,11 +145,13 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
   if (r.nS > 1) return step_species(r, s, t);
   if (r.sp.no_linking) {
     vls::launch_spline(r, r.f, s, t);
+    vls::time_begin(t, s, "drift");
     vls::launch_drift(r, s);
     vls::time_mark(t, s, "drift");
   }
   vls::launch_spline_kick(r, s, t);
   vls::launch_spline(r, r.f, s, t);
+  vls::time_begin(t, s, "drift");
   vls::launch_drift(r, s);
   vls::time_mark(t, s, "drift");
   r.stepped = true;
@@ -169,18 +174,22 @@ void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut, bool adva
 // Capture one step of every run of an ensemble: each run's kernels go to a
 // stream of their own, forked from and joined back into the context stream,
 // so that small runs fill the GPU together; replayed once per step.
+// A stream (default priority) and a join event per run.
+void ensure_run_streams(vls::Context &C) {
+  const int n = (int)C.runs.size();
+  for (int k = (int)C.runStreams.size(); k < n; k++) {
+    hipStream_t st;
+    VLS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    C.runStreams.push_back(st);
+    hipEvent_t ev;
+    VLS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    C.joinEv.push_back(ev);
+  }
+}
+
 void build_ensemble_graph(vls::Context &C) {
   const int n = (int)C.runs.size();
-  if ((int)C.runStreams.size() < n) {
-    for (int k = (int)C.runStreams.size(); k < n; k++) {
-      hipStream_t st;
-      VLS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-      C.runStreams.push_back(st);
-      hipEvent_t ev;
-      VLS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      C.joinEv.push_back(ev);
-    }
-  }
+  ensure_run_streams(C);
   if (!C.forkEv) VLS_HIP_CHECK(hipEventCreateWithFlags(&C.forkEv, hipEventDisableTiming));
   hipGraph_t g = nullptr;
   VLS_HIP_CHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
@@ -693,7 +702,7 @@ int vls_step(vls_ctx *ctx, int nSteps) {
       for (auto e : t->ev) (void)hipEventDestroy(e);
       t->ev.clear();
       t->names.clear();
-      vls::time_mark(t, s, "start");
+      vls::time_mark(t, s, "start");  // every kernel: the first interval starts here
     }
     for (auto &r : ctx->c.runs)
       if (!r.have_f || !r.have_drift)
@@ -764,6 +773,35 @@ int vls_set_timing(vls_ctx *ctx, int on) {
   return guarded([&] {
     if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
     ctx->c.timing.on = on != 0;
+  });
+}
+
+int vls_autotune(vls_ctx *ctx, int run, char *desc, int descLen) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    vls::Context &C = ctx->c;
+    if (run < -1 || run >= (int)C.runs.size()) throw vls::Error(VLS_ERR_ARG, "run index out of range");
+    std::string all;
+    for (int k = 0; k < (int)C.runs.size(); k++) {
+      if (run >= 0 && k != run) continue;
+      vls::Run &r = C.runs[k];
+      if (!r.have_f) throw vls::Error(VLS_ERR_STATE, "vls_autotune: f not set");
+      if (!all.empty()) all += "; ";
+      all += vls::autotune(r, C.stream, C.maxLds);
+    }
+    drop_graph(C);  // the captured step holds the old launches
+    VLS_HIP_CHECK(hipStreamSynchronize(C.stream));
+    if (desc && descLen > 0) {
+      strncpy(desc, all.c_str(), (size_t)descLen - 1);
+      desc[descLen - 1] = 0;
+    }
+  });
+}
+
+int vls_set_timing_kernel(vls_ctx *ctx, const char *kernel) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    ctx->c.timing.only = kernel ? kernel : "";
   });
 }
 
@@ -844,130 +882,246 @@ int vls_density_potential(vls_ctx *ctx, int run, double *rho, double *phi) {
   });
 }
 
+namespace {
+// The recording and the reverse sweep of one run, in stages that enqueue on
+// a stream of the caller's choice, so that a batch of runs can interleave
+// them on streams of their own (vls_record_batch, vls_adjoint_batch).
+
+// Checks and every buffer of a recording except the kept steps.
+void record_buffers(vls::Run &r, hipStream_t s, int nSteps, int ckEvery) {
+  if (nSteps < 1 || ckEvery < 1) throw vls::Error(VLS_ERR_ARG, "vls_record: need nSteps, ckEvery >= 1");
+  if (!r.have_f || !r.have_drift)
+    throw vls::Error(VLS_ERR_STATE, "vls_record: f and drift feet must be set first");
+  if (r.sp.no_linking)
+    throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
+  if (r.nS > 1) throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers one species only");
+  // the kick's transpose finds the points of a coefficient row from
+  // n ~ u + a hinv, which needs v nodes on the uniform spline knots (to
+  // 0.01 of a cell; a linear v scale is there to rounding)
+  for (int u = 0; u < r.nU; u++)
+    if (std::fabs((r.hU[u] - r.start_v) * r.hinv_v - u) > 0.01)
+      throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint needs a uniform v grid");
+  auto &A = r.adj;
+  A.recorded = false;
+  const int nCk = (nSteps + ckEvery - 1) / ckEvery;
+  const int seg = std::min(ckEvery, nSteps);
+  const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+  const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+  if (!A.fb) {
+    A.fb = dalloc<double>(r, nv);
+    A.fsb = dalloc<double>(r, nv);
+    A.cb = dalloc<double>(r, coef);
+    A.abp = dalloc<double>(r, (size_t)r.nJ * r.nR);
+    A.dMb = dalloc<double>(r, r.nR);
+    A.Mbar = dalloc<double>(r, r.nR);
+    A.endF = dalloc<double>(r, nv);
+  }
+  // the chain-sized arrays follow the current delay lines
+  const bool chainsChanged = A.cells != (int64_t)nc || A.chains != (int64_t)nch;
+  if (chainsChanged || A.capCk < nCk || A.capSeg < seg)
+    VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier adjoint work may still use them
+  if (chainsChanged) {
+    dfree(r, A.ringb);
+    dfree(r, A.endVals);
+    dfree(r, A.endHead);
+    A.ringb = dalloc<double>(r, nc);
+    A.endVals = dalloc<double>(r, nc);
+    A.endHead = dalloc<int32_t>(r, nch);
+    A.cells = nc;
+    A.chains = nch;
+  }
+  if (A.capCk < nCk || chainsChanged) {
+    dfree(r, A.ckF);
+    dfree(r, A.ckVals);
+    dfree(r, A.ckHead);
+    A.ckF = dalloc<double>(r, nv * nCk);
+    A.ckVals = dalloc<double>(r, nc * nCk);
+    A.ckHead = dalloc<int32_t>(r, nch * nCk);
+    A.capCk = nCk;
+  }
+  if (A.capSeg < seg) {
+    dfree(r, A.segC);
+    dfree(r, A.segAcc);
+    A.segC = dalloc<double>(r, coef * seg);
+    A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
+    A.capSeg = seg;
+  }
+  if (!A.tilesValid) {
+    std::vector<int64_t> off;
+    std::vector<uint32_t> pts;
+    vls::build_drift_tiles(r, r.hxr, r.hxu, off, pts);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
+    dfree(r, A.tileOff);
+    dfree(r, A.tilePts);
+    A.tileOff = dalloc<int64_t>(r, off.size());
+    A.tilePts = dalloc<uint32_t>(r, pts.size());
+    h2d(A.tileOff, off.data(), off.size(), s);
+    h2d(A.tilePts, pts.data(), pts.size(), s);
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    A.tilesValid = true;
+  }
+  A.nSteps = nSteps;
+  A.ckEvery = ckEvery;
+  A.nCk = nCk;
+}
+
+// Device bytes of one kept step: the kick's column splines and acceleration.
+size_t kept_step_bytes(const vls::Run &r) {
+  return ((size_t)r.nJ * r.cs + kCoefPad(r) + (size_t)r.nR) * sizeof(double);
+}
+
+// The trailing segments whose kept steps fit in `budget` bytes (plus what
+// the run already holds for them) are kept by the recorded run itself, so
+// that the reverse sweep need not recompute them (VLS_ADJ_KEEP_STEPS caps
+// the kept steps).
+void record_keep(vls::Run &r, hipStream_t s, size_t budget) {
+  auto &A = r.adj;
+  const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+  const size_t stepB = kept_step_bytes(r);
+  int64_t cap = (int64_t)((budget + (size_t)A.capKeep * stepB) / stepB);
+  if (const char *e = getenv("VLS_ADJ_KEEP_STEPS")) cap = std::min<int64_t>(cap, atoll(e));
+  int keepSegs = 0;
+  while (keepSegs < A.nCk) {
+    const int from = (A.nCk - keepSegs - 1) * A.ckEvery;
+    if (A.nSteps - from > cap) break;
+    keepSegs++;
+  }
+  A.keptFrom = (A.nCk - keepSegs) * A.ckEvery;
+  const int kept = A.nSteps - A.keptFrom;
+  if (kept > A.capKeep) {
+    VLS_HIP_CHECK(hipStreamSynchronize(s));
+    dfree(r, A.keepC);
+    dfree(r, A.keepAcc);
+    A.keepC = dalloc<double>(r, coef * kept);
+    A.keepAcc = dalloc<double>(r, (size_t)r.nR * kept);
+    A.capKeep = kept;
+  }
+}
+
+// Step n of the recorded run: the checkpoint at the start of a segment, then
+// the step (keeping its kick coefficients in the kept segments).
+void record_step(vls::Run &r, hipStream_t s, int n) {
+  auto &A = r.adj;
+  const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+  const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+  if (n % A.ckEvery == 0) {
+    const int k = n / A.ckEvery;
+    d2d(A.ckF + nv * k, r.f, nv, s);
+    d2d(A.ckVals + nc * k, r.ch_vals, nc, s);
+    d2d(A.ckHead + nch * k, r.ch_head, nch, s);
+  }
+  if (n >= A.keptFrom)
+    step_keep(r, s, A.keepC + coef * (n - A.keptFrom), A.keepAcc + (size_t)r.nR * (n - A.keptFrom));
+  else
+    step_once(r, s, nullptr);
+}
+
+void record_end(vls::Run &r, hipStream_t s) {
+  auto &A = r.adj;
+  d2d(A.endF, r.f, r.nVal, s);
+  d2d(A.endVals, r.ch_vals, r.nCells, s);
+  d2d(A.endHead, r.ch_head, r.nChains, s);
+  A.recorded = true;
+}
+
+void adjoint_check(const vls::Run &r) {
+  const auto &A = r.adj;
+  if (!A.recorded) throw vls::Error(VLS_ERR_STATE, "vls_adjoint: no recorded run (vls_record)");
+  if (A.cells != r.nCells || A.chains != r.nChains || !A.tilesValid)
+    throw vls::Error(VLS_ERR_STATE, "vls_adjoint: the delay lines or feet changed since vls_record");
+}
+
+// lambda_N: the loss's derivative in f_N, directly and through M(<r).
+void adjoint_seed(vls::Run &r, hipStream_t s, const double *dL_dM, const double *dL_df) {
+  auto &A = r.adj;
+  if (dL_df)
+    h2d(A.fb, dL_df, r.nVal, s);
+  else
+    vls::launch_fill(A.fb, (int64_t)r.nVal, 0.0, s);
+  if (dL_dM) {
+    h2d(A.Mbar, dL_dM, r.nR, s);
+    vls::launch_adj_mass(r, nullptr, A.Mbar, A.dMb, A.fb, true, s);
+  }
+  vls::launch_fill(A.ringb, (int64_t)r.nCells, 0.0, s);
+}
+
+// The reverse sweep through checkpoint segment sg.
+void adjoint_segment(vls::Run &r, hipStream_t s, int sg) {
+  auto &A = r.adj;
+  const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+  const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
+  const int n0 = sg * A.ckEvery, n1 = std::min(n0 + A.ckEvery, A.nSteps);
+  const bool kept = n0 >= A.keptFrom;
+  if (!kept) {
+    // recompute the segment from its checkpoint, keeping each step's kick
+    // coefficients and acceleration; the state after its last step is not
+    // needed (the next segment has its own checkpoint, and the recorded end
+    // state is restored at the end)
+    d2d(r.f, A.ckF + nv * sg, nv, s);
+    d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
+    d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
+    for (int n = n0; n < n1; n++)
+      step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0), n + 1 < n1);
+  }
+  for (int n = n1 - 1; n >= n0; n--) {
+    const double *cK = kept ? A.keepC + coef * (n - A.keptFrom) : A.segC + coef * (n - n0);
+    const double *accK =
+        kept ? A.keepAcc + (size_t)r.nR * (n - A.keptFrom) : A.segAcc + (size_t)r.nR * (n - n0);
+    vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
+    vls::launch_adj_spline(r, A.cb, A.fsb, s);
+    vls::launch_adj_kick(r, A.fsb, cK, accK, A.cb, A.abp, s);
+    vls::launch_adj_scan(r, A.abp, nullptr, A.dMb, s);
+    vls::launch_adj_kick_cols(r, A.cb, A.fb, A.dMb, s);  // plus the moment's transpose
+  }
+}
+
+// The gradients to the host, and the run back in the state after the
+// recorded run.
+void adjoint_end(vls::Run &r, hipStream_t s, double *grad_f0, double *grad_vals0) {
+  auto &A = r.adj;
+  const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
+  VLS_HIP_CHECK(hipMemcpyAsync(grad_f0, A.fb, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (grad_vals0 && nc)
+    VLS_HIP_CHECK(hipMemcpyAsync(grad_vals0, A.ringb, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+  d2d(r.f, A.endF, nv, s);
+  d2d(r.ch_vals, A.endVals, nc, s);
+  d2d(r.ch_head, A.endHead, nch, s);
+}
+
+size_t free_hbm() {
+  size_t freeB = 0, totalB = 0;
+  VLS_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  return freeB;
+}
+
+// The ensemble's run streams, forked from the context stream.
+void fork_runs(vls::Context &C) {
+  ensure_run_streams(C);
+  if (!C.forkEv) VLS_HIP_CHECK(hipEventCreateWithFlags(&C.forkEv, hipEventDisableTiming));
+  VLS_HIP_CHECK(hipEventRecord(C.forkEv, C.stream));
+  for (size_t k = 0; k < C.runs.size(); k++)
+    VLS_HIP_CHECK(hipStreamWaitEvent(C.runStreams[k], C.forkEv, 0));
+}
+
+void join_runs(vls::Context &C) {
+  for (size_t k = 0; k < C.runs.size(); k++) {
+    VLS_HIP_CHECK(hipEventRecord(C.joinEv[k], C.runStreams[k]));
+    VLS_HIP_CHECK(hipStreamWaitEvent(C.stream, C.joinEv[k], 0));
+  }
+}
+}  // namespace
+
 int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
-    if (nSteps < 1 || ckEvery < 1) throw vls::Error(VLS_ERR_ARG, "vls_record: need nSteps, ckEvery >= 1");
-    if (!r.have_f || !r.have_drift)
-      throw vls::Error(VLS_ERR_STATE, "vls_record: f and drift feet must be set first");
-    if (r.sp.no_linking)
-      throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers linking steps only");
-    if (r.nS > 1) throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint covers one species only");
-    // the kick's transpose finds the points of a coefficient row from
-    // n ~ u + a hinv, which needs v nodes on the uniform spline knots (to
-    // 0.01 of a cell; a linear v scale is there to rounding)
-    for (int u = 0; u < r.nU; u++)
-      if (std::fabs((r.hU[u] - r.start_v) * r.hinv_v - u) > 0.01)
-        throw vls::Error(VLS_ERR_ARG, "vls_record: the adjoint needs a uniform v grid");
     hipStream_t s = ctx->c.stream;
-    auto &A = r.adj;
-    const int nCk = (nSteps + ckEvery - 1) / ckEvery;
-    const int seg = std::min(ckEvery, nSteps);
-    const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
-    const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
-    if (!A.fb) {
-      A.fb = dalloc<double>(r, nv);
-      A.fsb = dalloc<double>(r, nv);
-      A.cb = dalloc<double>(r, coef);
-      A.abp = dalloc<double>(r, (size_t)r.nJ * r.nR);
-      A.dMb = dalloc<double>(r, r.nR);
-      A.Mbar = dalloc<double>(r, r.nR);
-      A.endF = dalloc<double>(r, nv);
-    }
-    // the chain-sized arrays follow the current delay lines
-    const bool chainsChanged = A.cells != (int64_t)nc || A.chains != (int64_t)nch;
-    if (chainsChanged || A.capCk < nCk || A.capSeg < seg)
-      VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier adjoint work may still use them
-    if (chainsChanged) {
-      dfree(r, A.ringb);
-      dfree(r, A.endVals);
-      dfree(r, A.endHead);
-      A.ringb = dalloc<double>(r, nc);
-      A.endVals = dalloc<double>(r, nc);
-      A.endHead = dalloc<int32_t>(r, nch);
-      A.cells = nc;
-      A.chains = nch;
-    }
-    if (A.capCk < nCk || chainsChanged) {
-      dfree(r, A.ckF);
-      dfree(r, A.ckVals);
-      dfree(r, A.ckHead);
-      A.ckF = dalloc<double>(r, nv * nCk);
-      A.ckVals = dalloc<double>(r, nc * nCk);
-      A.ckHead = dalloc<int32_t>(r, nch * nCk);
-      A.capCk = nCk;
-    }
-    if (A.capSeg < seg) {
-      dfree(r, A.segC);
-      dfree(r, A.segAcc);
-      A.segC = dalloc<double>(r, coef * seg);
-      A.segAcc = dalloc<double>(r, (size_t)r.nR * seg);
-      A.capSeg = seg;
-    }
-    // Trailing segments whose kick coefficients fit in 7/8 of the free HBM
-    // (counting the buffer this run already holds for them) are kept by the
-    // recorded run itself, so that the reverse sweep need not recompute them
-    // (all of them for grids like 129x129x64; VLS_ADJ_KEEP_STEPS caps the
-    // kept steps).
-    {
-      size_t freeB = 0, totalB = 0;
-      VLS_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-      const size_t stepB = (coef + (size_t)r.nR) * sizeof(double);
-      int64_t cap = (int64_t)((freeB + (size_t)A.capKeep * stepB) / 8 * 7 / stepB);
-      if (const char *e = getenv("VLS_ADJ_KEEP_STEPS")) cap = std::min<int64_t>(cap, atoll(e));
-      int keepSegs = 0;
-      while (keepSegs < nCk) {
-        const int from = (nCk - keepSegs - 1) * ckEvery;
-        if (nSteps - from > cap) break;
-        keepSegs++;
-      }
-      A.keptFrom = (nCk - keepSegs) * ckEvery;
-      const int kept = nSteps - A.keptFrom;
-      if (kept > A.capKeep) {
-        VLS_HIP_CHECK(hipStreamSynchronize(s));
-        dfree(r, A.keepC);
-        dfree(r, A.keepAcc);
-        A.keepC = dalloc<double>(r, coef * kept);
-        A.keepAcc = dalloc<double>(r, (size_t)r.nR * kept);
-        A.capKeep = kept;
-      }
-    }
-    if (!A.tilesValid) {
-      std::vector<int64_t> off;
-      std::vector<uint32_t> pts;
-      vls::build_drift_tiles(r, r.hxr, r.hxu, off, pts);
-      VLS_HIP_CHECK(hipStreamSynchronize(s));  // earlier tables may still be in use
-      dfree(r, A.tileOff);
-      dfree(r, A.tilePts);
-      A.tileOff = dalloc<int64_t>(r, off.size());
-      A.tilePts = dalloc<uint32_t>(r, pts.size());
-      h2d(A.tileOff, off.data(), off.size(), s);
-      h2d(A.tilePts, pts.data(), pts.size(), s);
-      VLS_HIP_CHECK(hipStreamSynchronize(s));
-      A.tilesValid = true;
-    }
-    for (int n = 0; n < nSteps; n++) {
-      if (n % ckEvery == 0) {
-        const int k = n / ckEvery;
-        d2d(A.ckF + nv * k, r.f, nv, s);
-        d2d(A.ckVals + nc * k, r.ch_vals, nc, s);
-        d2d(A.ckHead + nch * k, r.ch_head, nch, s);
-      }
-      if (n >= A.keptFrom)
-        step_keep(r, s, A.keepC + coef * (n - A.keptFrom),
-                  A.keepAcc + (size_t)r.nR * (n - A.keptFrom));
-      else
-        step_once(r, s, nullptr);
-    }
-    d2d(A.endF, r.f, nv, s);
-    d2d(A.endVals, r.ch_vals, nc, s);
-    d2d(A.endHead, r.ch_head, nch, s);
+    record_buffers(r, s, nSteps, ckEvery);
+    // the kept steps may fill seven eighths of the free HBM
+    record_keep(r, s, free_hbm() / 8 * 7);
+    for (int n = 0; n < nSteps; n++) record_step(r, s, n);
+    record_end(r, s);
     VLS_HIP_CHECK(hipGetLastError());
     VLS_HIP_CHECK(hipStreamSynchronize(s));
-    A.nSteps = nSteps;
-    A.ckEvery = ckEvery;
-    A.nCk = nCk;
-    A.recorded = true;
   });
 }
 
@@ -975,62 +1129,70 @@ int vls_adjoint(vls_ctx *ctx, int run, const double *dL_dM, const double *dL_df,
                 double *grad_f0, double *grad_vals0) {
   return guarded([&] {
     vls::Run &r = run_of(ctx, run);
-    auto &A = r.adj;
-    if (!A.recorded) throw vls::Error(VLS_ERR_STATE, "vls_adjoint: no recorded run (vls_record)");
-    if (A.cells != r.nCells || A.chains != r.nChains || !A.tilesValid)
-      throw vls::Error(VLS_ERR_STATE,
-                       "vls_adjoint: the delay lines or feet changed since vls_record");
+    adjoint_check(r);
     if (!grad_f0) throw vls::Error(VLS_ERR_ARG, "null gradient output");
     hipStream_t s = ctx->c.stream;
-    const size_t nv = r.nVal, nc = r.nCells, nch = r.nChains;
-    const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
-    // lambda_N: the loss's derivative in f_N, directly and through M(<r)
-    if (dL_df)
-      h2d(A.fb, dL_df, nv, s);
-    else
-      vls::launch_fill(A.fb, (int64_t)nv, 0.0, s);
-    if (dL_dM) {
-      h2d(A.Mbar, dL_dM, r.nR, s);
-      vls::launch_adj_mass(r, nullptr, A.Mbar, A.dMb, A.fb, true, s);
-    }
-    vls::launch_fill(A.ringb, (int64_t)nc, 0.0, s);
-    for (int sg = A.nCk - 1; sg >= 0; sg--) {
-      const int n0 = sg * A.ckEvery, n1 = std::min(n0 + A.ckEvery, A.nSteps);
-      const bool kept = n0 >= A.keptFrom;
-      if (!kept) {
-        // recompute the segment from its checkpoint, keeping each step's kick
-        // coefficients and acceleration; the state after its last step is
-        // not needed (the next segment has its own checkpoint, and the
-        // recorded end state is restored below)
-        d2d(r.f, A.ckF + nv * sg, nv, s);
-        d2d(r.ch_vals, A.ckVals + nc * sg, nc, s);
-        d2d(r.ch_head, A.ckHead + nch * sg, nch, s);
-        for (int n = n0; n < n1; n++)
-          step_keep(r, s, A.segC + coef * (n - n0), A.segAcc + (size_t)r.nR * (n - n0),
-                    n + 1 < n1);
-      }
-      // reverse through the segment
-      for (int n = n1 - 1; n >= n0; n--) {
-        const double *cK = kept ? A.keepC + coef * (n - A.keptFrom) : A.segC + coef * (n - n0);
-        const double *accK = kept ? A.keepAcc + (size_t)r.nR * (n - A.keptFrom)
-                                  : A.segAcc + (size_t)r.nR * (n - n0);
-        vls::launch_adj_drift(r, A.fb, A.cb, A.ringb, A.ckHead + nch * sg, n - n0, s);
-        vls::launch_adj_spline(r, A.cb, A.fsb, s);
-        vls::launch_adj_kick(r, A.fsb, cK, accK, A.cb, A.abp, s);
-        vls::launch_adj_scan(r, A.abp, nullptr, A.dMb, s);
-        vls::launch_adj_kick_cols(r, A.cb, A.fb, A.dMb, s);  // plus the moment's transpose
-      }
-    }
+    adjoint_seed(r, s, dL_dM, dL_df);
+    for (int sg = r.adj.nCk - 1; sg >= 0; sg--) adjoint_segment(r, s, sg);
     VLS_HIP_CHECK(hipGetLastError());
-    VLS_HIP_CHECK(hipMemcpyAsync(grad_f0, A.fb, nv * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (grad_vals0 && nc)
-      VLS_HIP_CHECK(hipMemcpyAsync(grad_vals0, A.ringb, nc * sizeof(double),
-                                   hipMemcpyDeviceToHost, s));
-    // back to the state after the recorded run
-    d2d(r.f, A.endF, nv, s);
-    d2d(r.ch_vals, A.endVals, nc, s);
-    d2d(r.ch_head, A.endHead, nch, s);
+    adjoint_end(r, s, grad_f0, grad_vals0);
     VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_record_batch(vls_ctx *ctx, int nSteps, int ckEvery) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    vls::Context &C = ctx->c;
+    const int nRuns = (int)C.runs.size();
+    ensure_run_streams(C);
+    for (int k = 0; k < nRuns; k++) record_buffers(C.runs[k], C.runStreams[k], nSteps, ckEvery);
+    // seven eighths of the free HBM for kept steps, shared in proportion to
+    // the runs' step sizes, so that every run keeps the same trailing steps
+    const size_t budget = free_hbm() / 8 * 7;
+    size_t total = 0;
+    for (auto &r : C.runs) total += kept_step_bytes(r);
+    for (int k = 0; k < nRuns; k++)
+      record_keep(C.runs[k], C.runStreams[k],
+                  (size_t)((double)budget * kept_step_bytes(C.runs[k]) / (double)total));
+    VLS_HIP_CHECK(hipStreamSynchronize(C.stream));
+    fork_runs(C);
+    for (int n = 0; n < nSteps; n++)
+      for (int k = 0; k < nRuns; k++) record_step(C.runs[k], C.runStreams[k], n);
+    for (int k = 0; k < nRuns; k++) record_end(C.runs[k], C.runStreams[k]);
+    join_runs(C);
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipStreamSynchronize(C.stream));
+  });
+}
+
+int vls_adjoint_batch(vls_ctx *ctx, const double *const *dL_dM, const double *const *dL_df,
+                      double *const *grad_f0, double *const *grad_vals0) {
+  return guarded([&] {
+    if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
+    vls::Context &C = ctx->c;
+    const int nRuns = (int)C.runs.size();
+    if (!grad_f0) throw vls::Error(VLS_ERR_ARG, "null gradient outputs");
+    int nCk = -1;
+    for (int k = 0; k < nRuns; k++) {
+      adjoint_check(C.runs[k]);
+      if (!grad_f0[k]) throw vls::Error(VLS_ERR_ARG, "null gradient output");
+      if (nCk >= 0 && C.runs[k].adj.nCk != nCk)
+        throw vls::Error(VLS_ERR_STATE, "vls_adjoint_batch: the runs were not recorded together "
+                                        "(vls_record_batch)");
+      nCk = C.runs[k].adj.nCk;
+    }
+    fork_runs(C);
+    for (int k = 0; k < nRuns; k++)
+      adjoint_seed(C.runs[k], C.runStreams[k], dL_dM ? dL_dM[k] : nullptr,
+                   dL_df ? dL_df[k] : nullptr);
+    for (int sg = nCk - 1; sg >= 0; sg--)
+      for (int k = 0; k < nRuns; k++) adjoint_segment(C.runs[k], C.runStreams[k], sg);
+    for (int k = 0; k < nRuns; k++)
+      adjoint_end(C.runs[k], C.runStreams[k], grad_f0[k], grad_vals0 ? grad_vals0[k] : nullptr);
+    join_runs(C);
+    VLS_HIP_CHECK(hipGetLastError());
+    VLS_HIP_CHECK(hipStreamSynchronize(C.stream));
   });
 }
 
@@ -1039,12 +1201,15 @@ int vls_kernel_times(vls_ctx *ctx, const char **names, double *ms, int *n) {
     if (!ctx || !n) throw vls::Error(VLS_ERR_ARG, "null argument");
     vls::Timing &t = ctx->c.timing;
     VLS_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
-    int cnt = t.ev.empty() ? 0 : (int)t.ev.size() - 1;
+    // every kernel: consecutive events; one kernel: (before, after) pairs
+    const bool pairs = !t.only.empty();
+    int cnt = pairs ? (int)t.ev.size() / 2 : (t.ev.empty() ? 0 : (int)t.ev.size() - 1);
     int cap = *n;
     for (int k = 0; k < cnt && k < cap; k++) {
+      const int a = pairs ? 2 * k : k, b = a + 1;
       float x = 0;
-      VLS_HIP_CHECK(hipEventElapsedTime(&x, t.ev[k], t.ev[k + 1]));
-      if (names) names[k] = t.names[k + 1];
+      VLS_HIP_CHECK(hipEventElapsedTime(&x, t.ev[a], t.ev[b]));
+      if (names) names[k] = t.names[b];
       if (ms) ms[k] = x;
     }
     *n = cnt;

@@ -41,8 +41,13 @@ struct ThomasAxis {
 void thomas_table(int dim, double hinv, ThomasAxis &ax, std::vector<double> &u,
                   std::vector<double> &d, std::vector<double> &rd);
 
+// Per-kernel timing with HIP events on the stream of the launches.  Every
+// kernel: an event after each launch (consecutive differences).  One kernel
+// (`only` set): an event before and after each of its launches, so that the
+// other kernels run undisturbed.
 struct Timing {
   bool on = false;
+  std::string only;
   std::vector<hipEvent_t> ev;
   std::vector<const char *> names;
   std::vector<double> ms;
@@ -59,6 +64,10 @@ struct Run {
   int TR = 32, LS = 0;  // r-sweep tile: rows per block, odd LDS row stride (>= TR)
   int TC = 32;          // v-sweep tile: columns per block
   bool streamR = false, streamV = false;  // streaming solvers for long lines
+  // schedule choices (vls_autotune): -1 from the shape and the environment;
+  // lineMode 0 / 1: LDS-tiled or streaming solvers / register lines (shapes
+  // that have them); kickMode 0 / 1: two-kernel / fused kick
+  int lineMode = -1, kickMode = -1;
   vls_solver_params sp{};
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
 
@@ -137,8 +146,20 @@ struct Context {
 
 // kernel launchers (vls_kernels.hip)
 void configure_tiles(Run &r, int maxLdsPerBlock);
+// Times the schedules of the run's line solves and kick on its current f
+// (f is left as it was) and keeps the fastest; cached per grid shape.
+// Returns a one-line description of the choice.
+std::string autotune(Run &r, hipStream_t s, int maxLdsPerBlock);
 void launch_moment(Run &r, hipStream_t s);       // partials of f, then the scan
 void launch_moment_scan(Run &r, hipStream_t s);  // scan of the partials in P
+// The same into other buffers: the moment of f (the run's shape and species
+// planes) with partials P [nJ][nR] (overwritten), dM, Mx, acc [nR], mTot [1]
+struct MomentBufs {
+  const double *f;
+  double *P, *dM, *Mx, *acc, *mTot;
+};
+void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s);
+void launch_moment_scan_into(Run &r, const MomentBufs &b, hipStream_t s);
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 // the kick of the plane r.f in column form: v-spline of each column of f
 // into d (slice stride cs, row stride nR), then the evaluation
@@ -154,8 +175,11 @@ void mark_chains(Run &r, hipStream_t s);  // attach the delay lines to their poi
 void launch_stats(Run &r, hipStream_t s);
 // rho[nR], phi[nR] (device) from the moment and enclosed mass of the last launch_moment
 void launch_density_potential(Run &r, double *rho, double *phi, hipStream_t s);
+void launch_density_potential_from(Run &r, const double *dM, const double *Mx, double *rho,
+                                   double *phi, hipStream_t s);
 
-void time_mark(Timing *t, hipStream_t s, const char *name);
+void time_mark(Timing *t, hipStream_t s, const char *name);   // after a launch
+void time_begin(Timing *t, hipStream_t s, const char *name);  // before a launch
 
 // adjoint launchers (vls_adjoint.hip)
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);

@@ -9,7 +9,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 #include "vls_div.hpp"
 #include "vls_internal.hpp"
@@ -271,6 +276,33 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
 // on the pivots are the scalars dS, rdS, uS: the steady loops load nothing
 // but the data stream, kPF rows ahead.
 constexpr int kHead = 16;
+
+// The head's pivots staged in LDS, for the register line kernels whose loops
+// stream their line: a global load of a table entry inside such a loop makes
+// its wait (vmcnt counts every global load and store of the wave, in order)
+// wait for all the line data issued before it, so that each head row would
+// cost a whole memory latency.  head_load issues the loads (lane k takes row
+// k, clamped to the rows a line of M entries has) before the line's own
+// loads; head_store writes them to LDS once they are in, and the block's
+// barrier orders them before the loops' reads.
+struct HeadTab {
+  double u[kHead + 1], d[kHead + 1], rd[kHead + 1];
+};
+struct HeadRegs {
+  double u, d, rd;
+};
+__device__ __forceinline__ HeadRegs head_load(const ThomasAxis &ax, int lane, int lz) {
+  const int last = kHead < ax.M - 2 ? kHead : ax.M - 2;
+  const int k = lane < 1 ? 1 : (lane > last ? last : lane);
+  return {ax.u[k + lz], ax.d[k + lz], ax.rd[k + lz]};
+}
+__device__ __forceinline__ void head_store(HeadTab &t, const HeadRegs &h, int lane) {
+  if (lane >= 1 && lane <= kHead) {
+    t.u[lane] = h.u;
+    t.d[lane] = h.d;
+    t.rd[lane] = h.rd;
+  }
+}
 
 // Forward elimination over rows 1..M-2.  RHS(k) returns rhs_k (any k <= M-2),
 // STORE(k, y) keeps y_k.  Row 1 takes y_0's contribution A03 in place of
@@ -645,9 +677,18 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
   // the line streams in kLA rows ahead of the elimination, which runs while
   // it arrives (issued all at once, the loads would wait on each other)
   constexpr int kLA = M - 2 < 48 ? M - 2 : 48;
+  __shared__ HeadTab htab;
+  const HeadRegs hr = head_load(ax, lane, lz);
   double y[M];
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int k = 1; k <= kLA; k++) y[k] = in[k * is];
+  __builtin_amdgcn_sched_barrier(0);
+  head_store(htab, hr, lane);
+  // the block is one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
   double prev = ax.A03;
 #pragma unroll
@@ -655,8 +696,8 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     if (k + kLA <= M - 2) y[k + kLA] = in[(k + kLA) * is];
     double dv = dS, ev = rdS;
     if (k <= kHead) {
-      dv = ax.d[k + lz];
-      ev = ax.rd[k + lz];
+      dv = htab.d[k];
+      ev = htab.rd[k];
     }
     y[k] = div_known(y[k] - c16 * prev, dv, ev);
     prev = y[k];
@@ -670,7 +711,7 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 #pragma unroll
   for (int k = M - 2; k >= 1; k--) {
     double uv = uS;
-    if (k <= kHead) uv = ax.u[k + lz];
+    if (k <= kHead) uv = htab.u[k];
     y[k] = y[k] - uv * cn;
     cn = y[k];
     if (live) out[k * os] = cn;
@@ -742,6 +783,8 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
 #pragma unroll
   for (int q = 0; q < nUq; q++) uq[q] = U_Val[min(lane + 64 * q, nU - 1)];
   const double a = outLane ? acc[k] : 0.0;
+  __shared__ HeadTab htab;
+  const HeadRegs hr = head_load(ax, lane, lz);
   double *fo = fj + (outLane ? k : 0);
   // KEEP (the adjoint's recorded steps): the coefficients d_m of the column
   // are also stored, to dKeep[j][m][k] (slice stride ds)
@@ -761,6 +804,7 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
 #pragma unroll
   for (int q = 0; q < nUq; q++)
     if (lane + 64 * q < nU) Us[lane + 64 * q] = uq[q];
+  head_store(htab, hr, lane);
   // the ring starts zeroed: a point whose target row is the last knot reads
   // the row past the padded column with weight BV2(0) = 0
 #pragma unroll
@@ -791,8 +835,8 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
     }
     double dv = dS, ev = rdS;
     if (m <= kHead) {
-      dv = ax.d[m + lz];
-      ev = ax.rd[m + lz];
+      dv = htab.d[m];
+      ev = htab.rd[m];
     }
     const double yk = div_known(y[m] - c16 * prev, dv, ev);
     if (m <= kYL)
@@ -928,7 +972,7 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
 #pragma unroll
     for (int m = Mv - 2; m >= 1; m--) {
       double uv = uS;
-      if (m <= kHead) uv = ax.u[m + lz];
+      if (m <= kHead) uv = htab.u[m];
       const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
       c2 = cn;
       cn = ym - uv * cn;
@@ -956,7 +1000,7 @@ k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restric
 #pragma unroll
     for (int m = Mv - 2; m >= 1; m--) {
       double uv = uS;
-      if (m <= kHead) uv = ax.u[m + lz];
+      if (m <= kHead) uv = htab.u[m];
       const double ym = m <= kYL ? ylds[(m - 1) * 64 + lane] : y[m];
       c2 = cn;
       cn = ym - uv * cn;
@@ -1295,8 +1339,7 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBl
 
 }  // namespace
 
-void time_mark(Timing *t, hipStream_t s, const char *name) {
-  if (!t || !t->on) return;
+static void time_record(Timing *t, hipStream_t s, const char *name) {
   hipEvent_t e;
   VLS_HIP_CHECK(hipEventCreate(&e));
   VLS_HIP_CHECK(hipEventRecord(e, s));
@@ -1304,24 +1347,41 @@ void time_mark(Timing *t, hipStream_t s, const char *name) {
   t->names.push_back(name);
 }
 
+void time_mark(Timing *t, hipStream_t s, const char *name) {
+  if (!t || !t->on) return;
+  if (t->only.empty() || t->only == name) time_record(t, s, name);
+}
+
+void time_begin(Timing *t, hipStream_t s, const char *name) {
+  if (t && t->on && !t->only.empty() && t->only == name) time_record(t, s, name);
+}
+
 void launch_moment(Run &r, hipStream_t s) {
+  launch_moment_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot}, s);
+}
+
+void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   if (r.nS > 1)
-    hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, r.f, r.Ct1, r.dU, r.dR, r.P,
+    hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, b.f, r.Ct1, r.dU, r.dR, b.P,
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
   else
     hipLaunchKernelGGL(k_moment_cols, dim3((r.nR + 62) / 63 * r.nJ), dim3(256),
-                       64 * ((r.nU + 63) / 64) * sizeof(double), s, r.f, r.Ct1, r.dU,
-                       r.dR, r.P, r.nR, r.nU);
-  launch_moment_scan(r, s);
+                       64 * ((r.nU + 63) / 64) * sizeof(double), s, b.f, r.Ct1, r.dU,
+                       r.dR, b.P, r.nR, r.nU);
+  launch_moment_scan_into(r, b, s);
 }
 
 void launch_moment_scan(Run &r, hipStream_t s) {
+  launch_moment_scan_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot}, s);
+}
+
+void launch_moment_scan_into(Run &r, const MomentBufs &b, hipStream_t s) {
   const int nC = (r.nJ + kJC - 1) / kJC;
   hipLaunchKernelGGL(k_moment_chunks, dim3(nC, (r.nR + kChunkThreads - 1) / kChunkThreads),
-                     dim3(kChunkThreads), 0, s, r.P, r.nR, r.nJ);
-  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kScanThreads), r.nR * sizeof(double), s, r.P,
-                     r.gfac, r.dM, r.Mx, r.acc, r.mTot, r.nR, r.nJ);
+                     dim3(kChunkThreads), 0, s, b.P, r.nR, r.nJ);
+  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kScanThreads), r.nR * sizeof(double), s, b.P,
+                     r.gfac, b.dM, b.Mx, b.acc, b.mTot, r.nR, r.nJ);
 }
 
 void configure_tiles(Run &r, int maxLdsPerBlock) {
@@ -1360,12 +1420,12 @@ void configure_tiles(Run &r, int maxLdsPerBlock) {
   const size_t ldsV = (size_t)r.TC * (r.nU + 2) * sizeof(double);
   if ((!r.streamR && ldsR > (size_t)maxLdsPerBlock) || (!r.streamV && ldsV > (size_t)maxLdsPerBlock))
     throw Error(VLS_ERR_ARG, "grid too large for the LDS of one workgroup");
-  if (!r.streamR)
-    VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_r,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsR));
-  if (!r.streamV)
-    VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_v,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsV));
+  // the tiled solvers may use all of a workgroup's LDS (runs of an ensemble,
+  // and the schedules vls_autotune tries, need different amounts)
+  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_r,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxLdsPerBlock));
+  VLS_HIP_CHECK(hipFuncSetAttribute((const void *)k_spline_v,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxLdsPerBlock));
 }
 
 // Row version of the register line solver with coalesced memory access.
@@ -1394,6 +1454,8 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   const double *src = f + ((size_t)j * nU + min(u0, nU - 1)) * nIn;
   double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double y[M];
+  __shared__ HeadTab htab;
+  const HeadRegs hr = head_load(ax, lane, lz);
   // chunks of loads in flight: two (the loads of a chunk take a memory
   // latency, the elimination of a chunk far less)
   constexpr int kAh = 2;
@@ -1411,6 +1473,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 #pragma unroll
   for (int ch = 0; ch < kAh; ch++)
     if (ch < nChIn) load_chunk(ch, tmp[ch]);
+  head_store(htab, hr, lane);  // ordered before its reads by the first chunk's barrier
   // Per chunk: stage it (and issue the loads of chunk ch + kAh into the
   // registers just staged), then the elimination.
 #pragma unroll
@@ -1428,8 +1491,8 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
       if (k <= M - 2) {
         double dv = dS, ev = rdS;
         if (k <= kHead) {
-          dv = ax.d[k + lz];
-          ev = ax.rd[k + lz];
+          dv = htab.d[k];
+          ev = htab.rd[k];
         }
         y[k] = div_known(y[k] - c16 * prev, dv, ev);
         prev = y[k];
@@ -1450,7 +1513,7 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
       const int k = ch * kC + i;
       if (k >= 1 && k <= M - 2) {
         double uv = uS;
-        if (k <= kHead) uv = ax.u[k + lz];
+        if (k <= kHead) uv = htab.u[k];
         y[k] = y[k] - uv * cn;
         cn = y[k];
       }
@@ -1505,23 +1568,19 @@ static const LineShape kLineShapes[] = {
 #undef VLS_LINE_SHAPE
 
 static const LineShape *line_shape(const Run &r) {
-  if (getenv("VLS_NO_REGISTER_LINES")) return nullptr;
+  if (r.lineMode == 0 || (r.lineMode < 0 && getenv("VLS_NO_REGISTER_LINES"))) return nullptr;
   for (const LineShape &ls : kLineShapes)
     if (ls.nR == r.nR && ls.nU == r.nU) return &ls;
   return nullptr;
 }
 
-void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
+// The drift's r-direction solves (rows of src -> padded rows of c) and its
+// v-direction solves (in place on c), with the run's schedule.
+static void launch_spline_r(Run &r, const double *src, hipStream_t s) {
   if (const LineShape *ls = line_shape(r)) {
     hipLaunchKernelGGL(ls->rows, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c, r.nR,
                        r.nU, r.nJ, r.cs, 0, r.tr);
-    time_mark(t, s, "spline_r");
-    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, src, r.c, r.nR,
-                       r.nU, r.nJ, r.cs, 0, r.tv);
-    time_mark(t, s, "spline_v");
-    return;
-  }
-  if (r.streamR) {
+  } else if (r.streamR) {
     hipLaunchKernelGGL(k_spline_r_stream, dim3((r.nU + 63) / 64, r.nJ), dim3(64), 0, s, src, r.c,
                        r.nR, r.nU, r.cs, 0, r.tr);
   } else {
@@ -1529,8 +1588,13 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
     hipLaunchKernelGGL(k_spline_r, gr, dim3(64), (size_t)r.LS * (r.nR + 2) * sizeof(double), s,
                        src, r.c, r.nR, r.nU, r.nJ, r.cs, r.TR, r.LS, 0, r.tr);
   }
-  time_mark(t, s, "spline_r");
-  if (r.streamV) {
+}
+
+static void launch_spline_v(Run &r, const double *src, hipStream_t s) {
+  if (const LineShape *ls = line_shape(r)) {
+    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, src, r.c, r.nR,
+                       r.nU, r.nJ, r.cs, 0, r.tv);
+  } else if (r.streamV) {
     hipLaunchKernelGGL(k_spline_v_stream, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, r.c,
                        r.nR, r.cs, 0, r.tv);
   } else {
@@ -1538,6 +1602,14 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
     hipLaunchKernelGGL(k_spline_v, gv, dim3(64), (size_t)r.TC * (r.nU + 2) * sizeof(double), s,
                        r.c, r.nR, r.nU, r.nJ, r.cs, r.TC, 0, r.tv);
   }
+}
+
+void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
+  time_begin(t, s, "spline_r");
+  launch_spline_r(r, src, s);
+  time_mark(t, s, "spline_r");
+  time_begin(t, s, "spline_v");
+  launch_spline_v(r, src, s);
   time_mark(t, s, "spline_v");
 }
 
@@ -1549,6 +1621,7 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
 // the evaluation as two kernels, the evaluation a thread per point.
 static bool fused_kick(const Run &r) {
   const int64_t waves = (int64_t)r.nJ * ((r.nR + kKW - 1) / kKW);
+  if (r.kickMode >= 0) return r.kickMode == 1 && line_shape(r) != nullptr;
   if (getenv("VLS_FUSED_KICK")) return line_shape(r) != nullptr;
   return line_shape(r) != nullptr && waves >= 512 && !getenv("VLS_NO_FUSED_KICK");
 }
@@ -1561,19 +1634,183 @@ size_t kick_scratch_doubles(const Run &r) {
 
 void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *dKeep) {
   if (moment) {
+    time_begin(t, s, "moment");
     launch_moment(r, s);
     time_mark(t, s, "moment");
   }
   if (fused_kick(r) && r.gsc) {
     const LineShape *ls = line_shape(r);
+    time_begin(t, s, "kick_line");
     hipLaunchKernelGGL(dKeep ? ls->kickKeep : ls->kick, dim3((r.nR + kKW - 1) / kKW * r.nJ),
                        dim3(64), 0, s, r.f, r.U_Val, r.acc, r.gsc, r.Ulo, r.Uhi, r.start_v,
                        r.hinv_v, r.kickScratch ? 1 : 0, 0, r.tv, dKeep, r.cs);
     time_mark(t, s, "kick_line");
     return;
   }
+  time_begin(t, s, "kick");
   launch_kick(r, s, dKeep ? dKeep : r.c);
   time_mark(t, s, "kick");
+}
+
+// ---------------------------------------------------------------- schedules
+// vls_autotune: the drift's line solves and the kick are timed on the run's
+// own data in each of their schedules and the fastest is kept.  Every
+// schedule runs the same operations per line in the same order, so the
+// choice changes the time of a step, never its bits.
+namespace {
+struct Sched {
+  int lineMode, kickMode, TR, LS, TC;
+  bool streamR, streamV;
+  double splineUs, kickUs;
+};
+std::mutex g_tuneMu;
+std::map<std::tuple<int, int, int, int>, Sched> g_tuned;
+
+void apply_sched(Run &r, const Sched &c) {
+  r.lineMode = c.lineMode;
+  r.kickMode = c.kickMode;
+  r.TR = c.TR;
+  r.LS = c.LS;
+  r.TC = c.TC;
+  r.streamR = c.streamR;
+  r.streamV = c.streamV;
+}
+
+// mean duration in microseconds of fn() on stream s over reps launches,
+// after one launch to warm up
+template <class F>
+double time_us(hipStream_t s, F &&fn, int reps = 5) {
+  hipEvent_t a, b;
+  VLS_HIP_CHECK(hipEventCreate(&a));
+  VLS_HIP_CHECK(hipEventCreate(&b));
+  fn();
+  VLS_HIP_CHECK(hipEventRecord(a, s));
+  for (int k = 0; k < reps; k++) fn();
+  VLS_HIP_CHECK(hipEventRecord(b, s));
+  VLS_HIP_CHECK(hipEventSynchronize(b));
+  float ms = 0;
+  VLS_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  VLS_HIP_CHECK(hipGetLastError());
+  return ms * 1e3 / reps;
+}
+
+std::string describe(const Run &r, const Sched &c, bool cached) {
+  char buf[256];
+  std::string sp;
+  if (c.lineMode == 1) {
+    sp = "register lines";
+  } else {
+    sp = c.streamR ? std::string("r-solve streaming") : "r-solve LDS tile of " + std::to_string(c.TR) + " rows";
+    sp += c.streamV ? ", v-solve streaming" : ", v-solve LDS tile of " + std::to_string(c.TC) + " columns";
+  }
+  snprintf(buf, sizeof buf, "%dx%dx%d: %s, %s kick (spline %.1f us, kick %.1f us%s)", r.nR, r.nU,
+           r.nJ, sp.c_str(), c.kickMode == 1 ? "fused" : "two-kernel", c.splineUs, c.kickUs,
+           cached ? ", cached" : "");
+  return buf;
+}
+}  // namespace
+
+std::string autotune(Run &r, hipStream_t s, int maxLds) {
+  const auto key = std::make_tuple(r.nR, r.nU, r.nJ, maxLds);
+  {
+    std::lock_guard<std::mutex> lock(g_tuneMu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      apply_sched(r, it->second);
+      return describe(r, it->second, true);
+    }
+  }
+  // the drift's spline: register lines where the shape has them, else the
+  // best LDS-tiled or streaming solver for each direction
+  Sched best{0, 0, r.TR, r.LS, r.TC, r.streamR, r.streamV, 1e30, 0};
+  Run &probe = r;  // tried in place; the best schedule is applied at the end
+  probe.lineMode = 0;
+  double bestR = 1e30, bestV = 1e30;
+  std::vector<int> lsCands = {7, 15, 31, 63, 65};
+  for (int k = -1; k < (int)lsCands.size(); k++) {
+    if (k >= 0 && (size_t)lsCands[k] * (r.nR + 2) * sizeof(double) > (size_t)maxLds) continue;
+    probe.streamR = k < 0;
+    if (k >= 0) {
+      probe.LS = lsCands[k];
+      probe.TR = probe.LS > 64 ? 64 : probe.LS;
+    }
+    const double us = time_us(s, [&] { launch_spline_r(probe, r.f, s); });
+    if (us < bestR) {
+      bestR = us;
+      best.streamR = probe.streamR;
+      best.LS = probe.LS;
+      best.TR = probe.TR;
+    }
+  }
+  for (int tc : {-1, 8, 16, 32, 64}) {
+    if (tc > 0 && (size_t)tc * (r.nU + 2) * sizeof(double) > (size_t)maxLds) continue;
+    probe.streamV = tc < 0;
+    if (tc > 0) probe.TC = tc;
+    const double us = time_us(s, [&] { launch_spline_v(probe, r.f, s); });
+    if (us < bestV) {
+      bestV = us;
+      best.streamV = probe.streamV;
+      best.TC = probe.TC;
+    }
+  }
+  best.splineUs = bestR + bestV;
+  probe.lineMode = 1;
+  if (line_shape(probe)) {
+    const double us = time_us(s, [&] { launch_spline(probe, r.f, s, nullptr); });
+    if (us < best.splineUs) {
+      best.lineMode = 1;
+      best.splineUs = us;
+    }
+  }
+  // the kick, on a copy of f (it works in place), with the acceleration of
+  // the current f
+  apply_sched(r, best);
+  launch_moment(r, s);
+  double *f0 = r.f, *tmp = nullptr;
+  VLS_HIP_CHECK(hipMalloc(&tmp, (size_t)r.nVal * sizeof(double)));
+  r.f = tmp;
+  try {
+    auto time_kick = [&](int mode) {
+      r.kickMode = mode;
+      VLS_HIP_CHECK(hipMemcpyAsync(tmp, f0, (size_t)r.nVal * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      return time_us(s, [&] { launch_spline_kick(r, s, nullptr, false); });
+    };
+    best.kickMode = 0;
+    best.kickUs = time_kick(0);
+    if (best.lineMode == 1) {
+      if (!r.gsc) {
+        r.kickMode = 1;
+        const size_t n = kick_scratch_doubles(r);
+        void *p = nullptr;
+        VLS_HIP_CHECK(hipMalloc(&p, n * sizeof(double)));
+        r.allocs.push_back(p);
+        r.allocBytes.push_back(n * sizeof(double));
+        r.gsc = static_cast<double *>(p);
+      }
+      const double us = time_kick(1);
+      if (us < best.kickUs) {
+        best.kickMode = 1;
+        best.kickUs = us;
+      }
+    }
+  } catch (...) {
+    r.f = f0;
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    throw;
+  }
+  r.f = f0;
+  VLS_HIP_CHECK(hipStreamSynchronize(s));
+  VLS_HIP_CHECK(hipFree(tmp));
+  apply_sched(r, best);
+  {
+    std::lock_guard<std::mutex> lock(g_tuneMu);
+    g_tuned[key] = best;
+  }
+  return describe(r, best, false);
 }
 
 void launch_kick(Run &r, hipStream_t s, double *d) {
@@ -1605,7 +1842,12 @@ void mark_chains(Run &r, hipStream_t s) {
 }
 
 void launch_density_potential(Run &r, double *rho, double *phi, hipStream_t s) {
-  hipLaunchKernelGGL(k_density_potential, dim3(1), dim3(kBlock), 0, s, r.R_Val, r.dM, r.Mx, r.sp.G,
+  launch_density_potential_from(r, r.dM, r.Mx, rho, phi, s);
+}
+
+void launch_density_potential_from(Run &r, const double *dM, const double *Mx, double *rho,
+                                   double *phi, hipStream_t s) {
+  hipLaunchKernelGGL(k_density_potential, dim3(1), dim3(kBlock), 0, s, r.R_Val, dM, Mx, r.sp.G,
                      (4.0 * M_PI) / 3.0, rho, phi, r.nR);
 }
 
