@@ -212,6 +212,25 @@ int vls_mass_profile(vls_ctx *ctx, int run, double *M);
  * be NULL.  Refreshes the moments of vls_moments. */
 int vls_density_potential(vls_ctx *ctx, int run, double *rho, double *phi);
 
+/* Allocation-free variant of vls_density_potential.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  f_dev is the caller's device array of the run's shape,
+ * [nS][nJ][nU][nR] (one plane per species); rho_dev, phi_dev and M_dev are
+ * device arrays of nR entries (any may be NULL, not all) that receive the
+ * density and potential of vls_density_potential and the enclosed mass
+ * M(<R_r) of vls_mass_profile, bit for bit.  The moment's partial sums go to
+ * a static device array of VLS_NOALLOC_SCRATCH_DOUBLES doubles per GPU (2 MiB,
+ * held until the process ends): a run with nJ*nR + 5*nR + 1 above it is
+ * rejected with VLS_ERR_ARG.  The call allocates nothing, leaves the
+ * context's state (f, the moments of vls_moments) untouched, launches on
+ * `stream` (a hipStream_t, or NULL for the context's stream) and returns
+ * without waiting.  The context's device must be current.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+#define VLS_NOALLOC_SCRATCH_DOUBLES (1 << 18)
+int vls_density_potential_noalloc(vls_ctx *ctx, int run, const double *f_dev, double *rho_dev,
+                                  double *phi_dev, double *M_dev, void *stream);
+
 /* Discrete adjoint of the step.  No counterpart in the reference (it has no
  * derivative); it differentiates exactly the discrete map that vls_step
  * applies (linking steps).

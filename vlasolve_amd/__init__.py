@@ -31,7 +31,7 @@ EXPORTS = (
     "vlsd_host_feet", "vlsd_host_chains", "vls_run_info", "vls_put_chains", "vls_mass_profile",
     "vls_record", "vls_adjoint", "vls_set_graph", "vlsd_ensemble_create", "vlsd_ensemble_destroy",
     "vlsd_ensemble_context", "vls_stats_total", "vlsd_species", "vlsd_stats_total",
-    "vlsd_host_species", "vls_memory", "vls_density_potential", "vls_set_timing_kernel", "vlsd_host_dparam",
+    "vlsd_host_species", "vls_memory", "vls_density_potential", "vls_density_potential_noalloc", "vls_set_timing_kernel", "vlsd_host_dparam",
     "vls_record_batch", "vls_adjoint_batch", "vls_autotune",
 )
 
@@ -113,6 +113,7 @@ def lib():
         L.vls_set_graph.argtypes = [vp, C.c_int]
         L.vls_memory.argtypes = [vp, C.c_int, P(i64), P(i64)]
         L.vls_density_potential.argtypes = [vp, C.c_int, P(d), P(d)]
+        L.vls_density_potential_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.vlsd_ensemble_create.argtypes = [C.c_int, P(C.c_int), P(P(C.c_char_p)), P(vp)]
         L.vlsd_ensemble_destroy.argtypes = [vp]
         L.vlsd_ensemble_destroy.restype = None
@@ -341,6 +342,14 @@ class Context:
         rho, phi = np.empty(n), np.empty(n)
         _check(lib().vls_density_potential(self._h, run, _ptr(rho), _ptr(phi)))
         return rho, phi
+
+    def density_potential_noalloc(self, f_dev, rho_dev=None, phi_dev=None, M_dev=None, run=0,
+                                  stream=None):
+        """vls_density_potential_noalloc on device addresses (ints, e.g. a
+        torch tensor's data_ptr()): f_dev [nS][nJ][nU][nR] in, rho, phi, M
+        [nR] out (None: not wanted).  Returns without waiting."""
+        _check(lib().vls_density_potential_noalloc(self._h, run, f_dev, rho_dev, phi_dev, M_dev,
+                                                   stream))
 
     def record(self, n_steps, ck_every, run=0):
         _check(lib().vls_record(self._h, run, int(n_steps), int(ck_every)))

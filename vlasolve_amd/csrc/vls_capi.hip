@@ -225,6 +225,35 @@ struct vls_ctx {
   vls::Context c;
 };
 
+// Scratch of vls_density_potential_noalloc: the moment's partials and radial
+// sums, one array per GPU, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: kernels receive
+// the device address that hipGetSymbolAddress returns for the current device
+// (scratch_dev, resolved once per device by vls_create).
+static __device__ __align__(256) double scratch[VLS_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_dev;  // index: device ordinal
+
+// With `device` current: its address of `scratch`, resolved on first use.
+double *resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    VLS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = static_cast<double *>(p);
+  }
+  return g_scratch_dev[device];
+}
+
+double *scratch_dev(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+}  // namespace
+
 namespace {
 vls::Run &run_of(vls_ctx *ctx, int run) {
   if (!ctx) throw vls::Error(VLS_ERR_ARG, "null context");
@@ -414,6 +443,7 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
   vls_ctx *ctx = new vls_ctx();
   int rc = guarded([&] {
     VLS_HIP_CHECK(hipGetDevice(&ctx->c.device));
+    (void)resolve_scratch(ctx->c.device);
     int &maxLds = ctx->c.maxLds;
     VLS_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeSharedMemPerBlockOptin,
                                         ctx->c.device));
@@ -879,6 +909,40 @@ int vls_density_potential(vls_ctx *ctx, int run, double *rho, double *phi) {
       VLS_HIP_CHECK(hipMemcpyAsync(phi, r.rhoPhi + r.nR, r.nR * sizeof(double),
                                    hipMemcpyDeviceToHost, s));
     VLS_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int vls_density_potential_noalloc(vls_ctx *ctx, int run, const double *f_dev, double *rho_dev,
+                                  double *phi_dev, double *M_dev, void *stream) {
+  return guarded([&] {
+    vls::Run &r = run_of(ctx, run);
+    if (!f_dev) throw vls::Error(VLS_ERR_ARG, "vls_density_potential_noalloc: null f");
+    if (!rho_dev && !phi_dev && !M_dev)
+      throw vls::Error(VLS_ERR_ARG, "vls_density_potential_noalloc: no output");
+    // P [nJ][nR], dM, acc, and Mx, rho, phi where the caller passes none, mTot
+    const size_t need = (size_t)r.nJ * r.nR + 5 * (size_t)r.nR + 1;
+    if (need > (size_t)VLS_NOALLOC_SCRATCH_DOUBLES)
+      throw vls::Error(VLS_ERR_ARG,
+                       "vls_density_potential_noalloc: run needs " + std::to_string(need) +
+                           " doubles of scratch (nJ*nR + 5*nR + 1), the maximum is " +
+                           std::to_string(VLS_NOALLOC_SCRATCH_DOUBLES));
+    int cur = -1;
+    VLS_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != ctx->c.device)
+      throw vls::Error(VLS_ERR_STATE, "vls_density_potential_noalloc: the context's device is "
+                                      "not the current device");
+    double *w = scratch_dev(cur);
+    if (!w) throw vls::Error(VLS_ERR_STATE, "vls_density_potential_noalloc: no scratch");
+    // every entry below is written before it is read: nothing to clear
+    double *P = w, *dM = P + (size_t)r.nJ * r.nR, *acc = dM + r.nR, *Mx = acc + r.nR;
+    double *rho = Mx + r.nR, *phi = rho + r.nR, *mTot = phi + r.nR;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->c.stream;
+    const vls::MomentBufs b{f_dev, P, dM, M_dev ? M_dev : Mx, acc, mTot};
+    vls::launch_moment_into(r, b, s);
+    if (rho_dev || phi_dev)
+      vls::launch_density_potential_from(r, dM, b.Mx, rho_dev ? rho_dev : rho,
+                                         phi_dev ? phi_dev : phi, s);
+    VLS_HIP_CHECK(hipGetLastError());
   });
 }
 
