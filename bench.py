@@ -71,7 +71,7 @@ def algorithmic_bytes(name, nR, nU, nJ):
 
 # The kernels of one step of the workload, as vls_set_timing_kernel names them
 # ("moment" covers the moment partials, the chunk sums and the scan).
-STEP_KERNELS = ("moment", "kick_line", "spline_r", "spline_v", "drift")
+STEP_KERNELS = ("moment", "kick_line", "kick", "spline_r", "spline_v", "drift")
 STEP_BYTES_PER_POINT = 72
 
 # The adjoint's workload: PLDF power-law sphere (n=1, alpha=1, eta=0.1, M=1,
@@ -163,6 +163,73 @@ def adjoint_ratio_hernquist(V, steps=HQ_STEPS, ck=HQ_CK):
             "loss": "sum_r (M(<r) - M_target)^2 of the final state"}
 
 
+# The batched adjoint: 8 Hernquist runs (M=1, R0=1, default ranges) on
+# uneven grids, each axis drawn from 128..384 cells (seeded), dt=0.0025,
+# 200 steps with checkpoints every 14 (about sqrt(200)); each member's f0
+# perturbed by a seeded multiplicative 1e-3, its loss the mismatch of its
+# final mass profile against the unperturbed run's.
+HQB_STEPS, HQB_CK, HQB_SEED = 200, 14, 7
+
+
+def hernquist_batch_cfgs(n=8, seed=HQB_SEED):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        pr, pu, pj = (int(x) for x in rng.integers(128, 385, size=3))
+        out.append(dict(HQ_WORKLOAD, **{"grid.Pres": pr, "grid.Ures": [pu, pj]}))
+    return out
+
+
+def adjoint_batch_ratio(V, steps=HQB_STEPS, ck=HQB_CK, seed=HQB_SEED):
+    """wall(gradients of the 8-run batch: vls_record_batch + vls_adjoint_batch)
+    / wall(the batch's forward run), every run on a stream of its own."""
+    import numpy as np
+
+    cfgs = hernquist_batch_cfgs()
+    e = V.Ensemble(cfgs)
+    ctx = e.context
+    f0 = [e.f(k) for k in range(e.n)]
+    ch0 = [ctx.get_chains(k) for k in range(e.n)]
+    e.step(steps)  # the unperturbed runs give the target profiles
+    Mt = [ctx.mass_profile(k) for k in range(e.n)]
+    rng = np.random.default_rng(seed)
+    fp = [f * (1 + 1e-3 * rng.standard_normal(f.shape)) for f in f0]
+
+    def reset():
+        for k in range(e.n):
+            ctx.upload_f(k, fp[k])
+            ctx.put_chains(ch0[k][0], ch0[k][1], run=k)
+        ctx.synchronize()
+
+    reset()
+    e.step(2)  # warm-up (graph capture)
+    reset()
+    t0 = time.perf_counter()
+    e.step(steps)
+    ctx.synchronize()
+    fwd = time.perf_counter() - t0
+    walls = []
+    for _ in range(2):
+        reset()
+        t0 = time.perf_counter()
+        ctx.record_batch(steps, ck)
+        dL = [2 * (ctx.mass_profile(k) - Mt[k]) for k in range(e.n)]
+        grads = ctx.adjoint_batch(dL_dM=dL)
+        walls.append(time.perf_counter() - t0)
+        if not all(np.isfinite(g[0]).all() for g in grads):
+            raise RuntimeError("non-finite gradient")
+    pts = sum(f.size for f in f0)
+    return {"workload": "8 Hernquist runs M=1 R0=1, axes drawn from 128..384 cells (seed 7), "
+                        "f0 x (1 + 1e-3 N(0,1)), dt 0.0025", "points": pts, "steps": steps,
+            "ck_every": ck, "grids": [list(f.shape[::-1]) for f in f0],
+            "forward_s": round(fwd, 3), "gradient_s": round(walls[0], 3),
+            "ratio": round(walls[0] / fwd, 3), "gradient_s_steady": round(walls[1], 3),
+            "ratio_steady": round(walls[1] / fwd, 3),
+            "loss": "sum_k sum_r (M_k(<r) - M_k,unperturbed(<r))^2 of the final states"}
+
+
 def ensemble_cfgs(n=32, seed=2024):
     """32 UDF runs: alpha ~ U[0.1, 1], R0 ~ U[1.5, 3], each axis drawn from
     {64, 96, 128, 192, 256} cells (j at most 128)."""
@@ -190,20 +257,30 @@ def ensemble_speedup(V):
     e = V.Ensemble(cfgs)
     pts = sum(e.context.info(k)["nR"] * e.context.info(k)["nU"] * e.context.info(k)["nJ"]
               for k in range(e.n))
-    res = {}
-    for mode in (False, True):
+
+    def timed(mode):
         e.set_graph(mode)
         e.step(5)
         e.context.synchronize()
         t0 = time.perf_counter()
         e.step(ENS_STEPS)
         e.context.synchronize()
-        res[mode] = time.perf_counter() - t0
+        return time.perf_counter() - t0
+
+    # the default schedules, then the tuned ones (vls_autotune, timed outside)
+    seq, graph = timed(False), timed(True)
+    t0 = time.perf_counter()
+    schedules = e.autotune()
+    tune_s = time.perf_counter() - t0
+    seq_t, graph_t = timed(False), timed(True)
     return {"workload": "32 UDF runs, alpha~U[0.1,1], R0~U[1.5,3], axes from {64..256} cells "
                         "(j <= 128), seed 2024", "steps": ENS_STEPS, "points": pts,
-            "sequential_s": round(res[False], 4), "graph_s": round(res[True], 4),
-            "point_updates_per_s": pts * ENS_STEPS / res[True],
-            "speedup": round(res[False] / res[True], 3)}
+            "sequential_s": round(seq, 4), "graph_s": round(graph, 4),
+            "speedup": round(seq / graph, 3),
+            "tuned_sequential_s": round(seq_t, 4), "tuned_graph_s": round(graph_t, 4),
+            "tuned_speedup": round(seq / graph_t, 3),
+            "point_updates_per_s": pts * ENS_STEPS / min(graph, graph_t),
+            "autotune_s": round(tune_s, 2), "schedules": schedules}
 
 
 def cpu_threads():
@@ -395,6 +472,13 @@ def main():
         except Exception as e:  # reported, not fatal for the forward figure
             ens = {"speedup": None, "error": str(e)[:200]}
 
+    hqb = None
+    if not args.no_hernquist:
+        try:
+            hqb = adjoint_batch_ratio(V)
+        except Exception as e:  # reported, not fatal for the forward figure
+            hqb = {"ratio": None, "error": str(e)[:200]}
+
     hq = None
     if not args.no_hernquist:
         try:
@@ -424,6 +508,8 @@ def main():
     }
     if hq is not None:
         line["hernquist"] = hq
+    if hqb is not None:
+        line["hernquist_batch"] = hqb
     print(json.dumps(line))
 
 

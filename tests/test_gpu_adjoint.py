@@ -244,7 +244,7 @@ def test_autograd_parameter_gradient():
     ctx.step(n)
     Mt = ctx.mass_profile() * 1.02
 
-    theta = torch.tensor([eta], requires_grad=True)
+    theta = torch.tensor([eta], dtype=torch.float64, requires_grad=True)
     f0, v0 = InitialState().apply(theta, cfg, ["eta"])
     M = MassProfileRun().apply(f0, v0, ctx, n, ck, head0)
     loss = ((M - torch.from_numpy(Mt)) ** 2).sum()

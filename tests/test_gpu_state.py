@@ -22,9 +22,14 @@ def test_repeated_record_keeps_memory_flat():
     ctx.record(8, 2)  # more checkpoints: the checkpoint arrays are replaced
     n2, b2 = ctx.memory()
     assert n2 == n1 and b2 > b1
-    ctx.record(8, 4)
+    ctx.record(8, 4)  # longer segments: the segment buffer is replaced
+    n3, b3 = ctx.memory()
+    assert n3 == n2 and b3 > b2
+    # smaller recordings reuse what is there
     ctx.record(4, 2)
-    assert ctx.memory() == (n2, b2)
+    ctx.record(8, 4)
+    ctx.record(2, 1)
+    assert ctx.memory() == (n3, b3)
 
 
 def test_adjoint_after_chain_change_is_refused():

@@ -70,7 +70,7 @@ def test_initial_state_autograd_function():
     rng = np.random.default_rng(3)
     w = rng.standard_normal(base.shape)
     wv = rng.standard_normal(base.n_cells)
-    theta = torch.tensor(theta0, requires_grad=True)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
     f0, v0 = InitialState().apply(theta, cfg, names)
     L = (f0 * torch.from_numpy(w)).sum() + (v0 * torch.from_numpy(wv)).sum()
     L.backward()
@@ -84,3 +84,5 @@ def test_initial_state_autograd_function():
         e[k] = 1e-6 * theta0[k]
         fd = (lin(theta0 + e) - lin(theta0 - e)) / (2 * e[k])
         assert abs(float(theta.grad[k]) - fd) <= 1e-6 * abs(fd), (names[k], float(theta.grad[k]), fd)
+    with pytest.raises(TypeError, match="float64"):
+        InitialState().apply(torch.tensor([0.1], requires_grad=True), cfg, ["eta"])

@@ -58,6 +58,9 @@ def _make_initial():
         def forward(fctx, theta, cfg, names):
             from . import HostSetup
 
+            if theta.dtype != torch.float64:
+                raise TypeError("InitialState: theta must be float64 (a float32 parameter "
+                                "already differs from the value meant)")
             vals = [float(x) for x in theta.detach().cpu().numpy().reshape(-1)]
             if len(vals) != len(names):
                 raise ValueError("one value per parameter name")
