@@ -429,7 +429,15 @@ class Ensemble:
         _check(lib().vlsd_ensemble_create(n, argc, argv, C.byref(h)), driver=True)
         self._h = h
         self.n = n
-        self.context = Context(lib().vlsd_ensemble_context(self._h), owner=self)
+
+    @property
+    def context(self):
+        """the ensemble's device context (borrowed: it lives as long as the
+        ensemble; made on access, so that no reference cycle delays the
+        release of the runs' device memory)"""
+        if not self._h:
+            raise VlsError("ensemble closed")
+        return Context(lib().vlsd_ensemble_context(self._h), owner=self)
 
     def step(self, n=1):
         self.context.step(n)
@@ -447,7 +455,6 @@ class Ensemble:
 
     def close(self):
         if getattr(self, "_h", None):
-            self.context._h = None
             lib().vlsd_ensemble_destroy(self._h)
             self._h = None
 

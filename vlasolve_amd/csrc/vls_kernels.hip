@@ -1042,13 +1042,18 @@ k_kick_vsolve(const double *__restrict__ f, double *__restrict__ d, int nR, int 
       [&](int k, double v) { dc[(int64_t)k * nR] = v; });
 }
 
+// A 1D grid of nT blocks per slice in the order of xcd_tile: the blocks of a
+// slice, which read the same rows of d (each row serves four rows of
+// points), run on one XCD and share its L2.
 __global__ void __launch_bounds__(kBlock)
 k_kick_cols(double *__restrict__ f, const double *__restrict__ d, const double *__restrict__ U_Val,
             const double *__restrict__ acc, int nR, int nU, int64_t ds, double Ulo, double Uhi,
             double start_v, double hinv_v) {
-  const int il = blockIdx.x * blockDim.x + threadIdx.x;  // index inside the slice
+  const int nT = (nU * nR + kBlock - 1) / kBlock;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nT;
+  const int il = (tile - j * nT) * kBlock + threadIdx.x;  // index inside the slice
   if (il >= nU * nR) return;
-  const int j = blockIdx.y;
   const int u = il / nR;
   const int r = il - u * nR;
   const size_t i = (size_t)j * nU * nR + il;
@@ -1820,9 +1825,8 @@ void launch_kick(Run &r, hipStream_t s, double *d) {
   else
     hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64 * r.nJ), dim3(64), 0, s, r.f, d, r.nR,
                        r.nU, r.cs, 0, r.tv);
-  dim3 g(blocks_for((int64_t)r.nU * r.nR), r.nJ);
-  hipLaunchKernelGGL(k_kick_cols, g, dim3(kBlock), 0, s, r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs,
-                     r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  hipLaunchKernelGGL(k_kick_cols, dim3(blocks_for((int64_t)r.nU * r.nR) * r.nJ), dim3(kBlock), 0, s,
+                     r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
 void launch_drift(Run &r, hipStream_t s) {
