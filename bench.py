@@ -141,6 +141,13 @@ def adjoint_ratio_hernquist(V, steps=HQ_STEPS, ck=HQ_CK):
     """The adjoint/forward wall ratio on the Hernquist workload, first and
     steady gradient as in adjoint_ratio."""
     s = V.Solver(HQ_WORKLOAD)
+    try:
+        return _hernquist_legs(s, steps, ck)
+    finally:
+        s.close()  # its kept steps hold most of the HBM
+
+
+def _hernquist_legs(s, steps, ck):
     ctx = s.context
     f0 = s.f()
     head0, vals0 = ctx.get_chains()
@@ -189,6 +196,15 @@ def adjoint_batch_ratio(V, steps=HQB_STEPS, ck=HQB_CK, seed=HQB_SEED):
 
     cfgs = hernquist_batch_cfgs()
     e = V.Ensemble(cfgs)
+    try:
+        return _batch_legs(e, steps, ck, seed)
+    finally:
+        e.close()  # its kept steps hold most of the HBM
+
+
+def _batch_legs(e, steps, ck, seed):
+    import numpy as np
+
     ctx = e.context
     f0 = [e.f(k) for k in range(e.n)]
     ch0 = [ctx.get_chains(k) for k in range(e.n)]
@@ -210,14 +226,16 @@ def adjoint_batch_ratio(V, steps=HQB_STEPS, ck=HQB_CK, seed=HQB_SEED):
     e.step(steps)
     ctx.synchronize()
     fwd = time.perf_counter() - t0
-    walls = []
+    walls, parts = [], []
     for _ in range(2):
         reset()
         t0 = time.perf_counter()
         ctx.record_batch(steps, ck)
+        t1 = time.perf_counter()
         dL = [2 * (ctx.mass_profile(k) - Mt[k]) for k in range(e.n)]
         grads = ctx.adjoint_batch(dL_dM=dL)
         walls.append(time.perf_counter() - t0)
+        parts.append((round(t1 - t0, 3), round(walls[-1] - (t1 - t0), 3)))
         if not all(np.isfinite(g[0]).all() for g in grads):
             raise RuntimeError("non-finite gradient")
     pts = sum(f.size for f in f0)
@@ -226,7 +244,7 @@ def adjoint_batch_ratio(V, steps=HQB_STEPS, ck=HQB_CK, seed=HQB_SEED):
             "ck_every": ck, "grids": [list(f.shape[::-1]) for f in f0],
             "forward_s": round(fwd, 3), "gradient_s": round(walls[0], 3),
             "ratio": round(walls[0] / fwd, 3), "gradient_s_steady": round(walls[1], 3),
-            "ratio_steady": round(walls[1] / fwd, 3),
+            "ratio_steady": round(walls[1] / fwd, 3), "record_reverse_s": parts,
             "loss": "sum_k sum_r (M_k(<r) - M_k,unperturbed(<r))^2 of the final states"}
 
 
@@ -472,19 +490,19 @@ def main():
         except Exception as e:  # reported, not fatal for the forward figure
             ens = {"speedup": None, "error": str(e)[:200]}
 
-    hqb = None
-    if not args.no_hernquist:
-        try:
-            hqb = adjoint_batch_ratio(V)
-        except Exception as e:  # reported, not fatal for the forward figure
-            hqb = {"ratio": None, "error": str(e)[:200]}
-
     hq = None
     if not args.no_hernquist:
         try:
             hq = adjoint_ratio_hernquist(V, args.hq_steps, args.hq_ck)
         except Exception as e:  # reported, not fatal for the forward figure
             hq = {"ratio": None, "error": str(e)[:200]}
+
+    hqb = None
+    if not args.no_hernquist:
+        try:
+            hqb = adjoint_batch_ratio(V)
+        except Exception as e:  # reported, not fatal for the forward figure
+            hqb = {"ratio": None, "error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:

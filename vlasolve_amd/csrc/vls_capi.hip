@@ -1034,15 +1034,18 @@ size_t kept_step_bytes(const vls::Run &r) {
   return ((size_t)r.nJ * r.cs + kCoefPad(r) + (size_t)r.nR) * sizeof(double);
 }
 
-// The trailing segments whose kept steps fit in `budget` bytes (plus what
-// the run already holds for them) are kept by the recorded run itself, so
+// Bytes the run holds for kept steps.
+size_t kept_held_bytes(const vls::Run &r) { return (size_t)r.adj.capKeep * kept_step_bytes(r); }
+
+// The trailing segments whose kept steps fit in `budget` bytes (what the run
+// already holds for them included) are kept by the recorded run itself, so
 // that the reverse sweep need not recompute them (VLS_ADJ_KEEP_STEPS caps
 // the kept steps).
 void record_keep(vls::Run &r, hipStream_t s, size_t budget) {
   auto &A = r.adj;
   const size_t coef = (size_t)r.nJ * r.cs + kCoefPad(r);
   const size_t stepB = kept_step_bytes(r);
-  int64_t cap = (int64_t)((budget + (size_t)A.capKeep * stepB) / stepB);
+  int64_t cap = (int64_t)(budget / stepB);
   if (const char *e = getenv("VLS_ADJ_KEEP_STEPS")) cap = std::min<int64_t>(cap, atoll(e));
   int keepSegs = 0;
   while (keepSegs < A.nCk) {
@@ -1180,8 +1183,9 @@ int vls_record(vls_ctx *ctx, int run, int nSteps, int ckEvery) {
     vls::Run &r = run_of(ctx, run);
     hipStream_t s = ctx->c.stream;
     record_buffers(r, s, nSteps, ckEvery);
-    // the kept steps may fill seven eighths of the free HBM
-    record_keep(r, s, free_hbm() / 8 * 7);
+    // the kept steps may fill seven eighths of the free HBM and of what the
+    // run already holds for them
+    record_keep(r, s, (free_hbm() + kept_held_bytes(r)) / 8 * 7);
     for (int n = 0; n < nSteps; n++) record_step(r, s, n);
     record_end(r, s);
     VLS_HIP_CHECK(hipGetLastError());
@@ -1211,11 +1215,15 @@ int vls_record_batch(vls_ctx *ctx, int nSteps, int ckEvery) {
     const int nRuns = (int)C.runs.size();
     ensure_run_streams(C);
     for (int k = 0; k < nRuns; k++) record_buffers(C.runs[k], C.runStreams[k], nSteps, ckEvery);
-    // seven eighths of the free HBM for kept steps, shared in proportion to
-    // the runs' step sizes, so that every run keeps the same trailing steps
-    const size_t budget = free_hbm() / 8 * 7;
-    size_t total = 0;
-    for (auto &r : C.runs) total += kept_step_bytes(r);
+    // seven eighths of the free HBM and of what the runs hold for kept steps,
+    // shared in proportion to the runs' step sizes, so that every run keeps
+    // the same trailing steps
+    size_t held = 0, total = 0;
+    for (auto &r : C.runs) {
+      held += kept_held_bytes(r);
+      total += kept_step_bytes(r);
+    }
+    const size_t budget = (free_hbm() + held) / 8 * 7;
     for (int k = 0; k < nRuns; k++)
       record_keep(C.runs[k], C.runStreams[k],
                   (size_t)((double)budget * kept_step_bytes(C.runs[k]) / (double)total));
