@@ -64,14 +64,18 @@ def algorithmic_bytes(name, nR, nU, nJ):
         "spline_v": 8 * rows + 8 * coef,  # read them, write all coefficients
         "kick": 16 * n + 16 * cols,      # column splines of f, then the kick: f -> d -> f
         "kick_line": 8 * n + 8 * n,      # fused column splines + kick: read f, write f
+        "kick_solve": 8 * n + 8 * cols,  # column splines and moment partials: read f, write d
+        "kick_eval": 8 * cols + 8 * n,   # the kick from d: read d, write f
         "drift": 8 * coef + 8 * n,       # read coefficients, write f
     }
     return table.get(name)
 
 
 # The kernels of one step of the workload, as vls_set_timing_kernel names them
-# ("moment" covers the moment partials, the chunk sums and the scan).
-STEP_KERNELS = ("moment", "kick_line", "kick", "spline_r", "spline_v", "drift")
+# ("moment" covers the moment partials, the chunk sums and the scan;
+# "kick_solve" the kick's column splines with the moment).
+STEP_KERNELS = ("moment", "kick_solve", "kick_eval", "kick_line", "kick", "spline_r", "spline_v",
+                "drift")
 STEP_BYTES_PER_POINT = 72
 
 # The adjoint's workload: PLDF power-law sphere (n=1, alpha=1, eta=0.1, M=1,

@@ -135,13 +135,54 @@ def test_strong_kick_bit_exact(monkeypatch, fused):
 
 
 def test_unfused_kick_bit_exact(monkeypatch):
-    """The column v-solve and evaluation kernels on the 201 x 201 x 200 grid,
-    which takes the fused kick by default."""
+    """The column v-solve and evaluation kernels after a separate moment on
+    the 201 x 201 x 200 grid, which takes the moment fused into the column
+    splines by default."""
     from tests.configs import UDF_C2
 
-    print("case udf_c2 with VLS_NO_FUSED_KICK", flush=True)
+    print("case udf_c2 with VLS_NO_KICK_MOMENT, VLS_NO_FUSED_KICK", flush=True)
+    monkeypatch.setenv("VLS_NO_KICK_MOMENT", "1")
     monkeypatch.setenv("VLS_NO_FUSED_KICK", "1")
     _bit_exact_run(UDF_C2, 3, "unfused kick")
+
+
+@pytest.mark.parametrize("env", [
+    {"VLS_NO_KICK_MOMENT": "1"},                       # one-launch moment, fused kick
+    {"VLS_NO_KICK_MOMENT": "1", "VLS_MOMENT_3K": "1"},  # three-launch moment, fused kick
+])
+def test_separate_moment_bit_exact(monkeypatch, env):
+    """The moment before the kick, in one launch (partials, chunk sums and
+    scan joined by completion counters) and in three, on the 201 x 201 x 200
+    grid."""
+    from tests.configs import UDF_C2
+
+    print(f"case udf_c2 with {env}", flush=True)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    _bit_exact_run(UDF_C2, 3, "separate moment")
+
+
+@pytest.mark.parametrize("name", ["udf_c1", "pldf"])
+def test_moment_counters_across_runs(name):
+    """The one-launch moment's counters return to zero after every launch:
+    two runs of one grid stepped in turn, then one of them alone, against
+    the oracle."""
+    cfg, nsteps = SMALL_CASES[name]
+    print(f"case {name} two runs in turn", flush=True)
+    a, b = V.Solver(cfg), V.Solver(cfg)
+    orc = O.Oracle(cfg, nthreads=0)
+    for _ in range(3):
+        a.step(1)
+        b.step(1)
+        orc.step(1)
+    a.step(2)
+    b.step(2)
+    orc.step(2)
+    assert np.count_nonzero(a.f() != orc.f()) == 0
+    assert np.count_nonzero(b.f() != orc.f()) == 0
+    dMg, ag, mg = b.moments()
+    dMo, ao, mo = orc.last_moments()
+    assert same(dMg, dMo) and same(ag, ao) and mg == mo
 
 
 @pytest.mark.parametrize("pres,ures", [(20, [18, 5]), (31, [33, 6])])

@@ -490,6 +490,8 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.Mx = dalloc<double>(r, r.nR);
       r.acc = dalloc<double>(r, r.nR);
       r.mTot = dalloc<double>(r, 1);
+      r.mCnt = dalloc<unsigned>(r, vls::moment_counters(r));
+      VLS_HIP_CHECK(hipMemsetAsync(r.mCnt, 0, vls::moment_counters(r) * sizeof(unsigned), s));
       r.rhoPhi = dalloc<double>(r, 2 * (size_t)r.nR);
       r.st_part = dalloc<double>(r, 7 * (size_t)r.nJ * r.nR);
       r.st_q = dalloc<double>(r, 6 * (size_t)r.nJ + r.nR);
@@ -937,7 +939,7 @@ int vls_density_potential_noalloc(vls_ctx *ctx, int run, const double *f_dev, do
     double *P = w, *dM = P + (size_t)r.nJ * r.nR, *acc = dM + r.nR, *Mx = acc + r.nR;
     double *rho = Mx + r.nR, *phi = rho + r.nR, *mTot = phi + r.nR;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->c.stream;
-    const vls::MomentBufs b{f_dev, P, dM, M_dev ? M_dev : Mx, acc, mTot};
+    const vls::MomentBufs b{f_dev, P, dM, M_dev ? M_dev : Mx, acc, mTot, nullptr};
     vls::launch_moment_into(r, b, s);
     if (rho_dev || phi_dev)
       vls::launch_density_potential_from(r, dM, b.Mx, rho_dev ? rho_dev : rho,

@@ -66,7 +66,8 @@ struct Run {
   bool streamR = false, streamV = false;  // streaming solvers for long lines
   // schedule choices (vls_autotune): -1 from the shape and the environment;
   // lineMode 0 / 1: LDS-tiled or streaming solvers / register lines (shapes
-  // that have them); kickMode 0 / 1: two-kernel / fused kick
+  // that have them); kickMode 0 / 1 / 2: two-kernel / fused kick / the moment
+  // fused into the kick's column splines
   int lineMode = -1, kickMode = -1;
   vls_solver_params sp{};
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
@@ -83,6 +84,7 @@ struct Run {
   // moment
   double *P = nullptr;      // [nJ][nR] per-slice partials
   double *dM = nullptr, *Mx = nullptr, *acc = nullptr, *mTot = nullptr;
+  unsigned *mCnt = nullptr;  // counters of the one-launch moment (zero between launches)
   double *rhoPhi = nullptr;  // [2][nR]: shell density and potential (vls_density_potential)
   // tables
   double *R_Val = nullptr, *U_Val = nullptr, *J_Val = nullptr;
@@ -157,13 +159,18 @@ void launch_moment_scan(Run &r, hipStream_t s);  // scan of the partials in P
 struct MomentBufs {
   const double *f;
   double *P, *dM, *Mx, *acc, *mTot;
+  unsigned *cnt;  // counters of the one-launch moment (zeroed), or null: three launches
 };
+// unsigned counters the one-launch moment needs
+size_t moment_counters(const Run &r);
 void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s);
 void launch_moment_scan_into(Run &r, const MomentBufs &b, hipStream_t s);
 void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 // the kick of the plane r.f in column form: v-spline of each column of f
 // into d (slice stride cs, row stride nR), then the evaluation
 void launch_kick(Run &r, hipStream_t s, double *d);
+// the kick's evaluation from the column splines d (layout of launch_kick)
+void launch_kick_eval(Run &r, hipStream_t s, const double *d);
 // (moment,) kick of the plane r.f; without the moment the acceleration of
 // the last launch_moment is used; with dKeep the kick's column coefficients
 // are also left there ([j][m][r], slice stride cs)

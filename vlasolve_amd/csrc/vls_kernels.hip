@@ -56,18 +56,40 @@ __device__ __forceinline__ double lane_next(double x) {  // lane i gets lane i+1
   hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
-__global__ void __launch_bounds__(256)
-k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
-              const double *__restrict__ dU, const double *__restrict__ dR,
-              double *__restrict__ P, int nR, int nU) {
-  extern __shared__ double Wl[];  // the blocks' partials W [nW][64]
+// Stores and loads that other XCDs see without cache maintenance: system
+// scope, so they go through the (per-XCD, non-coherent) L2 to memory; the
+// counters below are agent-scope atomics, which the device keeps coherent.
+__device__ __forceinline__ void coherent_store(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ double coherent_load(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// this wave's stores acknowledged (s_waitcnt vmcnt(0)), before it counts itself
+__device__ __forceinline__ void wait_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The partials P[j][r] of this block's slice j and column tile (returned:
+// the tile), written by wave 0; kCoherent: with system-scope stores, which
+// write through the caches to memory.
+template <bool kCoherent>
+__device__ __forceinline__ int moment_cols_block(const double *__restrict__ f,
+                                                 const double *__restrict__ Ct1,
+                                                 const double *__restrict__ dU,
+                                                 const double *__restrict__ dR,
+                                                 double *__restrict__ P, int nR, int nU,
+                                                 double *Wl, int &jOut) {
   const int lane = threadIdx.x & 63;
   // wave-uniform: the rows u below are too, so dU[u] is one load per wave
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nX = (nR + 62) / 63;  // column tiles per slice
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int j = tile / nX;
-  const int r = (tile - j * nX) * 63 + lane;
+  const int x = tile - j * nX;
+  const int r = x * 63 + lane;
   const bool term = lane < 63 && r < nR - 1;
   const int nW = (nU + 63) / 64;  // blocks of 64 rows
   const double *col = f + (size_t)j * nU * nR + min(r, nR - 1);
@@ -116,8 +138,24 @@ k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
     // the blocks left to right
     double Pj = Wl[lane];
     for (int w = 1; w < nW; w++) Pj = Pj + Wl[w * 64 + lane];
-    if (lane < 63 && r < nR) P[(size_t)j * nR + r] = term ? Pj : 0.0;
+    if (lane < 63 && r < nR) {
+      if constexpr (kCoherent)
+        coherent_store(&P[(size_t)j * nR + r], term ? Pj : 0.0);
+      else
+        P[(size_t)j * nR + r] = term ? Pj : 0.0;
+    }
   }
+  jOut = j;
+  return x;
+}
+
+__global__ void __launch_bounds__(256)
+k_moment_cols(const double *__restrict__ f, const double *__restrict__ Ct1,
+              const double *__restrict__ dU, const double *__restrict__ dR,
+              double *__restrict__ P, int nR, int nU) {
+  extern __shared__ double Wl[];  // the blocks' partials W [nW][64]
+  int j;
+  moment_cols_block<false>(f, Ct1, dU, dR, P, nR, nU, Wl, j);
 }
 
 // The partials of the sum of nS species planes (nVal apart): the integrand
@@ -258,6 +296,95 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
     sdm[r] = a;
   }
   scan_prefix_out(sdm, gfac, Mx, acc, mTot, nR, gf0);
+}
+
+// The moment's reduction after the partials, in the launch that forms them,
+// with the same sums in the same order as k_moment_chunks and k_moment_scan.
+// After its partials (written by wave 0 with coherent_store), a block counts
+// itself in its (chunk, column tile) counter; the block that completes a
+// chunk's count forms that chunk's sums S_c for the tile's columns and
+// counts itself in the last counter; the block that completes that count
+// runs the scan.  Each finisher resets the counter it completed (every other
+// block of the launch has counted by then), so the counters are zero again
+// for the next launch; they start zeroed at allocation.  No block waits for
+// another.  Partials and chunk sums pass between blocks (and XCDs) through
+// memory: written with coherent_store, acknowledged (wait_stores) before
+// the count, read with coherent_load after it; agent-scope fences would
+// instead write back and invalidate each XCD's whole L2 in every block.
+// Every thread of the block calls it; sdm holds nR doubles of LDS.
+struct MomentArgs {
+  const double *Ct1, *dU, *dR, *gfac;
+  double *P, *dM, *Mx, *acc, *mTot;
+  unsigned *cnt;
+};
+
+__device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, int nR, int nJ,
+                                            double *sdm, int *last) {
+  const int nX = (nR + 62) / 63, nC = (nJ + kJC - 1) / kJC;
+  const int c = j / kJC, jb = c * kJC, je = min(nJ, jb + kJC);
+  const int t = threadIdx.x;
+  unsigned *cnt = ma.cnt;
+  double *P = ma.P;
+  if (t < 64) {  // wave 0 wrote the partials
+    wait_stores();
+    if (t == 0) {
+      const unsigned o =
+          __hip_atomic_fetch_add(&cnt[c * nX + x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *last = o == (unsigned)(je - jb - 1);
+    }
+  }
+  __syncthreads();
+  if (!*last) return;  // block-uniform
+  if (t < 64) {
+    const int r = x * 63 + t;
+    if (t < 63 && r < nR) {
+      double v[kJC];
+      // clamped rows are loaded, not added
+#pragma unroll
+      for (int k = 0; k < kJC; k++) v[k] = coherent_load(&P[(size_t)min(jb + k, nJ - 1) * nR + r]);
+      double S = 0.0;
+#pragma unroll
+      for (int k = 0; k < kJC; k++)
+        if (jb + k < je) S += v[k];
+      coherent_store(&P[(size_t)jb * nR + r], S);
+    }
+    wait_stores();
+    if (t == 0) {
+      __hip_atomic_store(&cnt[c * nX + x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned o =
+          __hip_atomic_fetch_add(&cnt[nC * nX], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *last = o == (unsigned)(nC * nX - 1);
+    }
+  }
+  __syncthreads();
+  if (!*last) return;
+  const double gf0 = t < nR ? ma.gfac[t] : 0.0;
+  for (int r = t; r < nR; r += blockDim.x) {
+    constexpr int kB = 16;  // chunk sums in flight
+    double a = 0.0;
+    for (int c0 = 0; c0 < nC; c0 += kB) {
+      double v[kB];
+#pragma unroll
+      for (int k = 0; k < kB; k++) v[k] = coherent_load(&P[(size_t)min(c0 + k, nC - 1) * kJC * nR + r]);
+#pragma unroll
+      for (int k = 0; k < kB; k++)
+        if (c0 + k < nC) a += v[k];
+    }
+    ma.dM[r] = a;
+    sdm[r] = a;
+  }
+  if (t == 0) __hip_atomic_store(&cnt[nC * nX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  scan_prefix_out(sdm, ma.gfac, ma.Mx, ma.acc, ma.mTot, nR, gf0);
+}
+
+// The three moment kernels in one launch (moment_cols_block, moment_tail).
+__global__ void __launch_bounds__(256)
+k_moment_fused(const double *__restrict__ f, int nR, int nU, int nJ, MomentArgs ma) {
+  extern __shared__ double Wl[];  // [nW][64] partials, then [nR] for the scan
+  __shared__ int last;
+  int j;
+  const int x = moment_cols_block<true>(f, ma.Ct1, ma.dU, ma.dR, ma.P, nR, nU, Wl, j);
+  moment_tail(ma, j, x, nR, nJ, Wl, &last);
 }
 
 // ---------------------------------------------------------------- spline
@@ -717,6 +844,128 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     if (live) out[k * os] = cn;
   }
   if (live) out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
+}
+
+// The kick's column splines (k_line<M, true, S>: d_m of each column of f into
+// c, slice stride cs, row stride nR) and the moment of the same f in one pass:
+// the moment needs the whole f before the kick can evaluate, but the column
+// splines do not need the moment, and both read every column of f.  Tiles of
+// 63 columns as in k_moment_cols: lane 63 loads (and solves) the first column
+// of the next tile, which lane 62's moment terms take by a lane shift, and
+// stores nothing.  The moment's terms are formed from the raw rows while
+// they stream in for the elimination, in the order of k_moment_cols (serial
+// sums over groups of 16 rows, ((G0 + G1) + G2) + G3 per block of 64 rows,
+// blocks left to right); moment_tail then forms dM, M(<r) and the
+// acceleration.  Same bits as k_moment_cols + k_moment_chunks +
+// k_moment_scan and k_line<M, true, S>.
+template <int M, int S>
+__global__ void __launch_bounds__(64)
+k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nJ, int64_t cs,
+                    int lzmask, ThomasAxis ax, MomentArgs ma) {
+  constexpr int nU = M - 2, nR = S - 2;
+  constexpr int nX = (nR + 62) / 63;  // column tiles per slice
+  constexpr int nW = (nU + 63) / 64;  // blocks of 64 rows of the moment
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nX, x = tile - j * nX;
+  const int colRaw = x * 63 + lane;
+  const int col = colRaw < nR ? colRaw : nR - 1;
+  const bool owner = lane < 63 && colRaw < nR;     // stores d and the partial
+  const bool term = lane < 63 && colRaw < nR - 1;  // has moment terms
+  const double *in = f + (size_t)j * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
+  double *out = c + (size_t)j * cs + col;
+  constexpr int kLA = M - 2 < 48 ? M - 2 : 48;
+  __shared__ HeadTab htab;
+  __shared__ double dUs[nU];
+  __shared__ double sdm[nR];
+  __shared__ int last;
+  const HeadRegs hr = head_load(ax, lane, lz);
+  constexpr int nUq = (nU + 63) / 64;
+  double duq[nUq];
+#pragma unroll
+  for (int q = 0; q < nUq; q++) duq[q] = ma.dU[min(lane + 64 * q, nU - 1)];
+  const double ct = ma.Ct1[j], dr = term ? ma.dR[colRaw] : 0.0;
+  double y[M];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 1; k <= kLA; k++) y[k] = in[k * nR];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < nUq; q++)
+    if (lane + 64 * q < nU) dUs[lane + 64 * q] = duq[q];
+  head_store(htab, hr, lane);
+  // the block is one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double prev = ax.A03;
+  // moment state: the previous raw row (own and next column), the group sum,
+  // the groups of the current block, the partial over finished blocks
+  double aOwn = 0.0, aN = 0.0, g = 0.0, Pj = 0.0;
+  double G[4] = {0.0, 0.0, 0.0, 0.0};
+  double duNext = dUs[0];
+#pragma unroll
+  for (int k = 1; k <= M - 2; k++) {
+    if (k + kLA <= M - 2) y[k + kLA] = in[(k + kLA) * nR];
+    const double b = y[k];  // raw row k - 1
+    const double bN = lane_next(b);
+    if (k >= 2) {
+      // the term of row u = k - 2 (rows u and u + 1 of columns col, col + 1)
+      const int u = k - 2;
+      const double du = duNext;
+      if (u + 1 < nU) duNext = dUs[u + 1];
+      const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+      if (u % 16 == 0) g = 0.0;
+      g += ((ct * du) * dr) * avg;
+      // the sum is formed here, in this row's step: left to the end of the
+      // kernel, where it is used, it would keep every raw row live
+      asm volatile("" : "+v"(g));
+      if (u % 16 == 15 || u == nU - 2) G[(u / 16) % 4] = g;
+      if (u % 64 == 63 || u == nU - 2) {
+        // groups of this block past the last row hold no terms
+        const int qLast = (u / 16) % 4;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q > qLast) G[q] = 0.0;
+        const double W = ((G[0] + G[1]) + G[2]) + G[3];
+        Pj = u / 64 == 0 ? W : Pj + W;
+        asm volatile("" : "+v"(Pj));
+      }
+    }
+    aOwn = b;
+    aN = bN;
+    double dv = dS, ev = rdS;
+    if (k <= kHead) {
+      dv = htab.d[k];
+      ev = htab.rd[k];
+    }
+    y[k] = div_known(b - c16 * prev, dv, ev);
+    prev = y[k];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // blocks of 64 rows past the last term (none for the standard shapes)
+#pragma unroll
+  for (int w = (nU - 2) / 64 + 1; w < nW; w++) Pj = Pj + 0.0;
+  double tt = 0.0 - ax.hi2 * y[M - 3];
+  tt = tt - ax.p * y[M - 2];
+  double cn = div_known(tt, ax.q, ax.rq);
+  if (owner) out[(M - 1) * nR] = cn;
+  const double uS = ax.uS + (double)lz;
+#pragma unroll
+  for (int k = M - 2; k >= 1; k--) {
+    double uv = uS;
+    if (k <= kHead) uv = htab.u[k];
+    y[k] = y[k] - uv * cn;
+    cn = y[k];
+    if (owner) out[k * nR] = cn;
+  }
+  if (owner) {
+    out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
+    coherent_store(&ma.P[(size_t)j * nR + colRaw], term ? Pj : 0.0);
+  }
+  moment_tail(ma, j, x, nR, nJ, sdm, &last);
 }
 
 // ---------------------------------------------------------------- fused kick
@@ -1361,12 +1610,24 @@ void time_begin(Timing *t, hipStream_t s, const char *name) {
   if (t && t->on && !t->only.empty() && t->only == name) time_record(t, s, name);
 }
 
+// counters of k_moment_fused: one per (chunk of slices, column tile), one more
+size_t moment_counters(const Run &r) {
+  return (size_t)((r.nR + 62) / 63) * ((r.nJ + kJC - 1) / kJC) + 1;
+}
+
 void launch_moment(Run &r, hipStream_t s) {
-  launch_moment_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot}, s);
+  launch_moment_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt}, s);
 }
 
 void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
+  if (r.nS == 1 && b.cnt && !getenv("VLS_MOMENT_3K")) {
+    const size_t lds = std::max<size_t>(64 * ((r.nU + 63) / 64), r.nR) * sizeof(double);
+    const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, b.P, b.dM, b.Mx, b.acc, b.mTot, b.cnt};
+    hipLaunchKernelGGL(k_moment_fused, dim3((r.nR + 62) / 63 * r.nJ), dim3(256), lds, s, b.f,
+                       r.nR, r.nU, r.nJ, ma);
+    return;
+  }
   if (r.nS > 1)
     hipLaunchKernelGGL(k_moment_partial_sp, g, dim3(kBlock), 0, s, b.f, r.Ct1, r.dU, r.dR, b.P,
                        r.nR, r.nU, r.nJ, r.nS, r.nVal);
@@ -1378,7 +1639,7 @@ void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s) {
 }
 
 void launch_moment_scan(Run &r, hipStream_t s) {
-  launch_moment_scan_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot}, s);
+  launch_moment_scan_into(r, MomentBufs{r.f, r.P, r.dM, r.Mx, r.acc, r.mTot, nullptr}, s);
 }
 
 void launch_moment_scan_into(Run &r, const MomentBufs &b, hipStream_t s) {
@@ -1555,16 +1816,18 @@ using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, T
 using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
 using KickFn = void (*)(double *, const double *, const double *, double *, double, double, double,
                         double, int, int, ThomasAxis, double *, int64_t);
+using KickMomentFn = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, MomentArgs);
 struct LineShape {
   int nR, nU;
   RowsFn rows;
   LineFn cols, kickCols;
   KickFn kick, kickKeep;
+  KickMomentFn kickMoment;
 };
 #define VLS_LINE_SHAPE(R, U)                                                          \
   {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>,                      \
    k_line<(U) + 2, true, (R) + 2>, k_kick_line<(U) + 2, (R) + 2, false>,             \
-   k_kick_line<(U) + 2, (R) + 2, true>}
+   k_kick_line<(U) + 2, (R) + 2, true>, k_kick_solve_moment<(U) + 2, (R) + 2>}
 static const LineShape kLineShapes[] = {
     VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
     VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
@@ -1628,6 +1891,7 @@ static bool fused_kick(const Run &r) {
   const int64_t waves = (int64_t)r.nJ * ((r.nR + kKW - 1) / kKW);
   if (r.kickMode >= 0) return r.kickMode == 1 && line_shape(r) != nullptr;
   if (getenv("VLS_FUSED_KICK")) return line_shape(r) != nullptr;
+  if (!getenv("VLS_NO_KICK_MOMENT")) return false;  // the default is kick_moment's schedule
   return line_shape(r) != nullptr && waves >= 512 && !getenv("VLS_NO_FUSED_KICK");
 }
 
@@ -1637,7 +1901,30 @@ size_t kick_scratch_doubles(const Run &r) {
   return tiles * r.nJ * (size_t)(r.nU + 3) * 64;
 }
 
+// The moment fused into the kick's column splines (k_kick_solve_moment),
+// then the evaluation a thread per point: the default for one species on
+// shapes with register lines (kickMode 2 or -1; VLS_NO_KICK_MOMENT turns the
+// default off).  The adjoint's recorded steps, which keep the column
+// splines, take the other schedules.
+static bool kick_moment(const Run &r) {
+  if (!line_shape(r) || r.nS != 1 || !r.mCnt) return false;
+  if (r.kickMode >= 0) return r.kickMode == 2;
+  return getenv("VLS_NO_KICK_MOMENT") == nullptr && !getenv("VLS_FUSED_KICK");
+}
+
 void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *dKeep) {
+  if (moment && !dKeep && kick_moment(r)) {
+    const LineShape *ls = line_shape(r);
+    const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt};
+    time_begin(t, s, "kick_solve");
+    hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64), 0, s, r.f, r.c,
+                       r.nJ, r.cs, 0, r.tv, ma);
+    time_mark(t, s, "kick_solve");
+    time_begin(t, s, "kick_eval");
+    launch_kick_eval(r, s, r.c);
+    time_mark(t, s, "kick_eval");
+    return;
+  }
   if (moment) {
     time_begin(t, s, "moment");
     launch_moment(r, s);
@@ -1710,8 +1997,9 @@ std::string describe(const Run &r, const Sched &c, bool cached) {
     sp = c.streamR ? std::string("r-solve streaming") : "r-solve LDS tile of " + std::to_string(c.TR) + " rows";
     sp += c.streamV ? ", v-solve streaming" : ", v-solve LDS tile of " + std::to_string(c.TC) + " columns";
   }
-  snprintf(buf, sizeof buf, "%dx%dx%d: %s, %s kick (spline %.1f us, kick %.1f us%s)", r.nR, r.nU,
-           r.nJ, sp.c_str(), c.kickMode == 1 ? "fused" : "two-kernel", c.splineUs, c.kickUs,
+  static const char *const kKick[] = {"two-kernel", "fused", "moment-fused"};
+  snprintf(buf, sizeof buf, "%dx%dx%d: %s, %s kick (spline %.1f us, moment and kick %.1f us%s)",
+           r.nR, r.nU, r.nJ, sp.c_str(), kKick[c.kickMode], c.splineUs, c.kickUs,
            cached ? ", cached" : "");
   return buf;
 }
@@ -1769,8 +2057,7 @@ std::string autotune(Run &r, hipStream_t s, int maxLds) {
       best.splineUs = us;
     }
   }
-  // the kick, on a copy of f (it works in place), with the acceleration of
-  // the current f
+  // the moment and the kick, on a copy of f (the kick works in place)
   apply_sched(r, best);
   launch_moment(r, s);
   double *f0 = r.f, *tmp = nullptr;
@@ -1781,10 +2068,17 @@ std::string autotune(Run &r, hipStream_t s, int maxLds) {
       r.kickMode = mode;
       VLS_HIP_CHECK(hipMemcpyAsync(tmp, f0, (size_t)r.nVal * sizeof(double),
                                    hipMemcpyDeviceToDevice, s));
-      return time_us(s, [&] { launch_spline_kick(r, s, nullptr, false); });
+      return time_us(s, [&] { launch_spline_kick(r, s, nullptr, true); });
     };
     best.kickMode = 0;
     best.kickUs = time_kick(0);
+    if (best.lineMode == 1 && r.nS == 1 && r.mCnt) {
+      const double us = time_kick(2);
+      if (us < best.kickUs) {
+        best.kickMode = 2;
+        best.kickUs = us;
+      }
+    }
     if (best.lineMode == 1) {
       if (!r.gsc) {
         r.kickMode = 1;
@@ -1825,6 +2119,10 @@ void launch_kick(Run &r, hipStream_t s, double *d) {
   else
     hipLaunchKernelGGL(k_kick_vsolve, dim3((r.nR + 63) / 64 * r.nJ), dim3(64), 0, s, r.f, d, r.nR,
                        r.nU, r.cs, 0, r.tv);
+  launch_kick_eval(r, s, d);
+}
+
+void launch_kick_eval(Run &r, hipStream_t s, const double *d) {
   hipLaunchKernelGGL(k_kick_cols, dim3(blocks_for((int64_t)r.nU * r.nR) * r.nJ), dim3(kBlock), 0, s,
                      r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
