@@ -66,6 +66,7 @@ def algorithmic_bytes(name, nR, nU, nJ):
         "kick_line": 8 * n + 8 * n,      # fused column splines + kick: read f, write f
         "kick_solve": 8 * n + 8 * cols,  # column splines and moment partials: read f, write d
         "kick_eval": 8 * cols + 8 * n,   # the kick from d: read d, write f
+        "kick_spline_r": 8 * cols + 8 * rows,  # the kicked rows' r-solve: read d, write rows
         "drift": 8 * coef + 8 * n,       # read coefficients, write f
     }
     return table.get(name)
@@ -73,9 +74,10 @@ def algorithmic_bytes(name, nR, nU, nJ):
 
 # The kernels of one step of the workload, as vls_set_timing_kernel names them
 # ("moment" covers the moment partials, the chunk sums and the scan;
-# "kick_solve" the kick's column splines with the moment).
-STEP_KERNELS = ("moment", "kick_solve", "kick_eval", "kick_line", "kick", "spline_r", "spline_v",
-                "drift")
+# "kick_solve" the kick's column splines with the moment; "kick_spline_r" the
+# kick's evaluation folded into the drift's r-solve).
+STEP_KERNELS = ("moment", "kick_solve", "kick_eval", "kick_spline_r", "kick_line", "kick",
+                "spline_r", "spline_v", "drift")
 STEP_BYTES_PER_POINT = 72
 
 # The adjoint's workload: PLDF power-law sphere (n=1, alpha=1, eta=0.1, M=1,

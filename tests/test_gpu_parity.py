@@ -162,6 +162,28 @@ def test_separate_moment_bit_exact(monkeypatch, env):
     _bit_exact_run(UDF_C2, 3, "separate moment")
 
 
+@pytest.mark.parametrize("cfg_name", ["UDF_C1", "UDF_C2", "UDF_STRONG_KICK"])
+def test_kick_in_rsolve_bit_exact(monkeypatch, cfg_name):
+    """The kick folded into the drift's r-solve (its evaluation from the
+    column splines staged per chunk in LDS; kicks that spread wider than the
+    window read the splines directly, as on the strong-kick grid)."""
+    import tests.configs as C
+
+    print(f"case {cfg_name.lower()} with VLS_KICK_RSOLVE", flush=True)
+    monkeypatch.setenv("VLS_KICK_RSOLVE", "1")
+    _bit_exact_run(getattr(C, cfg_name), 4, "kick in r-solve")
+
+
+@pytest.mark.parametrize("name", ["udf_c1", "hernquist"])
+def test_streaming_kick_moment_bit_exact(monkeypatch, name):
+    """The moment fused into the streaming column splines of the kick
+    (k_kick_vsolve_moment), forced on grids that have register lines."""
+    cfg, _ = SMALL_CASES[name]
+    print(f"case {name} with VLS_NO_REGISTER_LINES", flush=True)
+    monkeypatch.setenv("VLS_NO_REGISTER_LINES", "1")
+    _bit_exact_run(cfg, 6, "streaming kick and moment")
+
+
 @pytest.mark.parametrize("name", ["udf_c1", "pldf"])
 def test_moment_counters_across_runs(name):
     """The one-launch moment's counters return to zero after every launch:

@@ -149,8 +149,13 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
     vls::launch_drift(r, s);
     vls::time_mark(t, s, "drift");
   }
-  vls::launch_spline_kick(r, s, t);
-  vls::launch_spline(r, r.f, s, t);
+  if (vls::kick_in_rsolve(r)) {
+    vls::launch_kick_rsolve(r, s, t);
+    vls::launch_spline_v_timed(r, s, t);
+  } else {
+    vls::launch_spline_kick(r, s, t);
+    vls::launch_spline(r, r.f, s, t);
+  }
   vls::time_begin(t, s, "drift");
   vls::launch_drift(r, s);
   vls::time_mark(t, s, "drift");
@@ -163,9 +168,8 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
 // Without advance the step stops after the kick (the state after it is not
 // needed: the last step of a recomputed segment).
 void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut, bool advance = true) {
-  vls::launch_moment(r, s);
-  d2d(accOut, r.acc, r.nR, s);
-  vls::launch_spline_kick(r, s, nullptr, false, cK);
+  vls::launch_spline_kick(r, s, nullptr, true, cK);
+  d2d(accOut, r.acc, r.nR, s);  // the moment's acceleration, which the kick only reads
   if (!advance) return;
   vls::launch_spline(r, r.f, s, nullptr);
   vls::launch_drift(r, s);
@@ -475,7 +479,11 @@ int vls_create(const vls_geometry *g, const vls_solver_params *sp, int nRuns, vl
       r.Ulo = G.U_Val[0];
       r.Uhi = G.U_Val[r.nU - 1];
       vls::configure_tiles(r, maxLds);
-      r.f = dalloc<double>(r, r.nVal * r.nS);
+      // the kick folded into the r-solve leaves the solved rows (c's layout) in
+      // f's buffer between the kick and the drift
+      r.fRows = vls::kick_rsolve_shape(r);
+      r.f = dalloc<double>(r, r.fRows ? std::max<size_t>(r.nVal, (size_t)r.nJ * r.cs)
+                                      : (size_t)(r.nVal * r.nS));
       // A foot on the last knot (integer part dim-1, fraction 0) makes the
       // 4x4 stencil touch one column past its padded row and one row past its
       // slice, with weight zero, as in the reference; the tail pad keeps

@@ -66,8 +66,9 @@ struct Run {
   bool streamR = false, streamV = false;  // streaming solvers for long lines
   // schedule choices (vls_autotune): -1 from the shape and the environment;
   // lineMode 0 / 1: LDS-tiled or streaming solvers / register lines (shapes
-  // that have them); kickMode 0 / 1 / 2: two-kernel / fused kick / the moment
-  // fused into the kick's column splines
+  // that have them); kickMode 0 / 1 / 2 / 3: two-kernel / fused kick / the
+  // moment fused into the kick's column splines / that, and the kick's
+  // evaluation folded into the drift's r-solve
   int lineMode = -1, kickMode = -1;
   vls_solver_params sp{};
   double start_v = 0, hinv_v = 0, Ulo = 0, Uhi = 0;
@@ -76,6 +77,8 @@ struct Run {
   double *f = nullptr;      // [nS][nJ][nU][nR]
   double *c = nullptr;      // [nJ][nU+2][nR+2] spline coefficients
   double *gsc = nullptr;    // fused kick: per-wave scratch for wide kicks
+  bool fRows = false;       // f's buffer holds nJ*cs doubles: the kick folded into the r-solve
+                            // leaves its rows there (f is rewritten by the drift)
   bool kickScratch = false; // fused kick: always take the scratch path
   // static feet, interleaved (xr, xu) per point: one 16-byte load each
   // (xr <= -1 marks points outside the grid or on delay lines)
@@ -171,6 +174,15 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t);
 void launch_kick(Run &r, hipStream_t s, double *d);
 // the kick's evaluation from the column splines d (layout of launch_kick)
 void launch_kick_eval(Run &r, hipStream_t s, const double *d);
+// the kick folded into the drift's r-solve: the shapes that have it (f's
+// buffer is then allocated with nJ*cs doubles), whether this run takes it,
+// and the launch (moment and the kick's column splines into c, then the
+// kicked rows' r-solve into f's buffer); then launch_spline_v_timed (those
+// rows into c) and the drift
+bool kick_rsolve_shape(const Run &r);
+bool kick_in_rsolve(const Run &r);
+void launch_kick_rsolve(Run &r, hipStream_t s, Timing *t);
+void launch_spline_v_timed(Run &r, hipStream_t s, Timing *t);
 // (moment,) kick of the plane r.f; without the moment the acceleration of
 // the last launch_moment is used; with dKeep the kick's column coefficients
 // are also left there ([j][m][r], slice stride cs)

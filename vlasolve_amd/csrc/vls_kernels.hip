@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -434,9 +435,11 @@ __device__ __forceinline__ void head_store(HeadTab &t, const HeadRegs &h, int la
 // Forward elimination over rows 1..M-2.  RHS(k) returns rhs_k (any k <= M-2),
 // STORE(k, y) keeps y_k.  Row 1 takes y_0's contribution A03 in place of
 // y_{k-1}.  Returns y_{M-3} and y_{M-2} through ym2/ym1.
-template <int kPF, class RHS, class STORE>
+// CONSUME(k, rhs_k), if given, sees each right-hand side once, in order, at
+// the step that uses it.
+template <int kPF, class RHS, class STORE, class CONSUME>
 __device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS rhs, STORE store,
-                                               double &ym2, double &ym1) {
+                                               double &ym2, double &ym1, CONSUME consume) {
   const int M = ax.M, P = ax.K0 - 1 < M - 2 ? ax.K0 - 1 : M - 2;  // head rows 1..P
   auto kc = [&](int k) { return k <= M - 2 ? k : M - 2; };
   double hr[kHead], hd[kHead], he[kHead], rb[kPF];
@@ -456,6 +459,7 @@ __device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS
 #pragma unroll
   for (int q = 0; q < kHead; q++) {
     if (q < P) {
+      consume(1 + q, hr[q]);
       double yk = div_known(hr[q] - c16 * ym1, hd[q], he[q]);
       store(1 + q, yk);
       ym2 = ym1;
@@ -467,6 +471,7 @@ __device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS
     for (int q = 0; q < kPF; q++) {  // rhs_kk sits in slot (kk-P-1) % kPF == q
       const int kk = k + q;
       if (kk <= M - 2) {
+        consume(kk, rb[q]);
         double yk = div_known(rb[q] - c16 * ym1, dS, rdS);
         store(kk, yk);
         ym2 = ym1;
@@ -475,6 +480,12 @@ __device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS
       rb[q] = rhs(kc(kk + kPF));
     }
   }
+}
+
+template <int kPF, class RHS, class STORE>
+__device__ __forceinline__ void thomas_forward(const ThomasAxis &ax, int lz, RHS rhs, STORE store,
+                                               double &ym2, double &ym1) {
+  thomas_forward<kPF>(ax, lz, rhs, store, ym2, ym1, [](int, double) {});
 }
 
 // Back substitution c_k = y_k - u_k c_{k+1} for k = M-2..1 and the first row,
@@ -767,7 +778,8 @@ k_spline_r_stream(const double *__restrict__ f, double *__restrict__ c, int nR, 
 // K0 <= kHead+1 on equal the fixed point), later rows the fixed point dS,
 // rdS, uS.
 //   KICK = false: the nR+2 = S padded columns (j, col); rhs_k = c[j][k][col],
-//                 c_k -> c[j][k][col]      (create_2D second loop)
+//                 c_k -> c[j][k][col]      (create_2D second loop); with src
+//                 non-null the right-hand sides come from src (same layout)
 //   KICK = true : the nR columns of f (j, r); rhs_k = f[j][k-1][r],
 //                 d_k -> d[j][k][r] (slice stride cs, row stride nR): the
 //                 kick's column splines (see k_kick_line)
@@ -796,8 +808,8 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     int col = bx * 64 + lane;
     live = col < nR + 2;
     if (!live) col = nR + 1;
-    in = c + (size_t)j * cs + col;
-    out = const_cast<double *>(in);
+    in = (src ? src : c) + (size_t)j * cs + col;
+    out = c + (size_t)j * cs + col;
     is = S;  // = nR + 2, a compile-time stride
     os = S;
   }
@@ -1291,6 +1303,80 @@ k_kick_vsolve(const double *__restrict__ f, double *__restrict__ d, int nR, int 
       [&](int k, double v) { dc[(int64_t)k * nR] = v; });
 }
 
+// The kick's value of a point from the four column-spline coefficients of
+// its target rows n..n+3 and its offset y in the cell (evaluateNxPy,
+// src/localSpline.hxx:423-441, in the column form)
+__device__ __forceinline__ double kick_value(double y, double d0, double d1, double d2, double d3) {
+  double result = 1.0 / 6.0;
+  result *= BV2(1. - y) * d0 + BV1(1. - y) * d1 + BV1(y) * d2 + BV2(y) * d3;
+  return result;
+}
+
+// k_kick_vsolve with the moment of the same f (as k_kick_solve_moment does
+// for the register lines): tiles of 63 columns, lane 63 solving the next
+// tile's first column for its values only; the moment's terms are taken as
+// the raw rows are used by the forward sweep, the group and block sums as
+// left folds (((G0 + G1) + G2) + G3 with the groups past the last row as 0),
+// then moment_tail.  Dynamic LDS: nU + nR doubles.
+__global__ void __launch_bounds__(64)
+k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int nR, int nU, int nJ,
+                     int64_t ds, int lzmask, ThomasAxis ax, MomentArgs ma) {
+  extern __shared__ double mlds[];  // dU [nU], then the scan's [nR]
+  __shared__ int last;
+  double *dUs = mlds, *sdm = mlds + nU;
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int nX = (nR + 62) / 63;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nX, x = tile - j * nX;
+  const int colRaw = x * 63 + lane;
+  const int col = colRaw < nR ? colRaw : nR - 1;
+  const bool owner = lane < 63 && colRaw < nR;
+  const bool term = lane < 63 && colRaw < nR - 1;
+  const double *fc = f + (size_t)j * nU * nR + col;
+  double *dc = d + (size_t)j * ds + col;
+  for (int u = lane; u < nU; u += 64) dUs[u] = ma.dU[u];
+  const double ct = ma.Ct1[j], dr = term ? ma.dR[colRaw] : 0.0;
+  __syncthreads();
+  const int nW = (nU + 63) / 64;
+  double aOwn = 0.0, aN = 0.0, g = 0.0, W = 0.0, Pj = 0.0, duNext = dUs[0];
+  // raw row k - 1 of the column: the term of row u = k - 2
+  auto consume = [&](int k, double b) {
+    const double bN = lane_next(b);
+    if (k >= 2) {
+      const int u = k - 2;
+      const double du = duNext;
+      if (u + 1 < nU) duNext = dUs[u + 1];
+      const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+      if (u % 16 == 0) g = 0.0;
+      g += ((ct * du) * dr) * avg;
+      const int q = (u / 16) % 4;
+      if (u % 16 == 15 || u == nU - 2) W = q == 0 ? g : W + g;
+      if (u % 64 == 63 || u == nU - 2) {
+        for (int q2 = q + 1; q2 < 4; q2++) W = W + 0.0;
+        Pj = u / 64 == 0 ? W : Pj + W;
+      }
+    }
+    aOwn = b;
+    aN = bN;
+  };
+  double ym2, ym1;
+  thomas_forward<16>(
+      ax, lz, [&](int k) { return fc[(int64_t)(k - 1) * nR]; },
+      [&](int k, double y) {
+        if (owner) dc[(int64_t)k * nR] = y;
+      },
+      ym2, ym1, consume);
+  for (int w = (nU - 2) / 64 + 1; w < nW; w++) Pj = Pj + 0.0;
+  thomas_backward<16>(
+      ax, lz, ym2, ym1, [&](int k) { return dc[(int64_t)k * nR]; },
+      [&](int k, double v) {
+        if (owner) dc[(int64_t)k * nR] = v;
+      });
+  if (owner) coherent_store(&ma.P[(size_t)j * nR + colRaw], term ? Pj : 0.0);
+  moment_tail(ma, j, x, nR, nJ, sdm, &last);
+}
+
 // A 1D grid of nT blocks per slice in the order of xcd_tile: the blocks of a
 // slice, which read the same rows of d (each row serves four rows of
 // points), run on one XCD and share its L2.
@@ -1319,9 +1405,7 @@ k_kick_cols(double *__restrict__ f, const double *__restrict__ d, const double *
   // BV2(0) is 0: the last row is read instead
   const double d0 = dc[(int64_t)n * nR], d1 = dc[(int64_t)(n + 1) * nR];
   const double d2 = dc[(int64_t)(n + 2) * nR], d3 = dc[(int64_t)min(n + 3, nU + 1) * nR];
-  double result = 1.0 / 6.0;
-  result *= BV2(1. - y) * d0 + BV1(1. - y) * d1 + BV1(y) * d2 + BV2(y) * d3;
-  f[i] = result;
+  f[i] = kick_value(y, d0, d1, d2, d3);
 }
 
 // ---------------------------------------------------------------- drift
@@ -1810,6 +1894,264 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
   }
 }
 
+// The kick folded into the drift's r-solve: the right-hand sides of a row are
+// the kicked values of the row's points, evaluated from the kick's column
+// splines d ([j][m][r], slice stride cs, row stride nR; k_kick_solve_moment)
+// as k_kick_cols evaluates them, so the kicked f never goes to memory.  The
+// solved rows go to c (rows 1..nU of c's layout), which may be another
+// buffer than the one d is in.  A
+// wave takes 64 rows of a slice (tiles in the order of xcd_tile) and, as
+// k_line_rows, a lane per row with the row in VGPRs (the first kYL forward
+// values in LDS); the rows are solved a chunk of 16 columns at a time.  The
+// rows m of d that a chunk's points read lie in [lo, hi], from the extreme
+// points of the chunk (the target row n grows with U[u] + a[r] in both
+// operands); they are staged in LDS, kWin rows at most, with the loads of
+// the next chunk's window in flight during the chunk's evaluation and
+// elimination.  A chunk whose window is taller (kicks that differ by more
+// than ~12 cells across 16 columns) reads d directly.  Same values, in the
+// same order, as k_kick_cols followed by k_line_rows.
+template <int M, int Mv>
+struct RowsKickCfg {
+  static constexpr int kYL = M > 160 ? 32 : 0;
+};
+
+template <int M, int Mv>
+__global__ void __launch_bounds__(64)
+k_line_rows_kick(const double *__restrict__ d, double *__restrict__ c,
+                 const double *__restrict__ U_Val, const double *__restrict__ acc, double Ulo,
+                 double Uhi, double start_v, double hinv_v, int64_t cs, int lzmask,
+                 ThomasAxis ax) {
+  constexpr int nR = M - 2, nU = Mv - 2;
+  constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
+  constexpr int nChIn = (nR + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
+  constexpr int kWin = 80, kWQ = kWin / 4;  // window rows, window loads per lane
+  constexpr int kB = 8;                     // points per batch of window reads
+  constexpr int nT = (nU + 63) / 64;        // row tiles per slice
+  constexpr int kYL = RowsKickCfg<M, Mv>::kYL;
+  __shared__ double chunk[64 * kP];
+  __shared__ double win[kWin * kP];
+  __shared__ double accs[nR];
+  __shared__ double us[64];
+  __shared__ double ylds[(kYL > 0 ? kYL : 1) * 64];
+  __shared__ HeadTab htab;
+  const int lane = threadIdx.x;
+  const int lz = lane & lzmask;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = tile / nT;
+  const int u0 = (tile - j * nT) * 64;
+  const int rows = min(64, nU - u0);
+  // staging role: 4 rows x kC columns per pass, a lane's points are rows
+  // q*4 + rsub (q = 0..15) of column ksub of the chunk
+  const int rsub = lane / kC, ksub = lane % kC;
+  const double *dj = d + (size_t)j * cs;
+  double *dst = c + (size_t)j * cs + (size_t)(u0 + 1) * M;
+  const HeadRegs hr = head_load(ax, lane, lz);
+  const double Uown = U_Val[u0 + min(lane, rows - 1)];  // extra rows: a copy of the last
+  const double Ufirst = U_Val[u0], Ulast = U_Val[u0 + rows - 1];
+  constexpr int nAq = (nR + 63) / 64;
+  double aq[nAq];
+#pragma unroll
+  for (int q = 0; q < nAq; q++) aq[q] = acc[min(lane + 64 * q, nR - 1)];
+  // the target row n of a point at velocity newU after the kick, with the
+  // evaluation's own operations, clamped to the padded column
+  auto row_of = [&](double newU) {
+    double yy = (newU - start_v) * hinv_v;
+    yy = yy < 0.0 ? 0.0 : (yy > (double)(nU + 1) ? (double)(nU + 1) : yy);
+    return (int)trunc(yy);
+  };
+  // window rows [lo, hi] of chunk ch; false: taller than kWin
+  auto bounds = [&](int ch, int &lo, int &hi) {
+    double amin = accs[min(ch * kC, nR - 1)], amax = amin;
+#pragma unroll
+    for (int i = 1; i < kC; i++) {
+      const double a = accs[min(ch * kC + i, nR - 1)];
+      amin = a < amin ? a : amin;
+      amax = a > amax ? a : amax;
+    }
+    lo = row_of(Ufirst + amin);
+    hi = min(row_of(Ulast + amax) + 3, nU + 1);
+    return hi - lo + 1 <= kWin;
+  };
+  auto load_win = [&](int ch, int lo, int hi, double *t) {
+    const int e = min(ch * kC + ksub, nR - 1);
+#pragma unroll
+    for (int q = 0; q < kWQ; q++) t[q] = dj[(size_t)min(lo + q * 4 + rsub, hi) * nR + e];
+  };
+  auto store_win = [&](const double *t) {
+#pragma unroll
+    for (int q = 0; q < kWQ; q++) win[(q * 4 + rsub) * kP + ksub] = t[q];
+  };
+  auto wave_sync = [] {  // the block is one wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // The kicked values of this lane's points of chunk ch (k_kick_cols), into
+  // the chunk's LDS transpose: per batch of kB points the targets first,
+  // then all their coefficient reads, then the sums.  Points outside the
+  // velocity range are 0 (their reads take row lo and are not used).
+  auto kick_chunk = [&](int ch, int lo, bool inWin) {
+    const int e = min(ch * kC + ksub, nR - 1);
+    const double a = accs[e];
+#pragma unroll
+    for (int b0 = 0; b0 < 16; b0 += kB) {
+      double fr[kB], dv[kB][4];
+      int m0[kB], m3[kB];
+      bool in[kB];
+#pragma unroll
+      for (int i = 0; i < kB; i++) {
+        const double newU = us[(b0 + i) * 4 + rsub] + a;
+        in[i] = !(newU <= Ulo) && !(newU >= Uhi);
+        double yy = in[i] ? (newU - start_v) * hinv_v : 0.0;
+        int n;
+        split(yy, n, yy);
+        fr[i] = yy;
+        m0[i] = in[i] ? n : lo;
+        m3[i] = in[i] ? min(n + 3, nU + 1) : lo;
+      }
+      if (inWin) {
+#pragma unroll
+        for (int i = 0; i < kB; i++) {
+          const double *w = win + (m0[i] - lo) * kP + ksub;
+          dv[i][0] = w[0];
+          dv[i][1] = w[kP];
+          dv[i][2] = w[2 * kP];
+          dv[i][3] = win[(m3[i] - lo) * kP + ksub];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kB; i++) {
+          const double *dc = dj + e;
+          dv[i][0] = dc[(int64_t)m0[i] * nR];
+          dv[i][1] = dc[(int64_t)(m0[i] + 1) * nR];
+          dv[i][2] = dc[(int64_t)(m0[i] + 2) * nR];
+          dv[i][3] = dc[(int64_t)m3[i] * nR];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kB; i++) {
+        const double v = kick_value(fr[i], dv[i][0], dv[i][1], dv[i][2], dv[i][3]);
+        chunk[((b0 + i) * 4 + rsub) * kP + ksub] = in[i] ? v : 0.0;
+      }
+    }
+  };
+
+#pragma unroll
+  for (int q = 0; q < nAq; q++)
+    if (lane + 64 * q < nR) accs[lane + 64 * q] = aq[q];
+  us[lane] = Uown;
+  head_store(htab, hr, lane);
+  wave_sync();
+  double wt[kWQ];
+  int lo0, hi0, lo1 = 0, hi1 = 0;
+  bool in0 = bounds(0, lo0, hi0), in1 = false;
+  load_win(0, lo0, hi0, wt);
+  store_win(wt);
+  if (nChIn > 1) {
+    in1 = bounds(1, lo1, hi1);
+    load_win(1, lo1, hi1, wt);
+  }
+  wave_sync();
+  double y[M];
+  const double c16 = ax.c16 + (double)lz, dS = ax.dS + (double)lz, rdS = ax.rdS + (double)lz;
+  double prev = ax.A03;
+  // Software pipeline: the evaluation of chunk ch + 1 (VALU-heavy, from the
+  // window in LDS into the chunk transpose) shares a step with the
+  // elimination of chunk ch (a dependent chain that leaves the VALU idle),
+  // whose right-hand sides are already in registers.  Each step then moves
+  // chunk ch + 1 into registers and the window of ch + 2 into LDS.
+  auto take_chunk = [&](int ch) {
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < nR) y[1 + ch * kC + i] = chunk[lane * kP + i];
+  };
+  auto next_window = [&](int ch) {  // window ch into LDS, the loads of ch + 1 out
+    if (ch < nChIn) {
+      store_win(wt);
+      lo0 = lo1;
+      hi0 = hi1;
+      in0 = in1;
+      if (ch + 1 < nChIn) {
+        in1 = bounds(ch + 1, lo1, hi1);
+        load_win(ch + 1, lo1, hi1, wt);
+      }
+    }
+  };
+  kick_chunk(0, lo0, in0);
+  wave_sync();
+  take_chunk(0);
+  next_window(1);
+  wave_sync();
+#pragma unroll
+  for (int ch = 0; ch < nChIn; ch++) {
+    if (ch + 1 < nChIn) kick_chunk(ch + 1, lo0, in0);
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+      const int k = 1 + ch * kC + i;
+      if (k <= M - 2) {
+        double dv = dS, ev = rdS;
+        if (k <= kHead) {
+          dv = htab.d[k];
+          ev = htab.rd[k];
+        }
+        const double yk = div_known(y[k] - c16 * prev, dv, ev);
+        if (k <= kYL)
+          ylds[(k - 1) * 64 + lane] = yk;
+        else
+          y[k] = yk;
+        prev = yk;
+      }
+    }
+    wave_sync();
+    if (ch + 1 < nChIn) {
+      take_chunk(ch + 1);
+      next_window(ch + 2);
+    }
+    wave_sync();
+  }
+  double tt = 0.0 - ax.hi2 * (M - 3 <= kYL ? ylds[(M - 4) * 64 + lane] : y[M - 3]);
+  tt = tt - ax.p * y[M - 2];
+  y[M - 1] = div_known(tt, ax.q, ax.rq);
+  const double uS = ax.uS + (double)lz;
+  double cn = y[M - 1], c2 = 0.0;
+  // back substitution and stores, last chunk first (as k_line_rows)
+#pragma unroll
+  for (int ch = nChOut - 1; ch >= 0; ch--) {
+#pragma unroll
+    for (int i = kC - 1; i >= 0; i--) {
+      const int k = ch * kC + i;
+      if (k >= 1 && k <= M - 2) {
+        double uv = uS;
+        if (k <= kHead) uv = htab.u[k];
+        const double yk = k <= kYL ? ylds[(k - 1) * 64 + lane] : y[k];
+        c2 = cn;
+        cn = yk - uv * cn;
+        y[k] = cn;
+      }
+    }
+    if (ch == 0) y[0] = (ax.A03 - ax.A01 * cn) - ax.A02 * c2;
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+      if (ch * kC + i < M) chunk[lane * kP + i] = y[ch * kC + i];
+    wave_sync();
+    const int k = ch * kC + ksub;
+    double sv[kC];
+#pragma unroll
+    for (int q = 0; q < kC; q++) sv[q] = chunk[(q * 4 + rsub) * kP + ksub];
+    wave_sync();
+    if (rows == 64 && (ch + 1) * kC <= M) {
+#pragma unroll
+      for (int q = 0; q < kC; q++) dst[(q * 4 + rsub) * M + k] = sv[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < kC; q++) {
+        const int row = q * 4 + rsub;
+        if (row < rows && k < M) dst[row * M + k] = sv[q];
+      }
+    }
+  }
+}
+
 // Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
 // standard grids.  Other shapes use the LDS-tiled kernels above.
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
@@ -1817,17 +2159,21 @@ using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, T
 using KickFn = void (*)(double *, const double *, const double *, double *, double, double, double,
                         double, int, int, ThomasAxis, double *, int64_t);
 using KickMomentFn = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, MomentArgs);
+using RowsKickFn = void (*)(const double *, double *, const double *, const double *, double, double,
+                            double, double, int64_t, int, ThomasAxis);
 struct LineShape {
   int nR, nU;
   RowsFn rows;
   LineFn cols, kickCols;
   KickFn kick, kickKeep;
   KickMomentFn kickMoment;
+  RowsKickFn rowsKick;
 };
 #define VLS_LINE_SHAPE(R, U)                                                          \
   {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>,                      \
    k_line<(U) + 2, true, (R) + 2>, k_kick_line<(U) + 2, (R) + 2, false>,             \
-   k_kick_line<(U) + 2, (R) + 2, true>, k_kick_solve_moment<(U) + 2, (R) + 2>}
+   k_kick_line<(U) + 2, (R) + 2, true>, k_kick_solve_moment<(U) + 2, (R) + 2>,       \
+   k_line_rows_kick<(R) + 2, (U) + 2>}
 static const LineShape kLineShapes[] = {
     VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
     VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
@@ -1840,6 +2186,13 @@ static const LineShape *line_shape(const Run &r) {
   for (const LineShape &ls : kLineShapes)
     if (ls.nR == r.nR && ls.nU == r.nU) return &ls;
   return nullptr;
+}
+
+bool kick_rsolve_shape(const Run &r) {
+  if (r.nS != 1) return false;
+  for (const LineShape &ls : kLineShapes)
+    if (ls.nR == r.nR && ls.nU == r.nU) return true;
+  return false;
 }
 
 // The drift's r-direction solves (rows of src -> padded rows of c) and its
@@ -1858,10 +2211,12 @@ static void launch_spline_r(Run &r, const double *src, hipStream_t s) {
   }
 }
 
-static void launch_spline_v(Run &r, const double *src, hipStream_t s) {
+// rowsIn: the r-solved rows in c's layout (null: in c, solved in place;
+// otherwise register lines only)
+static void launch_spline_v(Run &r, const double *rowsIn, hipStream_t s) {
   if (const LineShape *ls = line_shape(r)) {
-    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, src, r.c, r.nR,
-                       r.nU, r.nJ, r.cs, 0, r.tv);
+    hipLaunchKernelGGL(ls->cols, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, rowsIn, r.c,
+                       r.nR, r.nU, r.nJ, r.cs, 0, r.tv);
   } else if (r.streamV) {
     hipLaunchKernelGGL(k_spline_v_stream, dim3((r.nR + 2 + 63) / 64 * r.nJ), dim3(64), 0, s, r.c,
                        r.nR, r.cs, 0, r.tv);
@@ -1877,7 +2232,7 @@ void launch_spline(Run &r, const double *src, hipStream_t s, Timing *t) {
   launch_spline_r(r, src, s);
   time_mark(t, s, "spline_r");
   time_begin(t, s, "spline_v");
-  launch_spline_v(r, src, s);
+  launch_spline_v(r, nullptr, s);
   time_mark(t, s, "spline_v");
 }
 
@@ -1901,27 +2256,66 @@ size_t kick_scratch_doubles(const Run &r) {
   return tiles * r.nJ * (size_t)(r.nU + 3) * 64;
 }
 
-// The moment fused into the kick's column splines (k_kick_solve_moment),
-// then the evaluation a thread per point: the default for one species on
-// shapes with register lines (kickMode 2 or -1; VLS_NO_KICK_MOMENT turns the
-// default off).  The adjoint's recorded steps, which keep the column
-// splines, take the other schedules.
+static bool kick_moment_possible(const Run &r) { return r.nS == 1 && r.mCnt; }
+
+// The moment fused into the kick's column splines (k_kick_solve_moment on
+// register lines, k_kick_vsolve_moment on the others), then the evaluation
+// a thread per point: the default for one species (kickMode 2 or -1;
+// VLS_NO_KICK_MOMENT turns the default off).  The adjoint's recorded steps,
+// which keep the column splines, take it too (the splines go to the kept
+// buffer instead of c).
 static bool kick_moment(const Run &r) {
-  if (!line_shape(r) || r.nS != 1 || !r.mCnt) return false;
+  if (!kick_moment_possible(r)) return false;
   if (r.kickMode >= 0) return r.kickMode == 2;
   return getenv("VLS_NO_KICK_MOMENT") == nullptr && !getenv("VLS_FUSED_KICK");
 }
 
+// The kick folded into the drift's r-solve (kickMode 3, or VLS_KICK_RSOLVE):
+// k_kick_solve_moment into c, then k_line_rows_kick into f's buffer.  Same
+// bits as kick_moment's schedule; at 201x201x200 no faster (the evaluation
+// then runs at one wave per SIMD with the row solves), so not the default.
+bool kick_in_rsolve(const Run &r) {
+  if (!r.fRows || !line_shape(r) || !kick_moment_possible(r)) return false;
+  if (r.kickMode >= 0) return r.kickMode == 3;
+  return getenv("VLS_KICK_RSOLVE") != nullptr;
+}
+
+void launch_kick_rsolve(Run &r, hipStream_t s, Timing *t) {
+  const LineShape *ls = line_shape(r);
+  const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt};
+  time_begin(t, s, "kick_solve");
+  hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64), 0, s, r.f, r.c, r.nJ,
+                     r.cs, 0, r.tv, ma);
+  time_mark(t, s, "kick_solve");
+  // f is not needed again before the drift rewrites it: its buffer takes the
+  // kicked rows' r-solve (the working set stays f, c and the feet)
+  time_begin(t, s, "kick_spline_r");
+  hipLaunchKernelGGL(ls->rowsKick, dim3((r.nU + 63) / 64 * r.nJ), dim3(64), 0, s, r.c, r.f,
+                     r.U_Val, r.acc, r.Ulo, r.Uhi, r.start_v, r.hinv_v, r.cs, 0, r.tr);
+  time_mark(t, s, "kick_spline_r");
+}
+
+void launch_spline_v_timed(Run &r, hipStream_t s, Timing *t) {
+  time_begin(t, s, "spline_v");
+  launch_spline_v(r, r.f, s);  // the kicked rows' r-solve left them in f's buffer
+  time_mark(t, s, "spline_v");
+}
+
 void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *dKeep) {
-  if (moment && !dKeep && kick_moment(r)) {
-    const LineShape *ls = line_shape(r);
+  if (moment && kick_moment(r)) {
+    // the column splines into c, or where the adjoint keeps them
+    double *d = dKeep ? dKeep : r.c;
     const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt};
+    const dim3 grid((r.nR + 62) / 63 * r.nJ);
     time_begin(t, s, "kick_solve");
-    hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64), 0, s, r.f, r.c,
-                       r.nJ, r.cs, 0, r.tv, ma);
+    if (const LineShape *ls = line_shape(r))
+      hipLaunchKernelGGL(ls->kickMoment, grid, dim3(64), 0, s, r.f, d, r.nJ, r.cs, 0, r.tv, ma);
+    else
+      hipLaunchKernelGGL(k_kick_vsolve_moment, grid, dim3(64), (size_t)(r.nU + r.nR) * sizeof(double),
+                         s, r.f, d, r.nR, r.nU, r.nJ, r.cs, 0, r.tv, ma);
     time_mark(t, s, "kick_solve");
     time_begin(t, s, "kick_eval");
-    launch_kick_eval(r, s, r.c);
+    launch_kick_eval(r, s, d);
     time_mark(t, s, "kick_eval");
     return;
   }
@@ -1997,8 +2391,9 @@ std::string describe(const Run &r, const Sched &c, bool cached) {
     sp = c.streamR ? std::string("r-solve streaming") : "r-solve LDS tile of " + std::to_string(c.TR) + " rows";
     sp += c.streamV ? ", v-solve streaming" : ", v-solve LDS tile of " + std::to_string(c.TC) + " columns";
   }
-  static const char *const kKick[] = {"two-kernel", "fused", "moment-fused"};
-  snprintf(buf, sizeof buf, "%dx%dx%d: %s, %s kick (spline %.1f us, moment and kick %.1f us%s)",
+  static const char *const kKick[] = {"two-kernel", "fused", "moment-fused", "r-solve-folded"};
+  snprintf(buf, sizeof buf,
+           "%dx%dx%d: %s, %s kick (spline %.1f us, moment, kick and r-solve %.1f us%s)",
            r.nR, r.nU, r.nJ, sp.c_str(), kKick[c.kickMode], c.splineUs, c.kickUs,
            cached ? ", cached" : "");
   return buf;
@@ -2041,7 +2436,7 @@ std::string autotune(Run &r, hipStream_t s, int maxLds) {
     if (tc > 0 && (size_t)tc * (r.nU + 2) * sizeof(double) > (size_t)maxLds) continue;
     probe.streamV = tc < 0;
     if (tc > 0) probe.TC = tc;
-    const double us = time_us(s, [&] { launch_spline_v(probe, r.f, s); });
+    const double us = time_us(s, [&] { launch_spline_v(probe, nullptr, s); });
     if (us < bestV) {
       bestV = us;
       best.streamV = probe.streamV;
@@ -2057,26 +2452,39 @@ std::string autotune(Run &r, hipStream_t s, int maxLds) {
       best.splineUs = us;
     }
   }
-  // the moment and the kick, on a copy of f (the kick works in place)
+  // the moment, the kick and the drift's r-solve (which the kick can be
+  // folded into), on a copy of f (the kick works in place)
   apply_sched(r, best);
   launch_moment(r, s);
   double *f0 = r.f, *tmp = nullptr;
-  VLS_HIP_CHECK(hipMalloc(&tmp, (size_t)r.nVal * sizeof(double)));
+  // as large as f's buffer, which the kick folded into the r-solve writes
+  const size_t fDoubles = r.fRows ? std::max<size_t>(r.nVal, (size_t)r.nJ * r.cs) : r.nVal;
+  VLS_HIP_CHECK(hipMalloc(&tmp, fDoubles * sizeof(double)));
   r.f = tmp;
   try {
     auto time_kick = [&](int mode) {
       r.kickMode = mode;
       VLS_HIP_CHECK(hipMemcpyAsync(tmp, f0, (size_t)r.nVal * sizeof(double),
                                    hipMemcpyDeviceToDevice, s));
-      return time_us(s, [&] { launch_spline_kick(r, s, nullptr, true); });
+      return time_us(s, [&] {
+        if (mode == 3) {
+          launch_kick_rsolve(r, s, nullptr);
+        } else {
+          launch_spline_kick(r, s, nullptr, true);
+          launch_spline_r(r, r.f, s);
+        }
+      });
     };
     best.kickMode = 0;
     best.kickUs = time_kick(0);
-    if (best.lineMode == 1 && r.nS == 1 && r.mCnt) {
-      const double us = time_kick(2);
-      if (us < best.kickUs) {
-        best.kickMode = 2;
-        best.kickUs = us;
+    if (kick_moment_possible(r)) {
+      for (int mode : {2, 3}) {
+        if (mode == 3 && (!r.fRows || best.lineMode != 1)) continue;
+        const double us = time_kick(mode);
+        if (us < best.kickUs) {
+          best.kickMode = mode;
+          best.kickUs = us;
+        }
       }
     }
     if (best.lineMode == 1) {
