@@ -428,6 +428,8 @@ def main():
     ap.add_argument("--hq-steps", type=int, default=HQ_STEPS,
                     help="steps of the Hernquist adjoint leg (shorter runs for profiling)")
     ap.add_argument("--hq-ck", type=int, default=HQ_CK, help="its checkpoint interval")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batch of 8 Hernquist adjoint runs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -504,7 +506,7 @@ def main():
             hq = {"ratio": None, "error": str(e)[:200]}
 
     hqb = None
-    if not args.no_hernquist:
+    if not args.no_hernquist and not args.no_batch:
         try:
             hqb = adjoint_batch_ratio(V)
         except Exception as e:  # reported, not fatal for the forward figure

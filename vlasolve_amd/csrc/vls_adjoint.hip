@@ -216,7 +216,12 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
                 double *__restrict__ g, double *__restrict__ abp, int nR, int nU, int64_t cs,
                 double Ulo, double Uhi, double start_v, double hinv_v, ThomasAxis ax) {
   extern __shared__ double Us[];  // [nU]
+  // the head pivots in LDS: a table load inside the walk's loop would make
+  // every row's step wait for all the points' loads in flight (the vector
+  // memory counter retires in order)
+  __shared__ double uh[kHead + 1];
   for (int u = threadIdx.x; u < nU; u += 64) Us[u] = U_Val[u];
+  if (threadIdx.x <= kHead && (int)threadIdx.x < ax.M) uh[threadIdx.x] = ax.u[threadIdx.x];
   __syncthreads();
   const Tile2 tl = xcd_tile2((nR + 63) / 64);
   const int r = tl.x * 64 + threadIdx.x;
@@ -238,7 +243,7 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
     if (m == 0) c0 = v;
     if (m == 1) v = fma(-ax.A01, c0, v);
     if (m == 2) v = fma(-ax.A02, c0, v);
-    if (m >= 2) v = fma(-(m - 1 <= kHead ? ax.u[m - 1] : ax.uS), prev, v);
+    if (m >= 2) v = fma(-(m - 1 <= kHead ? uh[m - 1] : ax.uS), prev, v);
     prev = v;
     gc[(int64_t)m * nR] = v;
     w0 = w1;
@@ -250,11 +255,13 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
   // kQ points ahead: the loads of point u+kQ (its value, its target row and
   // the coefficient rows there) fly while points u..u+kQ-1 go through the
   // window; the v nodes come from LDS
+  // (the target row comes from the v nodes alone, so that the point's five
+  // loads leave together; whether its value is 0 is looked at when it is used)
   auto load = [&](int u, double &lam, double &t, int &n, bool &ok, double (&gk)[4]) {
     const int uc = min(u, nU - 1);
     lam = fj[(int64_t)uc * nR];
     const double newU = Us[uc] + a;
-    ok = u < nU && lam != 0.0 && !(newU <= Ulo) && !(newU >= Uhi);
+    ok = u < nU && !(newU <= Ulo) && !(newU >= Uhi);
     split(ok ? (newU - start_v) * hinv_v : 0.0, n, t);
 #pragma unroll
     for (int k = 0; k < 4; k++) gk[k] = dj[(int64_t)min(n + k, nU + 1) * nR];
@@ -268,7 +275,7 @@ k_adj_kick_walk(const double *__restrict__ fsb, const double *__restrict__ c,
   for (int u0 = 0; u0 < nU; u0 += kQ) {
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
-      if (okQ[q]) {  // (false past the last point)
+      if (okQ[q] && lamQ[q] != 0.0) {  // (false past the last point)
         const double lam = lamQ[q], t = tQ[q];
         const int n = nQ[q];
         // acceleration term over the rows n..n+3 inside the padded column
