@@ -156,20 +156,61 @@ __device__ __forceinline__ double adj_down(const ThomasAxis &ax, int k, double v
   return v;
 }
 
-// v lines (lane = padded column), in place: coalesced rows of 64 columns
+// The head rows of the pivot tables (u, rd for rows 0..kHead) in LDS, for the
+// streaming line adjoints: a table load inside their loops would wait for
+// every load in flight (the vector memory counter retires in order), and
+// the chunk after the current one is loaded ahead.  Every thread of the
+// block calls it (a barrier).
+struct AdjHead {
+  double u[kHead + 1], rd[kHead + 1];
+};
+__device__ __forceinline__ void adj_head_load(AdjHead &h, const ThomasAxis &ax) {
+  const int t = threadIdx.x;
+  if (t <= kHead && t < ax.M) {
+    h.u[t] = ax.u[t];
+    h.rd[t] = ax.rd[t];
+  }
+  __syncthreads();
+}
+// adj_down with the head rows from LDS
+__device__ __forceinline__ double adj_down_h(const AdjHead &h, const ThomasAxis &ax, int k,
+                                             double v, double tt, double &carry) {
+  v -= carry;
+  if (k == ax.M - 2) v = fma(-ax.p, tt, v);
+  if (k == ax.M - 3) v = fma(-ax.hi2, tt, v);
+  v *= k <= kHead ? h.rd[k] : ax.rdS;
+  carry = ax.c16 * v;
+  return v;
+}
+
+// v lines (lane = padded column), in place: coalesced rows of 64 columns,
+// a chunk of kLC rows at a time with the next chunk's loads in flight
 __global__ void __launch_bounds__(64)
 k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
+  __shared__ AdjHead hd;
+  adj_head_load(hd, ax);
   const Tile2 tl = xcd_tile2((S + 63) / 64);
   const int k = tl.x * 64 + threadIdx.x;
-  if (k >= S) return;
+  if (k >= S) return;  // no barriers below
   const int M = ax.M;
   double *col = cb + tl.j * cs + k;
-  double v[kLC];
-  double c0 = 0.0, prev = 0.0;
-  for (int m0 = 0; m0 < M; m0 += kLC) {
+  double v[kLC], nv[kLC];
+  auto load_up = [&](int m0) {
 #pragma unroll
     for (int i = 0; i < kLC; i++)
-      if (m0 + i < M) v[i] = col[(int64_t)(m0 + i) * S];
+      if (m0 + i < M) nv[i] = col[(int64_t)(m0 + i) * S];
+  };
+  auto load_down = [&](int m1) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++)
+      if (m1 - i >= 1) nv[i] = col[(int64_t)(m1 - i) * S];
+  };
+  double c0 = 0.0, prev = 0.0;
+  load_up(0);
+  for (int m0 = 0; m0 < M; m0 += kLC) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++) v[i] = nv[i];
+    if (m0 + kLC < M) load_up(m0 + kLC);
 #pragma unroll
     for (int i = 0; i < kLC; i++) {
       const int t = m0 + i;
@@ -177,7 +218,7 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
         if (t == 0) c0 = v[i];
         if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
         if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? ax.u[t - 1] : ax.uS), prev, v[i]);
+        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? hd.u[t - 1] : ax.uS), prev, v[i]);
         prev = v[i];
       }
     }
@@ -187,13 +228,14 @@ k_adj_v(double *__restrict__ cb, int S, int64_t cs, ThomasAxis ax) {
   }
   const double tt = prev * ax.rq;  // prev = c[M-1]
   double carry = 0.0;
+  load_down(M - 2);
   for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {  // elements m1, m1-1, ... down to max(1, m1-kLC+1)
 #pragma unroll
-    for (int i = 0; i < kLC; i++)
-      if (m1 - i >= 1) v[i] = col[(int64_t)(m1 - i) * S];
+    for (int i = 0; i < kLC; i++) v[i] = nv[i];
+    if (m1 - kLC >= 1) load_down(m1 - kLC);
 #pragma unroll
     for (int i = 0; i < kLC; i++)
-      if (m1 - i >= 1) v[i] = adj_down(ax, m1 - i, v[i], tt, carry);
+      if (m1 - i >= 1) v[i] = adj_down_h(hd, ax, m1 - i, v[i], tt, carry);
 #pragma unroll
     for (int i = 0; i < kLC; i++)
       if (m1 - i >= 1) col[(int64_t)(m1 - i) * S] = v[i];
@@ -306,26 +348,34 @@ k_adj_gcol_down(const double *__restrict__ g, double *__restrict__ fb, int nR, i
                 int64_t cs, ThomasAxis ax, const double *__restrict__ dMb,
                 const double *__restrict__ Ct1, const double *__restrict__ dU,
                 const double *__restrict__ dR) {
+  __shared__ AdjHead hd;
+  adj_head_load(hd, ax);
   const Tile2 tl = xcd_tile2((nR + 63) / 64);
   const int r = tl.x * 64 + threadIdx.x;
   const int j = tl.j;
-  if (r >= nR) return;
+  if (r >= nR) return;  // no barriers below
   const int M = ax.M;
   const double *col = g + j * cs + r;
   const double tt = col[(int64_t)(M - 1) * nR] * ax.rq;
   const double ct = dMb ? Ct1[j] : 0.0, w = dMb ? mom_w(r, nR, dR, dMb) : 0.0;
   double *out = fb + (int64_t)j * nU * nR + r;
   double carry = 0.0;
-  double v[kLC];
-  for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {
+  double v[kLC], nv[kLC];
+  auto load_down = [&](int m1) {  // the next chunk's rows, in flight during this one
 #pragma unroll
     for (int i = 0; i < kLC; i++)
-      if (m1 - i >= 1) v[i] = col[(int64_t)(m1 - i) * nR];
+      if (m1 - i >= 1) nv[i] = col[(int64_t)(m1 - i) * nR];
+  };
+  load_down(M - 2);
+  for (int m1 = M - 2; m1 >= 1; m1 -= kLC) {
+#pragma unroll
+    for (int i = 0; i < kLC; i++) v[i] = nv[i];
+    if (m1 - kLC >= 1) load_down(m1 - kLC);
 #pragma unroll
     for (int i = 0; i < kLC; i++) {
       const int m = m1 - i;
       if (m >= 1) {
-        v[i] = adj_down(ax, m, v[i], tt, carry);
+        v[i] = adj_down_h(hd, ax, m, v[i], tt, carry);
         out[(int64_t)(m - 1) * nR] = dMb ? v[i] + mom_cu(m - 1, nU, ct, dU) * w : v[i];
       }
     }
@@ -341,6 +391,8 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
         ThomasAxis ax) {
   constexpr int kP = kLC + 1;
   __shared__ double chunk[64 * kP];
+  __shared__ AdjHead hd;
+  adj_head_load(hd, ax);
   const int M = nR + 2;
   const int lane = threadIdx.x;
   const int j = blockIdx.y;
@@ -349,21 +401,22 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
   const int rsub = lane / kLC, ksub = lane % kLC;  // staging role: 4 rows x kLC columns per pass
   double *src = cb + (size_t)j * cs + (size_t)(u0 + 1) * M;
   double *dst = out + ((size_t)j * nU + u0) * nR;
-  double v[kLC];
-  // rows x columns [e0, e0 + kLC) of a (rows x width) array <-> LDS, lane-row view
-  auto gather = [&](const double *a, int width, int e0) {
+  double v[kLC], tn[kLC], tc[kLC];
+  // rows x columns [e0, e0 + kLC) of a (rows x width) array: the loads into
+  // tn (the chunk after the current one is loaded while this one is
+  // solved), then from tc through LDS into the lane-row view v
+  auto gload = [&](const double *a, int width, int e0) {
     const int e = e0 + ksub;
-    // every load of the chunk leaves before the first LDS write (the
-    // scheduler would otherwise issue a few, wait for them, and go on)
-    double t[kLC];
 #pragma unroll
     for (int q = 0; q < kLC; q++) {
       const int row = min(q * 4 + rsub, rows - 1);
-      t[q] = a[(size_t)row * width + min(e, width - 1)];
+      tn[q] = a[(size_t)row * width + min(e, width - 1)];
     }
-    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto gput = [&](int width, int e0) {
+    const int e = e0 + ksub;
 #pragma unroll
-    for (int q = 0; q < kLC; q++) chunk[(q * 4 + rsub) * kP + ksub] = e < width ? t[q] : 0.0;
+    for (int q = 0; q < kLC; q++) chunk[(q * 4 + rsub) * kP + ksub] = e < width ? tc[q] : 0.0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kLC; i++) v[i] = chunk[lane * kP + i];
@@ -382,8 +435,12 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
     __syncthreads();
   };
   double c0 = 0.0, prev = 0.0;
+  gload(src, M, 0);
   for (int m0 = 0; m0 < M; m0 += kLC) {
-    gather(src, M, m0);
+#pragma unroll
+    for (int q = 0; q < kLC; q++) tc[q] = tn[q];
+    if (m0 + kLC < M) gload(src, M, m0 + kLC);  // columns this chunk's stores do not touch
+    gput(M, m0);
 #pragma unroll
     for (int i = 0; i < kLC; i++) {
       const int t = m0 + i;
@@ -391,7 +448,7 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
         if (t == 0) c0 = v[i];
         if (t == 1) v[i] = fma(-ax.A01, c0, v[i]);
         if (t == 2) v[i] = fma(-ax.A02, c0, v[i]);
-        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? ax.u[t - 1] : ax.uS), prev, v[i]);
+        if (t >= 2) v[i] = fma(-(t - 1 <= kHead ? hd.u[t - 1] : ax.uS), prev, v[i]);
         prev = v[i];
       }
     }
@@ -400,11 +457,16 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
   const double tt = prev * ax.rq;
   double carry = 0.0;
   // descending over chunks of the output columns e = k - 1 (k = 1..M-2)
-  for (int e0 = ((nR - 1) / kLC) * kLC; e0 >= 0; e0 -= kLC) {
-    gather(src, M, e0 + 1);
+  const int eLast = ((nR - 1) / kLC) * kLC;
+  gload(src, M, eLast + 1);
+  for (int e0 = eLast; e0 >= 0; e0 -= kLC) {
+#pragma unroll
+    for (int q = 0; q < kLC; q++) tc[q] = tn[q];
+    if (e0 - kLC >= 0) gload(src, M, e0 - kLC + 1);
+    gput(M, e0 + 1);
 #pragma unroll
     for (int i = kLC - 1; i >= 0; i--)
-      if (e0 + i < nR) v[i] = adj_down(ax, e0 + i + 1, v[i], tt, carry);
+      if (e0 + i < nR) v[i] = adj_down_h(hd, ax, e0 + i + 1, v[i], tt, carry);
     scatter(dst, nR, e0);
   }
 }
