@@ -184,6 +184,18 @@ def test_streaming_kick_moment_bit_exact(monkeypatch, name):
     _bit_exact_run(cfg, 6, "streaming kick and moment")
 
 
+@pytest.mark.parametrize("pres,ures", [(96, [192, 8]), (192, [64, 6]), (128, [96, 8])])
+def test_register_lines_uneven_bit_exact(pres, ures):
+    """Register lines on grids whose axes differ (the r-direction kernels are
+    compiled per nR, the v-direction and kick kernels per nU): 97 x 193,
+    193 x 65 and 129 x 97 nodes, the kick, r- and v-solves all in VGPRs."""
+    from tests.configs import UDF_C1
+
+    cfg = dict(UDF_C1, **{"grid.Pres": pres, "grid.Ures": ures})
+    print(f"case udf {pres + 1} x {ures[0] + 1} x {ures[1]}", flush=True)
+    _bit_exact_run(cfg, 5, "uneven register lines")
+
+
 @pytest.mark.parametrize("name", ["udf_c1", "pldf"])
 def test_moment_counters_across_runs(name):
     """The one-launch moment's counters return to zero after every launch:

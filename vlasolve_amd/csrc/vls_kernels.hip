@@ -783,7 +783,7 @@ k_spline_r_stream(const double *__restrict__ f, double *__restrict__ c, int nR, 
 //   KICK = true : the nR columns of f (j, r); rhs_k = f[j][k-1][r],
 //                 d_k -> d[j][k][r] (slice stride cs, row stride nR): the
 //                 kick's column splines (see k_kick_line)
-template <int M, bool KICK, int S>
+template <int M, bool KICK>
 __global__ void __launch_bounds__(64)
 k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, int nJ, int64_t cs,
        int lzmask, ThomasAxis ax) {
@@ -802,16 +802,16 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
     if (!live) col = nR - 1;
     in = src + (size_t)j * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
     out = c + (size_t)j * cs + col;
-    is = S - 2;  // = nR, a compile-time stride
-    os = S - 2;
+    is = nR;
+    os = nR;
   } else {
     int col = bx * 64 + lane;
     live = col < nR + 2;
     if (!live) col = nR + 1;
     in = (src ? src : c) + (size_t)j * cs + col;
     out = c + (size_t)j * cs + col;
-    is = S;  // = nR + 2, a compile-time stride
-    os = S;
+    is = nR + 2;
+    os = nR + 2;
   }
   // the line streams in kLA rows ahead of the elimination, which runs while
   // it arrives (issued all at once, the loads would wait on each other)
@@ -858,7 +858,7 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
   if (live) out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
 }
 
-// The kick's column splines (k_line<M, true, S>: d_m of each column of f into
+// The kick's column splines (k_line<M, true>: d_m of each column of f into
 // c, slice stride cs, row stride nR) and the moment of the same f in one pass:
 // the moment needs the whole f before the kick can evaluate, but the column
 // splines do not need the moment, and both read every column of f.  Tiles of
@@ -869,13 +869,13 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 // sums over groups of 16 rows, ((G0 + G1) + G2) + G3 per block of 64 rows,
 // blocks left to right); moment_tail then forms dM, M(<r) and the
 // acceleration.  Same bits as k_moment_cols + k_moment_chunks +
-// k_moment_scan and k_line<M, true, S>.
-template <int M, int S>
+// k_moment_scan and k_line<M, true>.
+template <int M>
 __global__ void __launch_bounds__(64)
-k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nJ, int64_t cs,
-                    int lzmask, ThomasAxis ax, MomentArgs ma) {
-  constexpr int nU = M - 2, nR = S - 2;
-  constexpr int nX = (nR + 62) / 63;  // column tiles per slice
+k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR, int nJ,
+                    int64_t cs, int lzmask, ThomasAxis ax, MomentArgs ma) {
+  constexpr int nU = M - 2;
+  const int nX = (nR + 62) / 63;  // column tiles per slice
   constexpr int nW = (nU + 63) / 64;  // blocks of 64 rows of the moment
   const int lane = threadIdx.x;
   const int lz = lane & lzmask;
@@ -890,7 +890,7 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nJ
   constexpr int kLA = M - 2 < 48 ? M - 2 : 48;
   __shared__ HeadTab htab;
   __shared__ double dUs[nU];
-  __shared__ double sdm[nR];
+  extern __shared__ double sdm[];  // [nR], the scan of the moment's last block
   __shared__ int last;
   const HeadRegs hr = head_load(ax, lane, lz);
   constexpr int nUq = (nU + 63) / 64;
@@ -1015,12 +1015,13 @@ struct KickLineCfg {
   static constexpr int kYL = Mv > 160 ? 56 : 0;
 };
 
-template <int Mv, int S, bool KEEP>
+template <int Mv, bool KEEP>
 __global__ void __launch_bounds__(64)
 k_kick_line(double *f, const double *__restrict__ U_Val, const double *__restrict__ acc,
             double *__restrict__ gsc, double Ulo, double Uhi, double start_v, double hinv_v,
-            int forceScratch, int lzmask, ThomasAxis ax, double *__restrict__ dKeep, int64_t ds) {
-  constexpr int nU = Mv - 2, nR = S - 2;
+            int forceScratch, int lzmask, ThomasAxis ax, double *__restrict__ dKeep, int64_t ds,
+            int nR) {
+  constexpr int nU = Mv - 2;
   constexpr int kRing = KickLineCfg<Mv>::kRing, kYL = KickLineCfg<Mv>::kYL;
   // ring slot s holds row m with m % kRing == s; slots kRing..kRing+2 repeat
   // slots 0..2 so that the four rows n..n+3 of a point are read from slots
@@ -1910,24 +1911,24 @@ k_line_rows(const double *__restrict__ f, double *__restrict__ c, int nR, int nU
 // elimination.  A chunk whose window is taller (kicks that differ by more
 // than ~12 cells across 16 columns) reads d directly.  Same values, in the
 // same order, as k_kick_cols followed by k_line_rows.
-template <int M, int Mv>
+template <int M>
 struct RowsKickCfg {
   static constexpr int kYL = M > 160 ? 32 : 0;
 };
 
-template <int M, int Mv>
+template <int M>
 __global__ void __launch_bounds__(64)
 k_line_rows_kick(const double *__restrict__ d, double *__restrict__ c,
                  const double *__restrict__ U_Val, const double *__restrict__ acc, double Ulo,
                  double Uhi, double start_v, double hinv_v, int64_t cs, int lzmask,
-                 ThomasAxis ax) {
-  constexpr int nR = M - 2, nU = Mv - 2;
+                 ThomasAxis ax, int nU) {
+  constexpr int nR = M - 2;
   constexpr int kC = 16, kP = kC + 1;  // chunk width, LDS row pitch
   constexpr int nChIn = (nR + kC - 1) / kC, nChOut = (M + kC - 1) / kC;
   constexpr int kWin = 80, kWQ = kWin / 4;  // window rows, window loads per lane
   constexpr int kB = 8;                     // points per batch of window reads
-  constexpr int nT = (nU + 63) / 64;        // row tiles per slice
-  constexpr int kYL = RowsKickCfg<M, Mv>::kYL;
+  const int nT = (nU + 63) / 64;            // row tiles per slice
+  constexpr int kYL = RowsKickCfg<M>::kYL;
   __shared__ double chunk[64 * kP];
   __shared__ double win[kWin * kP];
   __shared__ double accs[nR];
@@ -2152,47 +2153,88 @@ k_line_rows_kick(const double *__restrict__ d, double *__restrict__ c,
   }
 }
 
-// Register line solvers compiled for fixed shapes: (nR, nU) pairs of the
-// standard grids.  Other shapes use the LDS-tiled kernels above.
+// Register line solvers compiled for fixed line lengths: the r-direction
+// kernels for the standard nR, the v-direction kernels (the drift's v-solve
+// and every kick kernel) for the standard nU; a run has register lines when
+// both of its axes are in the tables.  Other shapes use the LDS-tiled or
+// streaming kernels.
 using LineFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
 using RowsFn = void (*)(const double *, double *, int, int, int, int64_t, int, ThomasAxis);
 using KickFn = void (*)(double *, const double *, const double *, double *, double, double, double,
-                        double, int, int, ThomasAxis, double *, int64_t);
-using KickMomentFn = void (*)(const double *, double *, int, int64_t, int, ThomasAxis, MomentArgs);
+                        double, int, int, ThomasAxis, double *, int64_t, int);
+using KickMomentFn = void (*)(const double *, double *, int, int, int64_t, int, ThomasAxis,
+                              MomentArgs);
 using RowsKickFn = void (*)(const double *, double *, const double *, const double *, double, double,
-                            double, double, int64_t, int, ThomasAxis);
-struct LineShape {
-  int nR, nU;
+                            double, double, int64_t, int, ThomasAxis, int);
+struct RowShape {
+  int nR;
   RowsFn rows;
+  RowsKickFn rowsKick;
+};
+struct ColShape {
+  int nU;
   LineFn cols, kickCols;
   KickFn kick, kickKeep;
   KickMomentFn kickMoment;
-  RowsKickFn rowsKick;
 };
-#define VLS_LINE_SHAPE(R, U)                                                          \
-  {R, U, k_line_rows<(R) + 2>, k_line<(U) + 2, false, (R) + 2>,                      \
-   k_line<(U) + 2, true, (R) + 2>, k_kick_line<(U) + 2, (R) + 2, false>,             \
-   k_kick_line<(U) + 2, (R) + 2, true>, k_kick_solve_moment<(U) + 2, (R) + 2>,       \
-   k_line_rows_kick<(R) + 2, (U) + 2>}
-static const LineShape kLineShapes[] = {
-    VLS_LINE_SHAPE(17, 17),   VLS_LINE_SHAPE(25, 13),   VLS_LINE_SHAPE(33, 33),
-    VLS_LINE_SHAPE(49, 49),   VLS_LINE_SHAPE(65, 65),   VLS_LINE_SHAPE(129, 129),
-    VLS_LINE_SHAPE(201, 201),
+#define VLS_ROW_SHAPE(R) {R, k_line_rows<(R) + 2>, k_line_rows_kick<(R) + 2>}
+#define VLS_COL_SHAPE(U)                                                              \
+  {U, k_line<(U) + 2, false>, k_line<(U) + 2, true>, k_kick_line<(U) + 2, false>,    \
+   k_kick_line<(U) + 2, true>, k_kick_solve_moment<(U) + 2>}
+static const RowShape kRowShapes[] = {
+    VLS_ROW_SHAPE(17),  VLS_ROW_SHAPE(25),  VLS_ROW_SHAPE(33),  VLS_ROW_SHAPE(49),
+    VLS_ROW_SHAPE(65),  VLS_ROW_SHAPE(97),  VLS_ROW_SHAPE(129), VLS_ROW_SHAPE(193),
+    VLS_ROW_SHAPE(201),
 };
-#undef VLS_LINE_SHAPE
+static const ColShape kColShapes[] = {
+    VLS_COL_SHAPE(13),  VLS_COL_SHAPE(17),  VLS_COL_SHAPE(33),  VLS_COL_SHAPE(49),
+    VLS_COL_SHAPE(65),  VLS_COL_SHAPE(97),  VLS_COL_SHAPE(129), VLS_COL_SHAPE(193),
+    VLS_COL_SHAPE(201),
+};
+#undef VLS_ROW_SHAPE
+#undef VLS_COL_SHAPE
 
-static const LineShape *line_shape(const Run &r) {
-  if (r.lineMode == 0 || (r.lineMode < 0 && getenv("VLS_NO_REGISTER_LINES"))) return nullptr;
-  for (const LineShape &ls : kLineShapes)
-    if (ls.nR == r.nR && ls.nU == r.nU) return &ls;
+// the register kernels of one run's shape
+struct LineShape {
+  RowsFn rows;
+  RowsKickFn rowsKick;
+  LineFn cols, kickCols;
+  KickFn kick, kickKeep;
+  KickMomentFn kickMoment;
+};
+
+static const RowShape *row_shape(int nR) {
+  for (const RowShape &rs : kRowShapes)
+    if (rs.nR == nR) return &rs;
+  return nullptr;
+}
+static const ColShape *col_shape(int nU) {
+  for (const ColShape &cs : kColShapes)
+    if (cs.nU == nU) return &cs;
   return nullptr;
 }
 
+static const LineShape *line_shape(const Run &r) {
+  if (r.lineMode == 0 || (r.lineMode < 0 && getenv("VLS_NO_REGISTER_LINES"))) return nullptr;
+  const RowShape *rs = row_shape(r.nR);
+  const ColShape *cs = col_shape(r.nU);
+  if (!rs || !cs) return nullptr;
+  // one entry per (nR, nU) pair, made on first use (std::map keeps its
+  // elements in place)
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, LineShape> made;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = made.find({r.nR, r.nU});
+  if (it == made.end())
+    it = made.emplace(std::make_pair(r.nR, r.nU),
+                      LineShape{rs->rows, rs->rowsKick, cs->cols, cs->kickCols, cs->kick,
+                                cs->kickKeep, cs->kickMoment})
+             .first;
+  return &it->second;
+}
+
 bool kick_rsolve_shape(const Run &r) {
-  if (r.nS != 1) return false;
-  for (const LineShape &ls : kLineShapes)
-    if (ls.nR == r.nR && ls.nU == r.nU) return true;
-  return false;
+  return r.nS == 1 && row_shape(r.nR) && col_shape(r.nU);
 }
 
 // The drift's r-direction solves (rows of src -> padded rows of c) and its
@@ -2284,14 +2326,14 @@ void launch_kick_rsolve(Run &r, hipStream_t s, Timing *t) {
   const LineShape *ls = line_shape(r);
   const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt};
   time_begin(t, s, "kick_solve");
-  hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64), 0, s, r.f, r.c, r.nJ,
-                     r.cs, 0, r.tv, ma);
+  hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64),
+                     (size_t)r.nR * sizeof(double), s, r.f, r.c, r.nR, r.nJ, r.cs, 0, r.tv, ma);
   time_mark(t, s, "kick_solve");
   // f is not needed again before the drift rewrites it: its buffer takes the
   // kicked rows' r-solve (the working set stays f, c and the feet)
   time_begin(t, s, "kick_spline_r");
   hipLaunchKernelGGL(ls->rowsKick, dim3((r.nU + 63) / 64 * r.nJ), dim3(64), 0, s, r.c, r.f,
-                     r.U_Val, r.acc, r.Ulo, r.Uhi, r.start_v, r.hinv_v, r.cs, 0, r.tr);
+                     r.U_Val, r.acc, r.Ulo, r.Uhi, r.start_v, r.hinv_v, r.cs, 0, r.tr, r.nU);
   time_mark(t, s, "kick_spline_r");
 }
 
@@ -2309,7 +2351,8 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *d
     const dim3 grid((r.nR + 62) / 63 * r.nJ);
     time_begin(t, s, "kick_solve");
     if (const LineShape *ls = line_shape(r))
-      hipLaunchKernelGGL(ls->kickMoment, grid, dim3(64), 0, s, r.f, d, r.nJ, r.cs, 0, r.tv, ma);
+      hipLaunchKernelGGL(ls->kickMoment, grid, dim3(64), (size_t)r.nR * sizeof(double), s, r.f, d,
+                         r.nR, r.nJ, r.cs, 0, r.tv, ma);
     else
       hipLaunchKernelGGL(k_kick_vsolve_moment, grid, dim3(64), (size_t)(r.nU + r.nR) * sizeof(double),
                          s, r.f, d, r.nR, r.nU, r.nJ, r.cs, 0, r.tv, ma);
@@ -2329,7 +2372,7 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *d
     time_begin(t, s, "kick_line");
     hipLaunchKernelGGL(dKeep ? ls->kickKeep : ls->kick, dim3((r.nR + kKW - 1) / kKW * r.nJ),
                        dim3(64), 0, s, r.f, r.U_Val, r.acc, r.gsc, r.Ulo, r.Uhi, r.start_v,
-                       r.hinv_v, r.kickScratch ? 1 : 0, 0, r.tv, dKeep, r.cs);
+                       r.hinv_v, r.kickScratch ? 1 : 0, 0, r.tv, dKeep, r.cs, r.nR);
     time_mark(t, s, "kick_line");
     return;
   }
