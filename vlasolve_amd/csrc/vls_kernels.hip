@@ -1378,35 +1378,50 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
   moment_tail(ma, j, x, nR, nJ, sdm, &last);
 }
 
-// A 1D grid of nT blocks per slice in the order of xcd_tile: the blocks of a
-// slice, which read the same rows of d (each row serves four rows of
-// points), run on one XCD and share its L2.
+// The kick's evaluation, two points per thread (il and il + kBlock of a
+// block's 2 kBlock points of a slice), whose coefficient loads leave
+// together; a 1D grid of nT blocks per slice in the order of xcd_tile, so
+// that the blocks of a slice, which read the same rows of d (each row serves
+// four rows of points), run on one XCD and share its L2.  A point outside
+// the velocity range is 0 (it reads rows 0..3 of its column and does not use
+// them).
 __global__ void __launch_bounds__(kBlock)
 k_kick_cols(double *__restrict__ f, const double *__restrict__ d, const double *__restrict__ U_Val,
-            const double *__restrict__ acc, int nR, int nU, int64_t ds, double Ulo, double Uhi,
-            double start_v, double hinv_v) {
-  const int nT = (nU * nR + kBlock - 1) / kBlock;
+             const double *__restrict__ acc, int nR, int nU, int64_t ds, double Ulo, double Uhi,
+             double start_v, double hinv_v) {
+  const int nS = nU * nR;
+  const int nT = (nS + 2 * kBlock - 1) / (2 * kBlock);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int j = tile / nT;
-  const int il = (tile - j * nT) * kBlock + threadIdx.x;  // index inside the slice
-  if (il >= nU * nR) return;
-  const int u = il / nR;
-  const int r = il - u * nR;
-  const size_t i = (size_t)j * nU * nR + il;
-  const double newU = U_Val[u] + acc[r];
-  if ((newU <= Ulo) || (newU >= Uhi)) {
-    f[i] = 0;
-    return;
+  const int il0 = (tile - j * nT) * (2 * kBlock) + threadIdx.x;
+  if (il0 >= nS) return;
+  const int il1 = il0 + kBlock;
+  const bool has1 = il1 < nS;
+  const double *dj = d + (size_t)j * ds;
+  double dv[2][4], yy[2];
+  bool in[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int il = h == 0 ? il0 : (has1 ? il1 : il0);
+    const int u = il / nR;
+    const int r = il - u * nR;
+    const double newU = U_Val[u] + acc[r];
+    in[h] = !(newU <= Ulo) && !(newU >= Uhi);
+    double y = in[h] ? (newU - start_v) * hinv_v : 0.0;
+    int n;
+    split(y, n, y);
+    yy[h] = y;
+    // row n+3 is past the column (nU+2) only at y == 0, where its weight
+    // BV2(0) is 0: the last row is read instead
+    const double *dc = dj + r;
+    dv[h][0] = dc[(int64_t)n * nR];
+    dv[h][1] = dc[(int64_t)(n + 1) * nR];
+    dv[h][2] = dc[(int64_t)(n + 2) * nR];
+    dv[h][3] = dc[(int64_t)min(n + 3, nU + 1) * nR];
   }
-  double y = (newU - start_v) * hinv_v;
-  int n;
-  split(y, n, y);
-  const double *dc = d + (size_t)j * ds + r;
-  // row n+3 is past the column (nU+2) only at y == 0, where its weight
-  // BV2(0) is 0: the last row is read instead
-  const double d0 = dc[(int64_t)n * nR], d1 = dc[(int64_t)(n + 1) * nR];
-  const double d2 = dc[(int64_t)(n + 2) * nR], d3 = dc[(int64_t)min(n + 3, nU + 1) * nR];
-  f[i] = kick_value(y, d0, d1, d2, d3);
+  const size_t base = (size_t)j * nS;
+  f[base + il0] = in[0] ? kick_value(yy[0], dv[0][0], dv[0][1], dv[0][2], dv[0][3]) : 0.0;
+  if (has1) f[base + il1] = in[1] ? kick_value(yy[1], dv[1][0], dv[1][1], dv[1][2], dv[1][3]) : 0.0;
 }
 
 // ---------------------------------------------------------------- drift
@@ -1431,6 +1446,41 @@ __device__ __forceinline__ double eval_c(const double *d, int64_t st, double xr,
             BV1(1. - y) * (p1[-1] * b2xI + p1[0] * b1xI + p1[1] * b1x + p1[2] * b2x) +
             BV1(y) * (p2[-1] * b2xI + p2[0] * b1xI + p2[1] * b1x + p2[2] * b2x) +
             BV2(y) * (p3[-1] * b2xI + p3[0] * b1xI + p3[1] * b1x + p3[2] * b2x);
+  return result;
+}
+
+// eval_c in two parts, so that several points' loads can leave together:
+// the 4x4 coefficients and the offsets in the cell, then the same sum in the
+// same order as eval_c.
+struct CStencil {
+  double v[16];  // rows Ny-1..Ny+2, columns Nx-1..Nx+2 (v[4 a + b])
+  double x, y;
+};
+__device__ __forceinline__ void eval_c_fetch(const double *d, int64_t st, double xr, double xu,
+                                             CStencil &s) {
+  int Nx, Ny;
+  split(xr, Nx, s.x);
+  split(xu, Ny, s.y);
+  Nx++;
+  Ny++;
+  const double *p0 = d + Nx + (int64_t)(Ny - 1) * st;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) s.v[4 * a + b] = p0[(int64_t)a * st + b - 1];
+}
+__device__ __forceinline__ double eval_c_sum(const CStencil &s) {
+  const double x = s.x, y = s.y;
+  const double *v = s.v;
+  const double b2xI = BV2(1. - x);
+  const double b1xI = BV1(1. - x);
+  const double b2x = BV2(x);
+  const double b1x = BV1(x);
+  double result = 1.0 / 36.0;
+  result *= BV2(1. - y) * (v[0] * b2xI + v[1] * b1xI + v[2] * b1x + v[3] * b2x) +
+            BV1(1. - y) * (v[4] * b2xI + v[5] * b1xI + v[6] * b1x + v[7] * b2x) +
+            BV1(y) * (v[8] * b2xI + v[9] * b1xI + v[10] * b1x + v[11] * b2x) +
+            BV2(y) * (v[12] * b2xI + v[13] * b1xI + v[14] * b1x + v[15] * b2x);
   return result;
 }
 
@@ -1460,34 +1510,61 @@ struct ChainTables {
   int roll;      // turn the rings (the last species of the step)
 };
 
+// Two points per thread (i and i + kBlock of a block's 2 kBlock points):
+// both points' feet, then both stencils' loads, go out together, so that a
+// wave keeps twice the loads in flight (a stencil's reads wait on its foot's
+// read).  A point that is not evaluated fetches the stencil at (0, 0),
+// inside every slice, and does not use it.  Blocks are spread so that each
+// XCD (block id mod 8) walks a contiguous range of points: the 4x4 stencils
+// of neighbouring points share coefficient rows, which then stay in that
+// XCD's L2.
+__device__ __forceinline__ void drift_chain(double *__restrict__ f, const double *cj, int stride,
+                                            int64_t i, double kd, const ChainTables &ch) {
+  if (!(kd >= 0) || kd >= (double)ch.n) return;  // no delay line attached
+  const int64_t k = (int64_t)kd;
+  const int L = ch.len[k];
+  const int h = ch.head[k];
+  double *v = ch.vals + ch.off[k];
+  int back = h + L - 1;
+  if (back >= L) back -= L;
+  v[back] = ch.bout[k] ? 0.0 : eval_c(cj, stride, ch.bxr[k], ch.bxu[k]);
+  f[i] = v[h];
+  if (ch.roll) ch.head[k] = (h + 1 == L) ? 0 : h + 1;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_drift(double *__restrict__ f, const double *__restrict__ c, const double2 *__restrict__ feet,
-        int64_t nVal, int nSlice, int64_t cs, int stride, int perXcd, ChainTables ch) {
+         int64_t nVal, int nSlice, int64_t cs, int stride, int perXcd, ChainTables ch) {
   const int b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7) * perXcd + (b >> 3);  // XCD-contiguous block
-  const int64_t i = lb * kBlock + threadIdx.x;
-  if (i >= nVal) return;
-  const double2 ft = feet[i];
-  const double x = ft.x;
-  const int64_t j = i / nSlice;
-  const double *cj = c + j * cs;
-  if (x > kFootOut) {
-    f[i] = eval_c(cj, stride, x, ft.y);
-  } else if (x == kFootOut) {
-    f[i] = 0;
-  } else {
-    const double kd = ft.y;
-    if (!(kd >= 0) || kd >= (double)ch.n) return;  // no delay line attached
-    const int64_t k = (int64_t)kd;
-    const int L = ch.len[k];
-    const int h = ch.head[k];
-    double *v = ch.vals + ch.off[k];
-    int back = h + L - 1;
-    if (back >= L) back -= L;
-    v[back] = ch.bout[k] ? 0.0 : eval_c(cj, stride, ch.bxr[k], ch.bxu[k]);
-    f[i] = v[h];
-    if (ch.roll) ch.head[k] = (h + 1 == L) ? 0 : h + 1;
-  }
+  const int64_t i0 = lb * (2 * kBlock) + threadIdx.x, i1 = i0 + kBlock;
+  if (i0 >= nVal) return;
+  const bool has1 = i1 < nVal;
+  const double2 ft0 = feet[i0];
+  const double2 ft1 = feet[has1 ? i1 : i0];
+  const double *cj0 = c + (i0 / nSlice) * cs;
+  const double *cj1 = c + ((has1 ? i1 : i0) / nSlice) * cs;
+  const bool e0 = ft0.x > kFootOut, e1 = ft1.x > kFootOut;
+  CStencil s0, s1;
+  eval_c_fetch(cj0, stride, e0 ? ft0.x : 0.0, e0 ? ft0.y : 0.0, s0);
+  eval_c_fetch(cj1, stride, e1 ? ft1.x : 0.0, e1 ? ft1.y : 0.0, s1);
+  double r0 = eval_c_sum(s0), r1 = eval_c_sum(s1);
+  // both sums here, before the branches below: left to the branch that uses
+  // it, a point's stencil loads would wait for the other point's
+  asm volatile("" : "+v"(r0), "+v"(r1));
+  if (e0)
+    f[i0] = r0;
+  else if (ft0.x == kFootOut)
+    f[i0] = 0;
+  else
+    drift_chain(f, cj0, stride, i0, ft0.y, ch);
+  if (!has1) return;
+  if (e1)
+    f[i1] = r1;
+  else if (ft1.x == kFootOut)
+    f[i1] = 0;
+  else
+    drift_chain(f, cj1, stride, i1, ft1.y, ch);
 }
 
 // Attach delay line k to its point: xr = kFootChain, xu = k.
@@ -2574,18 +2651,18 @@ void launch_kick(Run &r, hipStream_t s, double *d) {
 }
 
 void launch_kick_eval(Run &r, hipStream_t s, const double *d) {
-  hipLaunchKernelGGL(k_kick_cols, dim3(blocks_for((int64_t)r.nU * r.nR) * r.nJ), dim3(kBlock), 0, s,
-                     r.f, d, r.U_Val, r.acc, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  const int nT = (r.nU * r.nR + 2 * kBlock - 1) / (2 * kBlock);
+  hipLaunchKernelGGL(k_kick_cols, dim3(nT * r.nJ), dim3(kBlock), 0, s, r.f, d, r.U_Val, r.acc,
+                     r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
 }
 
 void launch_drift(Run &r, hipStream_t s) {
-  const int64_t nb = (r.nVal + kBlock - 1) / kBlock;
-  const int perXcd = (int)((nb + 7) / 8);
   ChainTables ch{r.ch_off,  r.ch_len, r.ch_head,  r.ch_xr,
                  r.ch_xu,   r.ch_out, r.ch_vals,  r.nChains, r.rollChains ? 1 : 0};
-  hipLaunchKernelGGL(k_drift, dim3(8 * perXcd), dim3(kBlock), 0, s, r.f, r.c,
-                     (const double2 *)r.feet, r.nVal,
-                     r.nU * r.nR, r.cs, r.nR + 2, perXcd, ch);
+  const int64_t nb2 = (r.nVal + 2 * kBlock - 1) / (2 * kBlock);
+  const int per2 = (int)((nb2 + 7) / 8);
+  hipLaunchKernelGGL(k_drift, dim3(8 * per2), dim3(kBlock), 0, s, r.f, r.c,
+                     (const double2 *)r.feet, r.nVal, r.nU * r.nR, r.cs, r.nR + 2, per2, ch);
 }
 
 void mark_chains(Run &r, hipStream_t s) {
