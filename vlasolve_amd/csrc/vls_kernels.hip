@@ -360,19 +360,40 @@ __device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, 
   __syncthreads();
   if (!*last) return;
   const double gf0 = t < nR ? ma.gfac[t] : 0.0;
-  for (int r = t; r < nR; r += blockDim.x) {
-    constexpr int kB = 16;  // chunk sums in flight
-    double a = 0.0;
+  // the chunk sums of kFR of this thread's columns (r0, r0 + blockDim, ...)
+  // and kB chunks are in flight together: each round trip to memory is
+  // about a microsecond with the GPU otherwise idle, so the columns are not
+  // taken one after another; each column still adds its chunks in order
+  constexpr int kB = 16, kFR = 4;
+  const int bd = blockDim.x;
+  for (int r0 = t; r0 < nR; r0 += kFR * bd) {
+    double a[kFR];
+#pragma unroll
+    for (int i = 0; i < kFR; i++) a[i] = 0.0;
     for (int c0 = 0; c0 < nC; c0 += kB) {
-      double v[kB];
+      double v[kFR][kB];
+      // clamped columns and chunks are loaded, not added
 #pragma unroll
-      for (int k = 0; k < kB; k++) v[k] = coherent_load(&P[(size_t)min(c0 + k, nC - 1) * kJC * nR + r]);
+      for (int i = 0; i < kFR; i++) {
+        const int r = min(r0 + i * bd, nR - 1);
 #pragma unroll
-      for (int k = 0; k < kB; k++)
-        if (c0 + k < nC) a += v[k];
+        for (int k = 0; k < kB; k++)
+          v[i][k] = coherent_load(&P[(size_t)min(c0 + k, nC - 1) * kJC * nR + r]);
+      }
+#pragma unroll
+      for (int i = 0; i < kFR; i++)
+#pragma unroll
+        for (int k = 0; k < kB; k++)
+          if (c0 + k < nC) a[i] += v[i][k];
     }
-    ma.dM[r] = a;
-    sdm[r] = a;
+#pragma unroll
+    for (int i = 0; i < kFR; i++) {
+      const int r = r0 + i * bd;
+      if (r < nR) {
+        ma.dM[r] = a[i];
+        sdm[r] = a[i];
+      }
+    }
   }
   if (t == 0) __hip_atomic_store(&cnt[nC * nX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   scan_prefix_out(sdm, ma.gfac, ma.Mx, ma.acc, ma.mTot, nR, gf0);
