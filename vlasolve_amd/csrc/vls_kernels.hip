@@ -72,6 +72,17 @@ __device__ __forceinline__ void wait_stores() {
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
+// at most N of this wave's vector memory operations outstanding
+// (s_waitcnt vmcnt(N)): the wave's memory operations complete in the order
+// they were issued, so every one issued before the last N is acknowledged
+template <int N>
+__device__ __forceinline__ void wait_vm_upto() {
+  static_assert(N >= 0 && N < 64, "vmcnt holds 6 bits");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  // vmcnt bits 3:0 and 15:14; expcnt (6:4) and lgkmcnt (11:8) unconstrained
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // The partials P[j][r] of this block's slice j and column tile (returned:
 // the tile), written by wave 0; kCoherent: with system-scope stores, which
@@ -319,22 +330,32 @@ struct MomentArgs {
   unsigned *cnt;
 };
 
-__device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, int nR, int nJ,
-                                            double *sdm, int *last) {
+// The block's count in its (chunk, column tile) counter, issued once its
+// partials are acknowledged; the returned value o completes the chunk when
+// moment_chunk_done(o, ...) holds.  lz: a zero in a VGPR that the compiler
+// cannot see through, for callers that use o only much later (with an
+// address it knows to be uniform the compiler makes the count wave-wide and
+// waits for its result at once).
+__device__ __forceinline__ unsigned moment_count(const MomentArgs &ma, int j, int x, int nR,
+                                                 int lz = 0) {
+  const int nX = (nR + 62) / 63, c = j / kJC;
+  return __hip_atomic_fetch_add(&ma.cnt[c * nX + x + lz], 1u, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool moment_chunk_done(unsigned o, int j, int nJ) {
+  const int jb = j / kJC * kJC, je = min(nJ, jb + kJC);
+  return o == (unsigned)(je - jb - 1);
+}
+
+// The reduction after the block's count: *last (LDS) holds whether the
+// block completed its chunk, for every thread of the block.
+__device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x, int nR, int nJ,
+                                              double *sdm, int *last) {
   const int nX = (nR + 62) / 63, nC = (nJ + kJC - 1) / kJC;
   const int c = j / kJC, jb = c * kJC, je = min(nJ, jb + kJC);
   const int t = threadIdx.x;
   unsigned *cnt = ma.cnt;
   double *P = ma.P;
-  if (t < 64) {  // wave 0 wrote the partials
-    wait_stores();
-    if (t == 0) {
-      const unsigned o =
-          __hip_atomic_fetch_add(&cnt[c * nX + x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      *last = o == (unsigned)(je - jb - 1);
-    }
-  }
-  __syncthreads();
   if (!*last) return;  // block-uniform
   if (t < 64) {
     const int r = x * 63 + t;
@@ -397,6 +418,16 @@ __device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, 
   }
   if (t == 0) __hip_atomic_store(&cnt[nC * nX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   scan_prefix_out(sdm, ma.gfac, ma.Mx, ma.acc, ma.mTot, nR, gf0);
+}
+
+__device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, int nR, int nJ,
+                                            double *sdm, int *last) {
+  if (threadIdx.x < 64) {  // wave 0 wrote the partials
+    wait_stores();
+    if (threadIdx.x == 0) *last = moment_chunk_done(moment_count(ma, j, x, nR), j, nJ);
+  }
+  __syncthreads();
+  moment_finish(ma, j, x, nR, nJ, sdm, last);
 }
 
 // The three moment kernels in one launch (moment_cols_block, moment_tail).
@@ -885,7 +916,7 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 // splines do not need the moment, and both read every column of f.  Tiles of
 // 63 columns as in k_moment_cols: lane 63 loads (and solves) the first column
 // of the next tile, which lane 62's moment terms take by a lane shift, and
-// stores nothing.  The moment's terms are formed from the raw rows while
+// stores no partial.  The moment's terms are formed from the raw rows while
 // they stream in for the elimination, in the order of k_moment_cols (serial
 // sums over groups of 16 rows, ((G0 + G1) + G2) + G3 per block of 64 rows,
 // blocks left to right); moment_tail then forms dM, M(<r) and the
@@ -904,7 +935,7 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   const int j = tile / nX, x = tile - j * nX;
   const int colRaw = x * 63 + lane;
   const int col = colRaw < nR ? colRaw : nR - 1;
-  const bool owner = lane < 63 && colRaw < nR;     // stores d and the partial
+  const bool owner = lane < 63 && colRaw < nR;     // stores the partial
   const bool term = lane < 63 && colRaw < nR - 1;  // has moment terms
   const double *in = f + (size_t)j * nU * nR + col - nR;  // in[k * nR] = f[j][k-1][col]
   double *out = c + (size_t)j * cs + col;
@@ -984,21 +1015,54 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   double tt = 0.0 - ax.hi2 * y[M - 3];
   tt = tt - ax.p * y[M - 2];
   double cn = div_known(tt, ax.q, ax.rq);
-  if (owner) out[(M - 1) * nR] = cn;
+  // The partial goes out before the back substitution, and the block counts
+  // itself kCnt rows into it, once the partial's store is acknowledged: the
+  // count's round trip then overlaps the remaining rows instead of following
+  // the last one, and the block that completes its chunk starts the
+  // reduction as soon as its own rows are out.  cn, which ends the
+  // elimination chain, is pinned before the partial's store: the compiler
+  // would otherwise defer the whole chain past that store's branch, keeping
+  // every raw row live.
+  // Every lane stores its column: lane 63 and the lanes past the last column
+  // solve a column that another lane owns, bit for bit, and store the same
+  // values there (unmasked, the stores let the count's result be used
+  // without waiting for them).  The partial is the owner's only.
+  asm volatile("" : "+v"(cn));
+  if (owner) coherent_store(&ma.P[(size_t)j * nR + colRaw], term ? Pj : 0.0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  out[(M - 1) * nR] = cn;
+  __builtin_amdgcn_sched_barrier(0);
   const double uS = ax.uS + (double)lz;
+  constexpr int kCnt = M - 2 > 96 ? 64 : M - 2;  // rows stored before the count
+  unsigned o = 0;
 #pragma unroll
   for (int k = M - 2; k >= 1; k--) {
     double uv = uS;
     if (k <= kHead) uv = htab.u[k];
     y[k] = y[k] - uv * cn;
     cn = y[k];
-    if (owner) out[k * nR] = cn;
+    out[k * nR] = cn;
+    if (k == M - 1 - kCnt) {
+      // kCnt + 1 stores issued since the partial's: waiting until at most
+      // kCnt (or 62) are outstanding waits for the partial
+      __builtin_amdgcn_sched_barrier(0);
+      wait_vm_upto<(kCnt < 62 ? kCnt : 62)>();
+      if (lane == 0) {
+        int vz;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+        o = moment_count(ma, j, x, nR, vz);
+      }
+    }
+    // rows in order (moved across each other, they keep more of y live)
+    __builtin_amdgcn_sched_barrier(0);
   }
-  if (owner) {
-    out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
-    coherent_store(&ma.P[(size_t)j * nR + colRaw], term ? Pj : 0.0);
-  }
-  moment_tail(ma, j, x, nR, nJ, sdm, &last);
+  out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
+  if (lane == 0) last = moment_chunk_done(o, j, nJ);
+  // the block is one wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  moment_finish(ma, j, x, nR, nJ, sdm, &last);
 }
 
 // ---------------------------------------------------------------- fused kick
