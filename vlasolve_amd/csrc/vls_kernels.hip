@@ -215,33 +215,53 @@ k_moment_partial_sp(const double *__restrict__ f, const double *__restrict__ Ct1
 // (src/vlasovSolver21_spline.hxx:746).  The scan runs as one block and ends
 // in scan_prefix_out: one thread runs the serial prefix over r on registers,
 // kS entries at a time, then the block writes M(<r) and the acceleration.
-__device__ __forceinline__ void scan_prefix_out(double *sdm, const double *__restrict__ gfac,
+// The serial prefix of one batch of kS entries held in registers.
+template <int kS>
+__device__ __forceinline__ void scan_batch(double (&v)[kS], double &prev) {
+#pragma unroll
+  for (int k = 0; k < kS; k++) {
+    const double d = v[k];
+    v[k] = prev;
+    prev += d;
+  }
+}
+// sdm [nR] holds dM on entry and M(<r) on exit, sgf [nR] the factors
+// G/R^2 * dt; both in LDS, written by the block before the call (the
+// factors are loaded with the partials' sums, not in the write loop, where
+// each pass over r would wait for its own round trip to memory).
+__device__ __forceinline__ void scan_prefix_out(double *sdm, const double *sgf,
                                                 double *__restrict__ Mx, double *__restrict__ acc,
-                                                double *__restrict__ mTot, int nR, double gf0) {
+                                                double *__restrict__ mTot, int nR) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    // the next batch's LDS reads are issued before the current batch's adds
+    // batches alternate between two register sets, the next batch's LDS
+    // reads issued before the current batch's adds
     constexpr int kS = 16;
     double prev = 0.0;
-    double v[kS], nx[kS];
+    double A[kS], B[kS];
     const int nFull = nR / kS;
+    if (nFull > 0) {
 #pragma unroll
-    for (int k = 0; k < kS; k++) nx[k] = nFull > 0 ? sdm[k] : 0.0;
-    for (int b = 0; b < nFull; b++) {
+      for (int k = 0; k < kS; k++) A[k] = sdm[k];
+    }
+    for (int b = 0; b < nFull; b += 2) {
+      const bool hasB = b + 1 < nFull;
+      if (hasB) {
 #pragma unroll
-      for (int k = 0; k < kS; k++) v[k] = nx[k];
-      if (b + 1 < nFull) {
-#pragma unroll
-        for (int k = 0; k < kS; k++) nx[k] = sdm[(b + 1) * kS + k];
+        for (int k = 0; k < kS; k++) B[k] = sdm[(b + 1) * kS + k];
       }
+      scan_batch(A, prev);
 #pragma unroll
-      for (int k = 0; k < kS; k++) {
-        const double d = v[k];
-        v[k] = prev;
-        prev += d;
+      for (int k = 0; k < kS; k++) sdm[b * kS + k] = A[k];
+      if (hasB) {
+        if (b + 2 < nFull) {
+#pragma unroll
+          for (int k = 0; k < kS; k++) A[k] = sdm[(b + 2) * kS + k];
+        }
+        scan_batch(B, prev);
+#pragma unroll
+        for (int k = 0; k < kS; k++) sdm[(b + 1) * kS + k] = B[k];
       }
-#pragma unroll
-      for (int k = 0; k < kS; k++) sdm[b * kS + k] = v[k];
     }
     for (int r = nFull * kS; r < nR; r++) {
       const double d = sdm[r];
@@ -252,9 +272,9 @@ __device__ __forceinline__ void scan_prefix_out(double *sdm, const double *__res
   }
   __syncthreads();
   for (int r = threadIdx.x; r < nR; r += blockDim.x) {
-    double m = sdm[r];
+    const double m = sdm[r];
     Mx[r] = m;
-    acc[r] = m * (r == (int)threadIdx.x ? gf0 : gfac[r]);
+    acc[r] = m * sgf[r];
   }
 }
 
@@ -289,12 +309,13 @@ __global__ void __launch_bounds__(kScanThreads)
 k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, double *__restrict__ dM,
               double *__restrict__ Mx, double *__restrict__ acc, double *__restrict__ mTot, int nR,
               int nJ) {
-  extern __shared__ double sdm[];  // [nR]
+  extern __shared__ double sdm[];  // [nR] the scan, then [nR] the factors
+  double *sgf = sdm + nR;
   const int t = threadIdx.x;
   const int nC = (nJ + kJC - 1) / kJC;
-  const double gf0 = t < nR ? gfac[t] : 0.0;
   for (int r = t; r < nR; r += kScanThreads) {
     constexpr int kB = 16;  // chunk sums in flight
+    const double g = gfac[r];
     double a = 0.0;
     for (int c0 = 0; c0 < nC; c0 += kB) {
       double v[kB];
@@ -306,8 +327,9 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
     }
     dM[r] = a;
     sdm[r] = a;
+    sgf[r] = g;
   }
-  scan_prefix_out(sdm, gfac, Mx, acc, mTot, nR, gf0);
+  scan_prefix_out(sdm, sgf, Mx, acc, mTot, nR);
 }
 
 // The moment's reduction after the partials, in the launch that forms them,
@@ -323,7 +345,7 @@ k_moment_scan(const double *__restrict__ P, const double *__restrict__ gfac, dou
 // memory: written with coherent_store, acknowledged (wait_stores) before
 // the count, read with coherent_load after it; agent-scope fences would
 // instead write back and invalidate each XCD's whole L2 in every block.
-// Every thread of the block calls it; sdm holds nR doubles of LDS.
+// Every thread of the block calls it; sdm holds 2 nR doubles of LDS.
 struct MomentArgs {
   const double *Ct1, *dU, *dR, *gfac;
   double *P, *dM, *Mx, *acc, *mTot;
@@ -348,7 +370,11 @@ __device__ __forceinline__ bool moment_chunk_done(unsigned o, int j, int nJ) {
 }
 
 // The reduction after the block's count: *last (LDS) holds whether the
-// block completed its chunk, for every thread of the block.
+// block completed its chunk, for every thread of the block.  kFR: column
+// groups of the final sums in flight together (each thread holds 16 kFR
+// loaded values: wide for kernels whose main loops already need more
+// registers, 1 where the registers would cost occupancy).
+template <int kFR>
 __device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x, int nR, int nJ,
                                               double *sdm, int *last) {
   const int nX = (nR + 62) / 63, nC = (nJ + kJC - 1) / kJC;
@@ -380,20 +406,24 @@ __device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x
   }
   __syncthreads();
   if (!*last) return;
-  const double gf0 = t < nR ? ma.gfac[t] : 0.0;
   // the chunk sums of kFR of this thread's columns (r0, r0 + blockDim, ...)
-  // and kB chunks are in flight together: each round trip to memory is
-  // about a microsecond with the GPU otherwise idle, so the columns are not
-  // taken one after another; each column still adds its chunks in order
-  constexpr int kB = 16, kFR = 4;
+  // and kB chunks are in flight together, with the columns' factors
+  // G/R^2 * dt: each round trip to memory is about a microsecond with the
+  // GPU otherwise idle, so the columns are not taken one after another; each
+  // column still adds its chunks in order.  Clamped columns and chunks are
+  // loaded, not added (loads under conditions would each cost a branch).
+  double *sgf = sdm + nR;
+  constexpr int kB = 16;
   const int bd = blockDim.x;
   for (int r0 = t; r0 < nR; r0 += kFR * bd) {
-    double a[kFR];
+    double a[kFR], g[kFR];
 #pragma unroll
-    for (int i = 0; i < kFR; i++) a[i] = 0.0;
+    for (int i = 0; i < kFR; i++) {
+      a[i] = 0.0;
+      g[i] = ma.gfac[min(r0 + i * bd, nR - 1)];
+    }
     for (int c0 = 0; c0 < nC; c0 += kB) {
       double v[kFR][kB];
-      // clamped columns and chunks are loaded, not added
 #pragma unroll
       for (int i = 0; i < kFR; i++) {
         const int r = min(r0 + i * bd, nR - 1);
@@ -413,13 +443,15 @@ __device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x
       if (r < nR) {
         ma.dM[r] = a[i];
         sdm[r] = a[i];
+        sgf[r] = g[i];
       }
     }
   }
   if (t == 0) __hip_atomic_store(&cnt[nC * nX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  scan_prefix_out(sdm, ma.gfac, ma.Mx, ma.acc, ma.mTot, nR, gf0);
+  scan_prefix_out(sdm, sgf, ma.Mx, ma.acc, ma.mTot, nR);
 }
 
+template <int kFR>
 __device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, int nR, int nJ,
                                             double *sdm, int *last) {
   if (threadIdx.x < 64) {  // wave 0 wrote the partials
@@ -427,17 +459,17 @@ __device__ __forceinline__ void moment_tail(const MomentArgs &ma, int j, int x, 
     if (threadIdx.x == 0) *last = moment_chunk_done(moment_count(ma, j, x, nR), j, nJ);
   }
   __syncthreads();
-  moment_finish(ma, j, x, nR, nJ, sdm, last);
+  moment_finish<kFR>(ma, j, x, nR, nJ, sdm, last);
 }
 
 // The three moment kernels in one launch (moment_cols_block, moment_tail).
 __global__ void __launch_bounds__(256)
 k_moment_fused(const double *__restrict__ f, int nR, int nU, int nJ, MomentArgs ma) {
-  extern __shared__ double Wl[];  // [nW][64] partials, then [nR] for the scan
+  extern __shared__ double Wl[];  // [nW][64] partials, then [2 nR] for the scan
   __shared__ int last;
   int j;
   const int x = moment_cols_block<true>(f, ma.Ct1, ma.dU, ma.dR, ma.P, nR, nU, Wl, j);
-  moment_tail(ma, j, x, nR, nJ, Wl, &last);
+  moment_tail<1>(ma, j, x, nR, nJ, Wl, &last);
 }
 
 // ---------------------------------------------------------------- spline
@@ -942,7 +974,7 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   constexpr int kLA = M - 2 < 48 ? M - 2 : 48;
   __shared__ HeadTab htab;
   __shared__ double dUs[nU];
-  extern __shared__ double sdm[];  // [nR], the scan of the moment's last block
+  extern __shared__ double sdm[];  // [2 nR], the scan of the moment's last block
   __shared__ int last;
   const HeadRegs hr = head_load(ax, lane, lz);
   constexpr int nUq = (nU + 63) / 64;
@@ -1062,7 +1094,9 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  moment_finish(ma, j, x, nR, nJ, sdm, &last);
+  // four column groups of final sums in flight where the line's registers
+  // already exceed what two waves per SIMD could hold
+  moment_finish<(M > 128 ? 4 : 1)>(ma, j, x, nR, nJ, sdm, &last);
 }
 
 // ---------------------------------------------------------------- fused kick
@@ -1403,11 +1437,11 @@ __device__ __forceinline__ double kick_value(double y, double d0, double d1, dou
 // tile's first column for its values only; the moment's terms are taken as
 // the raw rows are used by the forward sweep, the group and block sums as
 // left folds (((G0 + G1) + G2) + G3 with the groups past the last row as 0),
-// then moment_tail.  Dynamic LDS: nU + nR doubles.
+// then moment_tail.  Dynamic LDS: nU + 2 nR doubles.
 __global__ void __launch_bounds__(64)
 k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int nR, int nU, int nJ,
                      int64_t ds, int lzmask, ThomasAxis ax, MomentArgs ma) {
-  extern __shared__ double mlds[];  // dU [nU], then the scan's [nR]
+  extern __shared__ double mlds[];  // dU [nU], then the scan's [2 nR]
   __shared__ int last;
   double *dUs = mlds, *sdm = mlds + nU;
   const int lane = threadIdx.x;
@@ -1460,7 +1494,7 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
         if (owner) dc[(int64_t)k * nR] = v;
       });
   if (owner) coherent_store(&ma.P[(size_t)j * nR + colRaw], term ? Pj : 0.0);
-  moment_tail(ma, j, x, nR, nJ, sdm, &last);
+  moment_tail<1>(ma, j, x, nR, nJ, sdm, &last);
 }
 
 // The kick's evaluation, two points per thread (il and il + kBlock of a
@@ -1869,7 +1903,7 @@ void launch_moment(Run &r, hipStream_t s) {
 void launch_moment_into(Run &r, const MomentBufs &b, hipStream_t s) {
   dim3 g(blocks_for(r.nR), r.nJ);
   if (r.nS == 1 && b.cnt && !getenv("VLS_MOMENT_3K")) {
-    const size_t lds = std::max<size_t>(64 * ((r.nU + 63) / 64), r.nR) * sizeof(double);
+    const size_t lds = std::max<size_t>(64 * ((r.nU + 63) / 64), 2 * r.nR) * sizeof(double);
     const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, b.P, b.dM, b.Mx, b.acc, b.mTot, b.cnt};
     hipLaunchKernelGGL(k_moment_fused, dim3((r.nR + 62) / 63 * r.nJ), dim3(256), lds, s, b.f,
                        r.nR, r.nU, r.nJ, ma);
@@ -1893,7 +1927,7 @@ void launch_moment_scan_into(Run &r, const MomentBufs &b, hipStream_t s) {
   const int nC = (r.nJ + kJC - 1) / kJC;
   hipLaunchKernelGGL(k_moment_chunks, dim3(nC, (r.nR + kChunkThreads - 1) / kChunkThreads),
                      dim3(kChunkThreads), 0, s, b.P, r.nR, r.nJ);
-  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kScanThreads), r.nR * sizeof(double), s, b.P,
+  hipLaunchKernelGGL(k_moment_scan, dim3(1), dim3(kScanThreads), 2 * r.nR * sizeof(double), s, b.P,
                      r.gfac, b.dM, b.Mx, b.acc, b.mTot, r.nR, r.nJ);
 }
 
@@ -2489,7 +2523,7 @@ void launch_kick_rsolve(Run &r, hipStream_t s, Timing *t) {
   const MomentArgs ma{r.Ct1, r.dU, r.dR, r.gfac, r.P, r.dM, r.Mx, r.acc, r.mTot, r.mCnt};
   time_begin(t, s, "kick_solve");
   hipLaunchKernelGGL(ls->kickMoment, dim3((r.nR + 62) / 63 * r.nJ), dim3(64),
-                     (size_t)r.nR * sizeof(double), s, r.f, r.c, r.nR, r.nJ, r.cs, 0, r.tv, ma);
+                     2 * (size_t)r.nR * sizeof(double), s, r.f, r.c, r.nR, r.nJ, r.cs, 0, r.tv, ma);
   time_mark(t, s, "kick_solve");
   // f is not needed again before the drift rewrites it: its buffer takes the
   // kicked rows' r-solve (the working set stays f, c and the feet)
@@ -2513,10 +2547,10 @@ void launch_spline_kick(Run &r, hipStream_t s, Timing *t, bool moment, double *d
     const dim3 grid((r.nR + 62) / 63 * r.nJ);
     time_begin(t, s, "kick_solve");
     if (const LineShape *ls = line_shape(r))
-      hipLaunchKernelGGL(ls->kickMoment, grid, dim3(64), (size_t)r.nR * sizeof(double), s, r.f, d,
+      hipLaunchKernelGGL(ls->kickMoment, grid, dim3(64), 2 * (size_t)r.nR * sizeof(double), s, r.f, d,
                          r.nR, r.nJ, r.cs, 0, r.tv, ma);
     else
-      hipLaunchKernelGGL(k_kick_vsolve_moment, grid, dim3(64), (size_t)(r.nU + r.nR) * sizeof(double),
+      hipLaunchKernelGGL(k_kick_vsolve_moment, grid, dim3(64), (size_t)(r.nU + 2 * r.nR) * sizeof(double),
                          s, r.f, d, r.nR, r.nU, r.nJ, r.cs, 0, r.tv, ma);
     time_mark(t, s, "kick_solve");
     time_begin(t, s, "kick_eval");
