@@ -1066,6 +1066,12 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   __builtin_amdgcn_sched_barrier(0);
   const double uS = ax.uS + (double)lz;
   constexpr int kCnt = M - 2 > 96 ? 64 : M - 2;  // rows stored before the count
+  // Rows stored before the count's result is used: on long columns the
+  // block that completes its chunk runs the moment's reduction there, with
+  // the rest of its column still in registers, while the other blocks' back
+  // substitutions (the kernel's store phase) go on; the reduction's chain of
+  // round trips then no longer follows the last row of the last block.
+  constexpr int kDec = M - 2 > 128 ? 112 : M - 2;
   unsigned o = 0;
 #pragma unroll
   for (int k = M - 2; k >= 1; k--) {
@@ -1085,18 +1091,21 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
         o = moment_count(ma, j, x, nR, vz);
       }
     }
+    if (k == M - 1 - kDec) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (lane == 0) last = moment_chunk_done(o, j, nJ);
+      // the block is one wave
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // four column groups of final sums in flight where the registers
+      // allow it (the line's first rows are still live here)
+      moment_finish<(M > 128 ? 4 : 1)>(ma, j, x, nR, nJ, sdm, &last);
+    }
     // rows in order (moved across each other, they keep more of y live)
     __builtin_amdgcn_sched_barrier(0);
   }
   out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
-  if (lane == 0) last = moment_chunk_done(o, j, nJ);
-  // the block is one wave
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // four column groups of final sums in flight where the line's registers
-  // already exceed what two waves per SIMD could hold
-  moment_finish<(M > 128 ? 4 : 1)>(ma, j, x, nR, nJ, sdm, &last);
 }
 
 // ---------------------------------------------------------------- fused kick
