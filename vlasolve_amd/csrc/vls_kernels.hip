@@ -1506,50 +1506,62 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
   moment_tail<1>(ma, j, x, nR, nJ, sdm, &last);
 }
 
-// The kick's evaluation, two points per thread (il and il + kBlock of a
-// block's 2 kBlock points of a slice), whose coefficient loads leave
-// together; a 1D grid of nT blocks per slice in the order of xcd_tile, so
-// that the blocks of a slice, which read the same rows of d (each row serves
-// four rows of points), run on one XCD and share its L2.  A point outside
-// the velocity range is 0 (it reads rows 0..3 of its column and does not use
-// them).
+// The kick's evaluation, kPts points per thread (il, il + kBlock, ... of a
+// block's kPts kBlock points of a slice), whose position loads and then
+// coefficient loads leave together: on large slices (513x513x256) the
+// coefficients come from HBM behind the dependent position loads, and four
+// points per thread keep twice the loads of two in flight at the same
+// occupancy (kick_eval 374 to 357 us there; at 201x201x200, where d stays
+// in the MALL, two points are 0.1 us faster).  A 1D grid of nT blocks per
+// slice in the order of xcd_tile, so that the blocks of a slice, which read
+// the same rows of d (each row serves four rows of points), run on one XCD
+// and share its L2.  A point outside the velocity range is 0 (it reads rows
+// 0..3 of its column and does not use them).
+
+template <int kPts>
 __global__ void __launch_bounds__(kBlock)
 k_kick_cols(double *__restrict__ f, const double *__restrict__ d, const double *__restrict__ U_Val,
              const double *__restrict__ acc, int nR, int nU, int64_t ds, double Ulo, double Uhi,
              double start_v, double hinv_v) {
   const int nS = nU * nR;
-  const int nT = (nS + 2 * kBlock - 1) / (2 * kBlock);
+  const int nT = (nS + kPts * kBlock - 1) / (kPts * kBlock);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int j = tile / nT;
-  const int il0 = (tile - j * nT) * (2 * kBlock) + threadIdx.x;
+  const int il0 = (tile - j * nT) * (kPts * kBlock) + threadIdx.x;
   if (il0 >= nS) return;
-  const int il1 = il0 + kBlock;
-  const bool has1 = il1 < nS;
   const double *dj = d + (size_t)j * ds;
-  double dv[2][4], yy[2];
-  bool in[2];
+  // points past the slice repeat the first one (evaluated, not stored)
+  int ils[kPts], rs[kPts];
+  double newU[kPts];
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int il = h == 0 ? il0 : (has1 ? il1 : il0);
-    const int u = il / nR;
-    const int r = il - u * nR;
-    const double newU = U_Val[u] + acc[r];
-    in[h] = !(newU <= Ulo) && !(newU >= Uhi);
-    double y = in[h] ? (newU - start_v) * hinv_v : 0.0;
+  for (int h = 0; h < kPts; h++) {
+    ils[h] = il0 + h * kBlock < nS ? il0 + h * kBlock : il0;
+    const int u = ils[h] / nR;
+    rs[h] = ils[h] - u * nR;
+    newU[h] = U_Val[u] + acc[rs[h]];
+  }
+  double dv[kPts][4], yy[kPts];
+  bool in[kPts];
+#pragma unroll
+  for (int h = 0; h < kPts; h++) {
+    in[h] = !(newU[h] <= Ulo) && !(newU[h] >= Uhi);
+    double y = in[h] ? (newU[h] - start_v) * hinv_v : 0.0;
     int n;
     split(y, n, y);
     yy[h] = y;
     // row n+3 is past the column (nU+2) only at y == 0, where its weight
     // BV2(0) is 0: the last row is read instead
-    const double *dc = dj + r;
+    const double *dc = dj + rs[h];
     dv[h][0] = dc[(int64_t)n * nR];
     dv[h][1] = dc[(int64_t)(n + 1) * nR];
     dv[h][2] = dc[(int64_t)(n + 2) * nR];
     dv[h][3] = dc[(int64_t)min(n + 3, nU + 1) * nR];
   }
   const size_t base = (size_t)j * nS;
-  f[base + il0] = in[0] ? kick_value(yy[0], dv[0][0], dv[0][1], dv[0][2], dv[0][3]) : 0.0;
-  if (has1) f[base + il1] = in[1] ? kick_value(yy[1], dv[1][0], dv[1][1], dv[1][2], dv[1][3]) : 0.0;
+#pragma unroll
+  for (int h = 0; h < kPts; h++)
+    if (h == 0 || il0 + h * kBlock < nS)
+      f[base + ils[h]] = in[h] ? kick_value(yy[h], dv[h][0], dv[h][1], dv[h][2], dv[h][3]) : 0.0;
 }
 
 // ---------------------------------------------------------------- drift
@@ -2779,9 +2791,16 @@ void launch_kick(Run &r, hipStream_t s, double *d) {
 }
 
 void launch_kick_eval(Run &r, hipStream_t s, const double *d) {
-  const int nT = (r.nU * r.nR + 2 * kBlock - 1) / (2 * kBlock);
-  hipLaunchKernelGGL(k_kick_cols, dim3(nT * r.nJ), dim3(kBlock), 0, s, r.f, d, r.U_Val, r.acc,
-                     r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  const int nS = r.nU * r.nR;
+  if (nS >= (1 << 17)) {
+    const int nT = (nS + 4 * kBlock - 1) / (4 * kBlock);
+    hipLaunchKernelGGL(k_kick_cols<4>, dim3(nT * r.nJ), dim3(kBlock), 0, s, r.f, d, r.U_Val,
+                       r.acc, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  } else {
+    const int nT = (nS + 2 * kBlock - 1) / (2 * kBlock);
+    hipLaunchKernelGGL(k_kick_cols<2>, dim3(nT * r.nJ), dim3(kBlock), 0, s, r.f, d, r.U_Val,
+                       r.acc, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  }
 }
 
 void launch_drift(Run &r, hipStream_t s) {
