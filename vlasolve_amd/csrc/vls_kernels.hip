@@ -987,9 +987,11 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
 #pragma unroll
   for (int k = 1; k <= kLA; k++) y[k] = in[k * nR];
   __builtin_amdgcn_sched_barrier(0);
+  // the table holds ct * dU[u], the first product of each term (the same
+  // rounding as forming it per row and lane)
 #pragma unroll
   for (int q = 0; q < nUq; q++)
-    if (lane + 64 * q < nU) dUs[lane + 64 * q] = duq[q];
+    if (lane + 64 * q < nU) dUs[lane + 64 * q] = ct * duq[q];
   head_store(htab, hr, lane);
   // the block is one wave
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1014,7 +1016,7 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
       if (u + 1 < nU) duNext = dUs[u + 1];
       const double avg = (((aOwn + aN) + b) + bN) * 0.25;
       if (u % 16 == 0) g = 0.0;
-      g += ((ct * du) * dr) * avg;
+      g += (du * dr) * avg;  // du = ct * dU[u]
       // the sum is formed here, in this row's step: left to the end of the
       // kernel, where it is used, it would keep every raw row live
       asm volatile("" : "+v"(g));
@@ -1464,8 +1466,9 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
   const bool term = lane < 63 && colRaw < nR - 1;
   const double *fc = f + (size_t)j * nU * nR + col;
   double *dc = d + (size_t)j * ds + col;
-  for (int u = lane; u < nU; u += 64) dUs[u] = ma.dU[u];
   const double ct = ma.Ct1[j], dr = term ? ma.dR[colRaw] : 0.0;
+  // ct * dU[u], the first product of each term
+  for (int u = lane; u < nU; u += 64) dUs[u] = ct * ma.dU[u];
   __syncthreads();
   const int nW = (nU + 63) / 64;
   double aOwn = 0.0, aN = 0.0, g = 0.0, W = 0.0, Pj = 0.0, duNext = dUs[0];
@@ -1478,7 +1481,7 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
       if (u + 1 < nU) duNext = dUs[u + 1];
       const double avg = (((aOwn + aN) + b) + bN) * 0.25;
       if (u % 16 == 0) g = 0.0;
-      g += ((ct * du) * dr) * avg;
+      g += (du * dr) * avg;  // du = ct * dU[u]
       const int q = (u / 16) % 4;
       if (u % 16 == 15 || u == nU - 2) W = q == 0 ? g : W + g;
       if (u % 64 == 63 || u == nU - 2) {
