@@ -504,6 +504,24 @@ static inline double moment_term(const orc_state *s, const double *f, long j, lo
   return Ct1 * dU * dR * avgS(s, f, (j * s->nU + u) * s->nR + r, ns);
 }
 
+/* the device's form of one species' term: ((Ct1 dU 0.25) dR) (sum of the
+ * four corners in get_avg's order), get_avg's factor 0.25 folded into the
+ * first product.  Scaling by 0.25 is exact, so this is the value of
+ * moment_term whenever the corners' sum times 0.25 is (sums above 2^-1020):
+ * both round the same product.  Several species keep the reference's form. */
+static inline double moment_term_dev(const orc_state *s, const double *f, long j, long u, long r,
+                                     double Ct1, int ns)
+{
+  if (ns > 1) return moment_term(s, f, j, u, r, Ct1, ns);
+  double dU = get_dU(s, u);
+  if (dU <= 0) return NAN;
+  double dR = get_dR(s, r);
+  if (dR <= 0) return NAN;
+  const long i = (j * s->nU + u) * s->nR + r;
+  const double sum = ((f[i] + f[i + 1]) + f[i + s->nR]) + f[i + s->nR + 1];
+  return ((Ct1 * dU * 0.25) * dR) * sum;
+}
+
 /* quadrature21::integrate1D with int21::f(0, ns) (quadrature21.hxx:88-142) */
 static void integrate1D(orc_state *s, const double *f, double *Mrv, int ns)
 {
@@ -556,7 +574,7 @@ static void integrate1D(orc_state *s, const double *f, double *Mrv, int ns)
           for (long t = 0; t < 16; t++) {
             long u = 64 * w + 16 * q + t;
             if (u >= nU - 1) continue;
-            double tval = moment_term(s, f, j, u, rr, Ct1, ns);
+            double tval = moment_term_dev(s, f, j, u, rr, Ct1, ns);
             if (tval == tval) acc += tval;
           }
           G[q] = acc;

@@ -134,8 +134,11 @@ __device__ __forceinline__ int moment_cols_block(const double *__restrict__ f,
           const double b = row[t + 1];  // row u + 1
           const double bN = lane_next(b);
           if (u < nU - 1) {
-            const double avg = (((aOwn + aN) + b) + bN) * 0.25;
-            acc += ((ct * dU[u]) * dr) * avg;
+            // the term ((ct dU 0.25) dR) (corner sum): get_avg's 0.25 in
+            // the first product (exact scaling: the value of
+            // ((ct dU) dR) (corner sum 0.25) for every sum above 2^-1020)
+            const double sum4 = ((aOwn + aN) + b) + bN;
+            acc += (((ct * dU[u]) * 0.25) * dr) * sum4;
           }
           aOwn = b;
           aN = bN;
@@ -987,11 +990,12 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
 #pragma unroll
   for (int k = 1; k <= kLA; k++) y[k] = in[k * nR];
   __builtin_amdgcn_sched_barrier(0);
-  // the table holds ct * dU[u], the first product of each term (the same
-  // rounding as forming it per row and lane)
+  // the table holds (ct dU[u]) 0.25, the first product of each term (the
+  // same rounding as forming it per row and lane; the moment's term is
+  // ((ct dU 0.25) dR) (corner sum), as in moment_cols_block)
 #pragma unroll
   for (int q = 0; q < nUq; q++)
-    if (lane + 64 * q < nU) dUs[lane + 64 * q] = ct * duq[q];
+    if (lane + 64 * q < nU) dUs[lane + 64 * q] = (ct * duq[q]) * 0.25;
   head_store(htab, hr, lane);
   // the block is one wave
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1014,9 +1018,9 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
       const int u = k - 2;
       const double du = duNext;
       if (u + 1 < nU) duNext = dUs[u + 1];
-      const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+      const double sum4 = ((aOwn + aN) + b) + bN;
       if (u % 16 == 0) g = 0.0;
-      g += (du * dr) * avg;  // du = ct * dU[u]
+      g += (du * dr) * sum4;  // du = (ct dU[u]) 0.25
       // the sum is formed here, in this row's step: left to the end of the
       // kernel, where it is used, it would keep every raw row live
       asm volatile("" : "+v"(g));
@@ -1467,8 +1471,8 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
   const double *fc = f + (size_t)j * nU * nR + col;
   double *dc = d + (size_t)j * ds + col;
   const double ct = ma.Ct1[j], dr = term ? ma.dR[colRaw] : 0.0;
-  // ct * dU[u], the first product of each term
-  for (int u = lane; u < nU; u += 64) dUs[u] = ct * ma.dU[u];
+  // (ct dU[u]) 0.25, the first product of each term
+  for (int u = lane; u < nU; u += 64) dUs[u] = (ct * ma.dU[u]) * 0.25;
   __syncthreads();
   const int nW = (nU + 63) / 64;
   double aOwn = 0.0, aN = 0.0, g = 0.0, W = 0.0, Pj = 0.0, duNext = dUs[0];
@@ -1479,9 +1483,9 @@ k_kick_vsolve_moment(const double *__restrict__ f, double *__restrict__ d, int n
       const int u = k - 2;
       const double du = duNext;
       if (u + 1 < nU) duNext = dUs[u + 1];
-      const double avg = (((aOwn + aN) + b) + bN) * 0.25;
+      const double sum4 = ((aOwn + aN) + b) + bN;
       if (u % 16 == 0) g = 0.0;
-      g += (du * dr) * avg;  // du = ct * dU[u]
+      g += (du * dr) * sum4;  // du = (ct dU[u]) 0.25
       const int q = (u / 16) % 4;
       if (u % 16 == 15 || u == nU - 2) W = q == 0 ? g : W + g;
       if (u % 64 == 63 || u == nU - 2) {
