@@ -372,14 +372,14 @@ __device__ __forceinline__ bool moment_chunk_done(unsigned o, int j, int nJ) {
   return o == (unsigned)(je - jb - 1);
 }
 
-// The reduction after the block's count: *last (LDS) holds whether the
-// block completed its chunk, for every thread of the block.  kFR: column
-// groups of the final sums in flight together (each thread holds 16 kFR
-// loaded values: wide for kernels whose main loops already need more
-// registers, 1 where the registers would cost occupancy).
-template <int kFR>
-__device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x, int nR, int nJ,
-                                              double *sdm, int *last) {
+// The reduction after the block's count, in two stages (moment_finish runs
+// both).  The chunk stage: if *last (LDS, for every thread of the block:
+// this block completed its chunk's count), the
+// chunk's sums S_c for the tile's columns and the count in the last
+// counter; *last then tells whether the final stage (moment_final) falls to
+// this block.
+__device__ __forceinline__ void moment_chunk(const MomentArgs &ma, int j, int x, int nR, int nJ,
+                                             int *last) {
   const int nX = (nR + 62) / 63, nC = (nJ + kJC - 1) / kJC;
   const int c = j / kJC, jb = c * kJC, je = min(nJ, jb + kJC);
   const int t = threadIdx.x;
@@ -408,7 +408,19 @@ __device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x
     }
   }
   __syncthreads();
-  if (!*last) return;
+}
+
+// The final stage, in the block that completed the last counter: dM[r] over
+// the chunks, the prefix and the acceleration.  kFR: column groups of the
+// final sums in flight together (each thread holds 16 kFR loaded values:
+// wide for kernels whose main loops already need more registers, 1 where
+// the registers would cost occupancy).
+template <int kFR>
+__device__ __forceinline__ void moment_final(const MomentArgs &ma, int nR, int nJ, double *sdm) {
+  const int nX = (nR + 62) / 63, nC = (nJ + kJC - 1) / kJC;
+  const int t = threadIdx.x;
+  unsigned *cnt = ma.cnt;
+  double *P = ma.P;
   // the chunk sums of kFR of this thread's columns (r0, r0 + blockDim, ...)
   // and kB chunks are in flight together, with the columns' factors
   // G/R^2 * dt: each round trip to memory is about a microsecond with the
@@ -452,6 +464,13 @@ __device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x
   }
   if (t == 0) __hip_atomic_store(&cnt[nC * nX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   scan_prefix_out(sdm, sgf, ma.Mx, ma.acc, ma.mTot, nR);
+}
+
+template <int kFR>
+__device__ __forceinline__ void moment_finish(const MomentArgs &ma, int j, int x, int nR, int nJ,
+                                              double *sdm, int *last) {
+  moment_chunk(ma, j, x, nR, nJ, last);
+  if (*last) moment_final<kFR>(ma, nR, nJ, sdm);  // block-uniform
 }
 
 template <int kFR>
@@ -1073,11 +1092,13 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
   const double uS = ax.uS + (double)lz;
   constexpr int kCnt = M - 2 > 96 ? 64 : M - 2;  // rows stored before the count
   // Rows stored before the count's result is used: on long columns the
-  // block that completes its chunk runs the moment's reduction there, with
-  // the rest of its column still in registers, while the other blocks' back
-  // substitutions (the kernel's store phase) go on; the reduction's chain of
-  // round trips then no longer follows the last row of the last block.
+  // block that completes its chunk forms the chunk's sums there, with the
+  // rest of its column still in registers, while the other blocks' back
+  // substitutions (the kernel's store phase) go on; that part of the
+  // reduction's chain of round trips then no longer follows the last row of
+  // the last block.
   constexpr int kDec = M - 2 > 128 ? 112 : M - 2;
+  constexpr bool kDeferFinal = M > 160;
   unsigned o = 0;
 #pragma unroll
   for (int k = M - 2; k >= 1; k--) {
@@ -1104,14 +1125,23 @@ k_kick_solve_moment(const double *__restrict__ f, double *__restrict__ c, int nR
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // four column groups of final sums in flight where the registers
-      // allow it (the line's first rows are still live here)
-      moment_finish<(M > 128 ? 4 : 1)>(ma, j, x, nR, nJ, sdm, &last);
+      // Four column groups of final sums in flight where the line's
+      // registers already exceed what two waves per SIMD could hold.  On
+      // the longest columns the final stage follows the block's own rows:
+      // run here, its loads would queue behind the other blocks' row stores
+      // (at 201x201x200 kick_solve 38.8 to 38.0 us; at 129 rows the
+      // remaining rows cost more than the queueing).
+      if constexpr (kDeferFinal)
+        moment_chunk(ma, j, x, nR, nJ, &last);
+      else
+        moment_finish<(M > 128 ? 4 : 1)>(ma, j, x, nR, nJ, sdm, &last);
     }
     // rows in order (moved across each other, they keep more of y live)
     __builtin_amdgcn_sched_barrier(0);
   }
   out[0] = (ax.A03 - ax.A01 * y[1]) - ax.A02 * y[2];
+  if constexpr (kDeferFinal)
+    if (last) moment_final<(M > 128 ? 4 : 1)>(ma, nR, nJ, sdm);  // block-uniform
 }
 
 // ---------------------------------------------------------------- fused kick
