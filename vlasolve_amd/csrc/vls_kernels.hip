@@ -973,7 +973,9 @@ k_line(const double *__restrict__ src, double *__restrict__ c, int nR, int nU, i
 // stores no partial.  The moment's terms are formed from the raw rows while
 // they stream in for the elimination, in the order of k_moment_cols (serial
 // sums over groups of 16 rows, ((G0 + G1) + G2) + G3 per block of 64 rows,
-// blocks left to right); moment_tail then forms dM, M(<r) and the
+// blocks left to right); the reduction's chunk and final stages
+// (moment_chunk, moment_final), in the blocks that complete its counters
+// during their back substitution, then form dM, M(<r) and the
 // acceleration.  Same bits as k_moment_cols + k_moment_chunks +
 // k_moment_scan and k_line<M, true>.
 template <int M>
