@@ -209,3 +209,29 @@ def test_quadrature_closed_forms_for_constant_f(orc):
 
 def test_renormalised_mass_is_exact(orc):
     assert orc.stats()["M"] == pytest.approx(1.0, rel=1e-14)
+
+
+def test_device_order_moment_matches_reference_order():
+    """The device order (nthreads=0: per-term ((Ct1 dU 0.25) dR) times the
+    corner sum, summed in groups of 16 rows and chunks of 20 slices) sums the
+    same terms as the reference's quadrature (nthreads=1), so dM agrees to
+    rounding; for constant f both give the closed form.  A dropped 0.25, a
+    wrong corner or a wrong weight would be off by far more."""
+    dev = O.Oracle(CFG, nthreads=0)
+    ref = O.Oracle(CFG, nthreads=1)
+    rng = np.random.default_rng(7)
+    f = rng.uniform(0.1, 2.0, dev.shape)
+    dev.set_f(f)
+    ref.set_f(f)
+    dMd, Mxd, td = dev.moments()
+    dMr, Mxr, tr = ref.moments()
+    scale = np.abs(dMr).max()
+    assert np.abs(dMd - dMr).max() <= 1e-13 * scale
+    assert td == pytest.approx(tr, rel=1e-13)
+    # constant f: dM[r] = 2 * 8 pi^2 (sum_j J dJ) (sum_u dU) dR[r]
+    R, U, J, JV = dev.axes()
+    dev.set_f(np.full(dev.shape, 2.0))
+    dMc, _, _ = dev.moments()
+    closed = 2.0 * 8 * math.pi ** 2 * (J * np.diff(JV)).sum() * np.diff(U).sum() * np.diff(R)
+    assert np.allclose(dMc[:-1], closed, rtol=1e-12)
+    assert dMc[-1] == 0
