@@ -472,8 +472,12 @@ k_adj_r(double *__restrict__ cb, double *__restrict__ out, int nR, int nU, int64
 }
 
 // scan^T: Mbar[r] = (sum_j abp[j][r]) * gfac[r] (or the given Mbar), then
-// dMbar[k] = sum_{r > k} Mbar[r].  One block.
-__global__ void __launch_bounds__(256)
+// dMbar[k] = sum_{r > k} Mbar[r].  One block of kAdjScanThreads: a thread
+// per radius (up to that many), the slices' terms kJ at a time in flight
+// (the block is alone on the GPU, so each batch is a round trip to memory),
+// each radius still adding them in slice order.
+constexpr int kAdjScanThreads = 1024;
+__global__ void __launch_bounds__(kAdjScanThreads)
 k_adj_scan(const double *__restrict__ abp, const double *__restrict__ MbarIn,
            const double *__restrict__ gfac, double *__restrict__ dMb, int nR, int nJ) {
   extern __shared__ double mb[];
@@ -482,7 +486,7 @@ k_adj_scan(const double *__restrict__ abp, const double *__restrict__ MbarIn,
     if (MbarIn) {
       v = MbarIn[r];
     } else {
-      constexpr int kJ = 16;
+      constexpr int kJ = 64;
       double s = 0.0;
       int j = 0;
       for (; j + kJ <= nJ; j += kJ) {
@@ -500,7 +504,25 @@ k_adj_scan(const double *__restrict__ abp, const double *__restrict__ MbarIn,
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
-    for (int k = nR - 1; k >= 0; k--) {
+    // descending, kS entries a batch: the batch's LDS reads leave together
+    // (one at a time, each would wait for the one before), then its adds in
+    // order
+    constexpr int kS = 16;
+    int k = nR;
+    for (; k >= kS; k -= kS) {
+      double v[kS];
+#pragma unroll
+      for (int q = 0; q < kS; q++) v[q] = mb[k - 1 - q];
+#pragma unroll
+      for (int q = 0; q < kS; q++) {
+        const double d = v[q];
+        v[q] = s;
+        s += d;
+      }
+#pragma unroll
+      for (int q = 0; q < kS; q++) mb[k - 1 - q] = v[q];
+    }
+    for (k = k - 1; k >= 0; k--) {
       const double v = mb[k];
       mb[k] = s;
       s += v;
@@ -955,8 +977,8 @@ void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipS
 }
 
 void launch_adj_scan(Run &r, const double *abp, const double *MbarIn, double *dMb, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_scan, dim3(1), dim3(256), r.nR * sizeof(double), s, abp, MbarIn,
-                     r.gfac, dMb, r.nR, r.nJ);
+  hipLaunchKernelGGL(k_adj_scan, dim3(1), dim3(kAdjScanThreads), r.nR * sizeof(double), s, abp,
+                     MbarIn, r.gfac, dMb, r.nR, r.nJ);
 }
 
 void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dMb, double *fb,
