@@ -8,6 +8,8 @@
 * Hernquist 513 x 513 x 256: the same identity over two steps (two
   segments), which runs the long-column adjoints (the column walk of the
   kick's transpose and the streaming line transposes) at 515 nodes.
+* UDF 201 x 201 x 200 (the step benchmark's grid): the same identity over
+  two steps, which runs the long-column adjoints at 203 nodes.
 * PLDF 129 x 129 x 64 for the full 2000 steps with checkpoints every 50:
   the gradient of L = sum_r (M_final(r) - M_target(r))^2, M_target the final
   mass profile of a run from f0 perturbed by a seeded multiplicative 1e-3,
@@ -70,6 +72,14 @@ def test_identity_hernquist_513x513x256(monkeypatch, keep):
     if keep is not None:
         monkeypatch.setenv("VLS_ADJ_KEEP_STEPS", keep)
     _identity(bench.HQ_WORKLOAD, 2, 1, 4)
+
+
+@pytest.mark.parametrize("keep", [None, "0"])
+def test_identity_udf_201x201x200(monkeypatch, keep):
+    print(f"case udf 201x201x200 adjoint identity, 2 steps, ck 1, keep={keep}", flush=True)
+    if keep is not None:
+        monkeypatch.setenv("VLS_ADJ_KEEP_STEPS", keep)
+    _identity(bench.WORKLOAD, 2, 1, 5)
 
 
 def _final_mass(ctx, f0, head0, slots0, n):
