@@ -72,25 +72,6 @@ __global__ void __launch_bounds__(kB) k_fill(double *__restrict__ p, int64_t n, 
   if (i < n) p[i] = v;
 }
 
-// Adjoint of evaluateC at spline coordinates (xr, xu): scatter lam times the
-// 16 weights into the coefficient adjoint of the slice.
-__device__ __forceinline__ void eval_c_adj(double *cb, int st, double xr, double xu, double lam) {
-  int nx, ny;
-  double x, y;
-  split(xr, nx, x);
-  split(xu, ny, y);
-  const double wx[4] = {BV2(1. - x), BV1(1. - x), BV1(x), BV2(x)};
-  const double wy[4] = {BV2(1. - y), BV1(1. - y), BV1(y), BV2(y)};
-  const double l36 = lam / 36.0;
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const double la = l36 * wy[a];
-    double *row = cb + (int64_t)(ny + a) * st + nx;
-#pragma unroll
-    for (int b = 0; b < 4; b++) atomicAdd(row + b, la * wx[b]);
-  }
-}
-
 // The transposed moment quadrature factors over the cell pairs around a
 // point: 1/4 (cu(u-1) + cu(u)) (w(r-1) + w(r)), with cu(u) = Ct1[j] dU[u] and
 // w(r) = dR[r] dMb[r] for the cells that exist (u <= nU-2, r <= nR-2) and 0
@@ -555,13 +536,17 @@ k_adj_moment(double *__restrict__ fb, const double *__restrict__ dMb,
 // coefficient belongs to one tile, so the global stores need no atomics,
 // and the table holds about 1.4 entries per point (16 for a list per
 // coefficient).  The points come in increasing order and one wave adds them,
-// a chunk of 64 at a time.
+// a chunk of 64 at a time.  After a tile's points come the delay lines whose
+// back cell's stencil touches it (entry kTileChain | k): their adjoint xb[k]
+// (k_adj_chain_rings) goes in at the back foot (bxr[k], bxu[k]) the same way.
 constexpr int kTile = 16;
+constexpr uint32_t kTileChain = 0x80000000u;
 __global__ void __launch_bounds__(64)
 k_adj_drift_tiles(const double *__restrict__ fb, double *__restrict__ cb,
                   const double2 *__restrict__ xy, const int64_t *__restrict__ toff,
-                  const uint32_t *__restrict__ tpts, int64_t nSlice, int64_t cs, int nR, int nU,
-                  int ntk) {
+                  const uint32_t *__restrict__ tpts, const double *__restrict__ xb,
+                  const double *__restrict__ bxr, const double *__restrict__ bxu,
+                  int64_t nSlice, int64_t cs, int nR, int nU, int ntk) {
   __shared__ double acc[kTile * kTile];
   const int tile = blockIdx.x, j = blockIdx.y;
   const int tm = tile / ntk, tk = tile - tm * ntk;
@@ -575,8 +560,16 @@ k_adj_drift_tiles(const double *__restrict__ fb, double *__restrict__ cb,
   const double2 *xyj = xy + j * nSlice;
   for (int64_t q = toff[t] + threadIdx.x; q < q1; q += 64) {
     const uint32_t p = tpts[q];
-    const double lam = fj[p] * (1.0 / 36.0);
-    const double2 ft = xyj[p];
+    double lam;
+    double2 ft;
+    if (p & kTileChain) {
+      const uint32_t k = p & ~kTileChain;
+      lam = xb[k] * (1.0 / 36.0);
+      ft = make_double2(bxr[k], bxu[k]);
+    } else {
+      lam = fj[p] * (1.0 / 36.0);
+      ft = xyj[p];
+    }
     int nx, ny;
     double x, y;
     split(ft.x, nx, x);
@@ -603,25 +596,26 @@ k_adj_drift_tiles(const double *__restrict__ fb, double *__restrict__ cb,
   }
 }
 
-// drift^T of the delay lines, one thread per chain (after the gather)
+// drift^T of the delay lines, one thread per chain: the point's adjoint goes
+// into the ring slot it read, and the slot written in the step hands its
+// adjoint (0 when the back cell is outside the grid) to xb[k], which
+// k_adj_drift_tiles adds into the back cell's stencil.
 __global__ void __launch_bounds__(kB)
-k_adj_chains(const double *__restrict__ fb, double *__restrict__ cb,
-             const int64_t *__restrict__ point, const int64_t *__restrict__ off,
-             const int32_t *__restrict__ len, const int32_t *__restrict__ head0, int64_t stepOff,
-             const double *__restrict__ bxr, const double *__restrict__ bxu,
-             const uint8_t *__restrict__ bout, double *__restrict__ ringb, int64_t nChains,
-             int64_t nSlice, int64_t cs, int st) {
+k_adj_chain_rings(const double *__restrict__ fb, const int64_t *__restrict__ point,
+                  const int64_t *__restrict__ off, const int32_t *__restrict__ len,
+                  const int32_t *__restrict__ head0, int64_t stepOff,
+                  const uint8_t *__restrict__ bout, double *__restrict__ ringb,
+                  double *__restrict__ xb, int64_t nChains) {
   const int64_t k = (int64_t)blockIdx.x * kB + threadIdx.x;
   if (k >= nChains) return;
-  const int64_t pt = point[k];
   const int L = len[k];
   const int h = (int)((head0[k] + stepOff) % L);
   const int back = (h + L - 1) % L;
   double *rb = ringb + off[k];
-  rb[h] += fb[pt];
-  const double xb = rb[back];
+  rb[h] += fb[point[k]];
+  const double x = rb[back];
   rb[back] = 0.0;
-  if (!bout[k] && xb != 0.0) eval_c_adj(cb + (pt / nSlice) * cs, st, bxr[k], bxu[k], xb);
+  xb[k] = bout[k] ? 0.0 : x;
 }
 
 // kick^T, acceleration part: abp[j][r] = hinv/6 sum_u fb*[j][u][r]
@@ -926,13 +920,13 @@ static const AdjShape *adj_shape(const Run &r) {
 void launch_adj_drift(Run &r, const double *fb, double *cb, double *ringb, const int32_t *head0,
                       int64_t stepOff, hipStream_t s) {
   const int ntm = (r.nU + 2 + kTile - 1) / kTile, ntk = (r.nR + 2 + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_adj_drift_tiles, dim3(ntm * ntk, r.nJ), dim3(64), 0, s, fb, cb,
-                     (const double2 *)r.feet, r.adj.tileOff, r.adj.tilePts,
-                     (int64_t)r.nU * r.nR, r.cs, r.nR, r.nU, ntk);
   if (r.nChains)
-    hipLaunchKernelGGL(k_adj_chains, dim3(nblocks(r.nChains)), dim3(kB), 0, s, fb, cb, r.ch_point,
-                       r.ch_off, r.ch_len, head0, stepOff, r.ch_xr, r.ch_xu, r.ch_out, ringb,
-                       r.nChains, (int64_t)r.nU * r.nR, r.cs, r.nR + 2);
+    hipLaunchKernelGGL(k_adj_chain_rings, dim3(nblocks(r.nChains)), dim3(kB), 0, s, fb,
+                       r.ch_point, r.ch_off, r.ch_len, head0, stepOff, r.ch_out, ringb,
+                       r.adj.chainXb, r.nChains);
+  hipLaunchKernelGGL(k_adj_drift_tiles, dim3(ntm * ntk, r.nJ), dim3(64), 0, s, fb, cb,
+                     (const double2 *)r.feet, r.adj.tileOff, r.adj.tilePts, r.adj.chainXb,
+                     r.ch_xr, r.ch_xu, (int64_t)r.nU * r.nR, r.cs, r.nR, r.nU, ntk);
 }
 
 void launch_adj_spline(Run &r, double *cb, double *out, hipStream_t s) {
@@ -993,30 +987,39 @@ void launch_adj_mass(Run &r, const double *abp, const double *MbarIn, double *dM
 // The drift's transposed foot table (built on the host from the folded foot
 // table): for every kTile x kTile tile of the padded slices, in tile order
 // (j, tile row, tile column), the slice-local indices of the points whose
-// 4x4 stencil touches the tile with a non-zero weight, in increasing order.
+// 4x4 stencil touches the tile with a non-zero weight, in increasing order,
+// then kTileChain | k for the delay lines k of the slice whose back cell's
+// stencil touches it (those with the back cell inside the grid).
 void build_drift_tiles(Run &r, const std::vector<double> &xr, const std::vector<double> &xu,
                        std::vector<int64_t> &off, std::vector<uint32_t> &pts) {
   const int64_t nSlice = (int64_t)r.nU * r.nR;
   const int S = r.nR + 2, Mv = r.nU + 2;
   const int ntm = (Mv + kTile - 1) / kTile, ntk = (S + kTile - 1) / kTile;
   const int64_t perSlice = (int64_t)ntm * ntk, nTiles = r.nJ * perSlice;
-  if (nSlice >= (int64_t)1 << 32) throw Error(VLS_ERR_ARG, "slice too large for the adjoint's tables");
+  if (nSlice >= (int64_t)kTileChain || r.nChains >= (int64_t)kTileChain)
+    throw Error(VLS_ERR_ARG, "slice or delay-line count too large for the adjoint's tables");
   off.assign(nTiles + 1, 0);
-  // the tiles of slice j that point p of the slice touches
+  // the delay lines of each slice, in increasing order
+  std::vector<std::vector<uint32_t>> chainsOf(r.nJ);
+  for (int64_t k = 0; k < r.nChains; k++)
+    if (!r.hch_out[k]) chainsOf[r.hch_point[k] / nSlice].push_back((uint32_t)k);
+  // the tiles of slice j that a stencil at (x, y) touches
+  auto tiles = [&](int64_t j, double x, double y, uint32_t e, auto &&fn) {
+    const double ix = std::trunc(x), iy = std::trunc(y);
+    const int nx = (int)ix, ny = (int)iy;
+    // last stencil row / column: weight BV2(0) = 0 at a fraction of 0
+    const int rowHi = std::min(ny + (y - iy == 0.0 ? 2 : 3), Mv - 1);
+    const int colHi = std::min(nx + (x - ix == 0.0 ? 2 : 3), S - 1);
+    for (int tm = ny / kTile; tm <= rowHi / kTile; tm++)
+      for (int tk = nx / kTile; tk <= colHi / kTile; tk++) fn((j * ntm + tm) * ntk + tk, e);
+  };
   auto each = [&](int64_t j, auto &&fn) {
     for (int64_t p = 0; p < nSlice; p++) {
       const int64_t i = j * nSlice + p;
-      const double x = xr[i];
-      if (x <= -1.0) continue;  // outside the grid or a delay line
-      const double ix = std::trunc(x), iy = std::trunc(xu[i]);
-      const int nx = (int)ix, ny = (int)iy;
-      // last stencil row / column: weight BV2(0) = 0 at a fraction of 0
-      const int rowHi = std::min(ny + (xu[i] - iy == 0.0 ? 2 : 3), Mv - 1);
-      const int colHi = std::min(nx + (x - ix == 0.0 ? 2 : 3), S - 1);
-      for (int tm = ny / kTile; tm <= rowHi / kTile; tm++)
-        for (int tk = nx / kTile; tk <= colHi / kTile; tk++)
-          fn((j * ntm + tm) * ntk + tk, (uint32_t)p);
+      if (xr[i] <= -1.0) continue;  // outside the grid or a delay line
+      tiles(j, xr[i], xu[i], (uint32_t)p, fn);
     }
+    for (uint32_t k : chainsOf[j]) tiles(j, r.hch_xr[k], r.hch_xu[k], kTileChain | k, fn);
   };
   // slices are independent: both passes run over the host's threads
   auto over_slices = [&](auto &&body) {

@@ -677,8 +677,13 @@ int vls_set_chains(vls_ctx *ctx, int run, int64_t nChains, const int64_t *point,
     dfree(r, r.ch_out);
     dfree(r, r.ch_vals);
     r.adj.recorded = false;
+    r.adj.tilesValid = false;  // the transposed foot table lists the back cells
     r.nChains = nChains;
     r.nCells = cells;
+    r.hch_point.assign(point, point + nChains);
+    r.hch_xr.assign(back_xr, back_xr + nChains);
+    r.hch_xu.assign(back_xu, back_xu + nChains);
+    r.hch_out.assign(back_out, back_out + nChains);
     r.ch_point = dalloc<int64_t>(r, nChains);
     r.ch_off = dalloc<int64_t>(r, nChains);
     r.ch_len = dalloc<int32_t>(r, nChains);
@@ -998,9 +1003,11 @@ void record_buffers(vls::Run &r, hipStream_t s, int nSteps, int ckEvery) {
     dfree(r, A.ringb);
     dfree(r, A.endVals);
     dfree(r, A.endHead);
+    dfree(r, A.chainXb);
     A.ringb = dalloc<double>(r, nc);
     A.endVals = dalloc<double>(r, nc);
     A.endHead = dalloc<int32_t>(r, nch);
+    A.chainXb = dalloc<double>(r, nch);
     A.cells = nc;
     A.chains = nch;
   }

@@ -123,10 +123,15 @@ struct Run {
     double *abp = nullptr, *dMb = nullptr, *Mbar = nullptr, *ringb = nullptr;
     int64_t *tileOff = nullptr;  // transposed foot table of the drift, per coefficient tile
     uint32_t *tilePts = nullptr;
+    double *chainXb = nullptr;   // [chains] adjoint of each delay line's back cell
     bool tilesValid = false;
     bool recorded = false;
   } adj;
   std::vector<double> hxr, hxu;  // host copies of the folded foot table
+  // host copies of the delay lines' points and back cells
+  std::vector<int64_t> hch_point;
+  std::vector<double> hch_xr, hch_xu;
+  std::vector<uint8_t> hch_out;
   std::vector<double> hU;        // host copy of U_Val
 
   bool have_f = false, have_drift = false, have_chains = false, stepped = false;
