@@ -620,18 +620,21 @@ k_adj_chain_rings(const double *__restrict__ fb, const int64_t *__restrict__ poi
 
 // kick^T, acceleration part: abp[j][r] = hinv/6 sum_u fb*[j][u][r]
 // sum_k B_k'(t) d_{n+k}, d the kept v-spline coefficients of column r.
-// A block takes 64 columns; its kRG waves take every kRG-th row u and add up
-// their rows, and the kRG partial sums are combined through LDS.
-constexpr int kRG = 16;
-__global__ void __launch_bounds__(64 * kRG)
+// A block takes kAccCols columns; its kAccGroups groups of kAccCols lanes
+// (two groups to a wave) take every kAccGroups-th row u and add up their
+// rows.  The two groups of a wave are combined by a lane swap, the waves'
+// sums through LDS.  Narrow column tiles give each lane fewer rows of
+// dependent loads and the grid more blocks.
+constexpr int kAccCols = 32, kAccGroups = 32;
+__global__ void __launch_bounds__(kAccCols * kAccGroups)
 k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
                  const double *__restrict__ U_Val, const double *__restrict__ acc,
                  double *__restrict__ abp, int nR, int nU, int64_t cs, double Ulo, double Uhi,
                  double start_v, double hinv_v) {
-  __shared__ double part[kRG][64];
-  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const Tile2 tl = xcd_tile2((nR + 63) / 64);
-  const int r = tl.x * 64 + lane;
+  __shared__ double part[kAccGroups / 2][kAccCols];
+  const int lane = threadIdx.x & (kAccCols - 1), g = threadIdx.x / kAccCols;
+  const Tile2 tl = xcd_tile2((nR + kAccCols - 1) / kAccCols);
+  const int r = tl.x * kAccCols + lane;
   const int j = tl.j;
   double s = 0.0;
   if (r < nR) {
@@ -641,7 +644,7 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
     // straight-line body (points outside the v range read row 0 and count
     // 0), two rows per trip so that their loads overlap
 #pragma unroll 2
-    for (int u = g; u < nU; u += kRG) {
+    for (int u = g; u < nU; u += kAccGroups) {
       const double lam = fj[(int64_t)u * nR];
       const double newU = U_Val[u] + a;
       const bool in = !(newU <= Ulo) && !(newU >= Uhi);
@@ -659,12 +662,13 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
       s += in ? lam * sk : 0.0;
     }
   }
-  part[g][lane] = s;
+  s += __shfl_xor(s, kAccCols);  // lanes l and l + 32: groups 2w and 2w + 1
+  if ((threadIdx.x & 63) < kAccCols) part[threadIdx.x >> 6][lane] = s;
   __syncthreads();
-  if (g == 0 && r < nR) {
+  if (threadIdx.x < kAccCols && r < nR) {
     double t = part[0][lane];
 #pragma unroll
-    for (int q = 1; q < kRG; q++) t += part[q][lane];
+    for (int q = 1; q < kAccGroups / 2; q++) t += part[q][lane];
     abp[(int64_t)j * nR + r] = t * (hinv_v / 6.0);
   }
 }
@@ -951,7 +955,8 @@ void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *
                        r.tv);
     return;
   }
-  hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + 63) / 64 * r.nJ), dim3(64 * kRG), 0, s, fsb, cK,
+  hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + kAccCols - 1) / kAccCols * r.nJ),
+                     dim3(kAccCols * kAccGroups), 0, s, fsb, cK,
                      r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
   // the fewest passes of at most kB threads over the row, split evenly
   const int passes = (r.nR + kB - 1) / kB;
