@@ -686,42 +686,43 @@ k_adj_kick_accel(const double *__restrict__ fsb, const double *__restrict__ c,
 // d_u the v node's offset from its knot plus rounding, below 0.01 of a cell
 // (checked when the run is recorded), so the points of row m satisfy
 // m - 3 - a_r hinv - 0.01 < u < m + 1 - a_r hinv + 0.01: the kW = 5 values of
-// u from floor(m - 3 - a_r hinv - 0.01) + 1 on.  One block per row (m, j).
+// u from floor(m - 3 - a_r hinv - 0.01) + 1 on.  One thread per (m, r).
 __global__ void __launch_bounds__(kB)
 k_adj_kick_coef(const double *__restrict__ fsb, const double *__restrict__ U_Val,
                 const double *__restrict__ acc, double *__restrict__ g, int nR, int nU,
                 int64_t cs, double Ulo, double Uhi, double start_v, double hinv_v) {
-  const int m = blockIdx.x, j = blockIdx.y;
+  // one thread per (m, r) of the slice's nU + 2 rows, r fastest
+  const int i = blockIdx.x * kB + threadIdx.x, j = blockIdx.y;
+  if (i >= (nU + 2) * nR) return;
+  const int m = i / nR, r = i - m * nR;
   const double *fj = fsb + (int64_t)j * nU * nR;
   double *gm = g + (int64_t)j * cs + (int64_t)m * nR;
-  for (int r = threadIdx.x; r < nR; r += blockDim.x) {
-    const double a = acc[r];
-    const double sh = a * hinv_v;
-    const int ulo = (int)floor((double)m - 3.0 - sh - 0.01) + 1;
-    constexpr int kW = 5;
-    double lam[kW], uu[kW];
+  const double a = acc[r];
+  const double sh = a * hinv_v;
+  const int ulo = (int)floor((double)m - 3.0 - sh - 0.01) + 1;
+  constexpr int kW = 5;
+  double lam[kW], uu[kW];
 #pragma unroll
-    for (int q = 0; q < kW; q++) {
-      const int u = min(max(ulo + q, 0), nU - 1);
-      lam[q] = fj[(int64_t)u * nR + r];
-      uu[q] = U_Val[u];
-    }
-    double acc6 = 0.0;
-#pragma unroll
-    for (int q = 0; q < kW; q++) {
-      const int u = ulo + q;
-      if (u < 0 || u >= nU) continue;
-      const double newU = uu[q] + a;
-      if ((newU <= Ulo) || (newU >= Uhi)) continue;
-      double t;
-      int n;
-      split((newU - start_v) * hinv_v, n, t);
-      const int k4 = m - n;
-      if (k4 < 0 || k4 > 3) continue;
-      acc6 += lam[q] * bweight(k4, t);
-    }
-    gm[r] = acc6 * (1.0 / 6.0);
+  for (int q = 0; q < kW; q++) {
+    const int u = min(max(ulo + q, 0), nU - 1);
+    lam[q] = fj[(int64_t)u * nR + r];
+    uu[q] = U_Val[u];
   }
+  double acc6 = 0.0;
+#pragma unroll
+  for (int q = 0; q < kW; q++) {
+    const int u = ulo + q;
+    if (u < 0 || u >= nU) continue;
+    const double newU = uu[q] + a;
+    if ((newU <= Ulo) || (newU >= Uhi)) continue;
+    double t;
+    int n;
+    split((newU - start_v) * hinv_v, n, t);
+    const int k4 = m - n;
+    if (k4 < 0 || k4 > 3) continue;
+    acc6 += lam[q] * bweight(k4, t);
+  }
+  gm[r] = acc6 * (1.0 / 6.0);
 }
 
 // Line adjoints in registers for a line length known at compile time
@@ -958,11 +959,9 @@ void launch_adj_kick(Run &r, const double *fsb, const double *cK, const double *
   hipLaunchKernelGGL(k_adj_kick_accel, dim3((r.nR + kAccCols - 1) / kAccCols * r.nJ),
                      dim3(kAccCols * kAccGroups), 0, s, fsb, cK,
                      r.U_Val, accK, abp, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
-  // the fewest passes of at most kB threads over the row, split evenly
-  const int passes = (r.nR + kB - 1) / kB;
-  const int bs = ((r.nR + passes - 1) / passes + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_adj_kick_coef, dim3(r.nU + 2, r.nJ), dim3(bs), 0, s, fsb, r.U_Val, accK, cb,
-                     r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v, r.hinv_v);
+  hipLaunchKernelGGL(k_adj_kick_coef, dim3(nblocks((int64_t)(r.nU + 2) * r.nR), r.nJ), dim3(kB), 0,
+                     s, fsb, r.U_Val, accK, cb, r.nR, r.nU, r.cs, r.Ulo, r.Uhi, r.start_v,
+                     r.hinv_v);
 }
 
 void launch_adj_kick_cols(Run &r, double *g, double *fb, const double *dMb, hipStream_t s) {
