@@ -9,7 +9,7 @@
 //             fb * (B-spline weight) into the tile in LDS, and the tile is
 //             stored; a delay-line point moves its fb into the ring slot it
 //             read, and the slot written in this step passes its adjoint to
-//             the back cell's stencil (global atomics, one chain per thread);
+//             the back cell's stencil (an entry of the same tile lists);
 //   spline^T  the transposed natural-spline solves, v lines then r rows, by
 //             reverse-mode differentiation of the Thomas elimination;
 //   kick^T    per point, the acceleration term fb* hinv/36 sum_k B_k'(t)
