@@ -2,6 +2,7 @@
 vls_adjoint, vls_set_chains): a recording made before the delay lines change
 is refused, and repeated recordings and delay-line uploads release the device
 memory they replace."""
+import numpy as np
 import pytest
 
 from tests.configs import UDF_TINY
@@ -45,6 +46,17 @@ def test_adjoint_after_chain_change_is_refused():
     assert n1 == n0  # the eight delay-line arrays were replaced, not added
     with pytest.raises(V.VlsError, match="status 3"):
         ctx.adjoint(dL_dM=[1.0] * ctx.info()["nR"])
-    # a new recording with the current delay lines is accepted
+    # a new recording with the current delay lines is accepted, and its
+    # transposed foot table (rebuilt without the delay lines' back cells)
+    # gives the gradient of a run whose first table was built without them
+    # (the same state: three steps with the delay lines, which the recording
+    # above leaves behind, then none)
     ctx.record(3, 2)
-    ctx.adjoint(dL_dM=[1.0] * ctx.info()["nR"])
+    g, _ = ctx.adjoint(dL_dM=[1.0] * ctx.info()["nR"])
+    fs = V.Solver(UDF_TINY)
+    fs.step(3)
+    fresh = fs.context
+    fresh.clear_chains()
+    fresh.record(3, 2)
+    g0, _ = fresh.adjoint(dL_dM=[1.0] * fresh.info()["nR"])
+    assert np.allclose(g, g0, rtol=1e-14, atol=1e-300)
