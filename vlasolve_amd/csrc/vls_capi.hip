@@ -168,8 +168,16 @@ void step_once(vls::Run &r, hipStream_t s, vls::Timing *t) {
 // Without advance the step stops after the kick (the state after it is not
 // needed: the last step of a recomputed segment).
 void step_keep(vls::Run &r, hipStream_t s, double *cK, double *accOut, bool advance = true) {
+  // the moment writes the acceleration straight into accOut (the step's
+  // kernels take r.acc when launched); r.acc is left as it was
+  struct AccSwap {
+    vls::Run &r;
+    double *own;
+    ~AccSwap() { r.acc = own; }
+  } swap{r, r.acc};
+  r.acc = accOut;
   vls::launch_spline_kick(r, s, nullptr, true, cK);
-  d2d(accOut, r.acc, r.nR, s);  // the moment's acceleration, which the kick only reads
+  r.acc = swap.own;
   if (!advance) return;
   vls::launch_spline(r, r.f, s, nullptr);
   vls::launch_drift(r, s);
@@ -1102,6 +1110,9 @@ void record_step(vls::Run &r, hipStream_t s, int n) {
 
 void record_end(vls::Run &r, hipStream_t s) {
   auto &A = r.adj;
+  // the acceleration of the last step, as after plain steps
+  if (A.keptFrom < A.nSteps)
+    d2d(r.acc, A.keepAcc + (size_t)r.nR * (A.nSteps - 1 - A.keptFrom), r.nR, s);
   d2d(A.endF, r.f, r.nVal, s);
   d2d(A.endVals, r.ch_vals, r.nCells, s);
   d2d(A.endHead, r.ch_head, r.nChains, s);
